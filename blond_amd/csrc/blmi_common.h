@@ -1,0 +1,158 @@
+// Shared device helpers of the MI355X tracking library (gfx950 only).
+//
+// Arithmetic contract: every kernel that must agree bit for bit with the
+// reference's strict C++ build evaluates its expressions in the same order
+// and with FMA contraction disabled (the translation units are compiled with
+// -ffp-contract=off).  FP64 division and square root are the IEEE correctly
+// rounded expansions the compiler emits without fast-math flags.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define BLMI_OK 0
+#define BLMI_ERR_ARG 1
+#define BLMI_ERR_HIP 2
+#define BLMI_ERR_UNSUPPORTED 3
+#define BLMI_ERR_ALLOC 4
+
+// Host-side error plumbing (defined in capi.hip).
+int blmi_set_error(int code, const char *fmt, ...);
+int blmi_check_launch(const char *what);
+
+#define BLMI_HIP_TRY(expr)                                                   \
+    do {                                                                     \
+        hipError_t _e = (expr);                                              \
+        if (_e != hipSuccess)                                                \
+            return blmi_set_error(BLMI_ERR_HIP, "%s: %s", #expr,             \
+                                  hipGetErrorString(_e));                    \
+    } while (0)
+
+namespace blmi {
+
+// Wavefront width of CDNA4.
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Sine and cosine by the Cephes method (S. L. Moshier, Cephes Math Library,
+// sin.c), arranged as in the VDT library the reference vendors
+// (blond/cpp_routines/sincos.h:72-160): even octant index j = (int(4|x|/pi)+1)&~1,
+// three-part Cody-Waite reduction r = ((|x| - j*DP1) - j*DP2) - j*DP3, the
+// Cephes degree-6 sine and cosine polynomials on r, then the octant picks the
+// polynomial and the sign.  Same constants and the same operation order as
+// the reference, so with contraction off the result is bit-identical to the
+// reference's vdt::fast_sin on x86-64.
+// ---------------------------------------------------------------------------
+struct CephesD {
+    static constexpr double kFourOverPi = 4.0 / 3.14159265358979323846;
+    static constexpr double kDP1 = 7.853981554508209228515625E-1;
+    static constexpr double kDP2 = 7.94662735614792836714E-9;
+    static constexpr double kDP3 = 3.06161699786838294307E-17;
+    static constexpr double kS0 = 1.58962301576546568060E-10;
+    static constexpr double kS1 = -2.50507477628578072866E-8;
+    static constexpr double kS2 = 2.75573136213857245213E-6;
+    static constexpr double kS3 = -1.98412698295895385996E-4;
+    static constexpr double kS4 = 8.33333333332211858878E-3;
+    static constexpr double kS5 = -1.66666666666666307295E-1;
+    static constexpr double kC0 = -1.13585365213876817300E-11;
+    static constexpr double kC1 = 2.08757008419747316778E-9;
+    static constexpr double kC2 = -2.75573141792967388112E-7;
+    static constexpr double kC3 = 2.48015872888517045348E-5;
+    static constexpr double kC4 = -1.38888888888730564116E-3;
+    static constexpr double kC5 = 4.16666666666665929218E-2;
+};
+
+// Reduced-argument polynomials: returns sin-branch and cos-branch values of r
+// and the octant index of |x|.
+__device__ __forceinline__ void cephes_reduce(double x, double &sp, double &cp,
+                                              int &oct) {
+    const double ax = fabs(x);
+    int j = (int)(CephesD::kFourOverPi * ax);
+    j = (j + 1) & ~1;
+    const double y = (double)j;
+    const double r = ((ax - y * CephesD::kDP1) - y * CephesD::kDP2) - y * CephesD::kDP3;
+    const double zz = r * r;
+    double ps = CephesD::kS0;
+    ps = ps * zz; ps = ps + CephesD::kS1;
+    ps = ps * zz; ps = ps + CephesD::kS2;
+    ps = ps * zz; ps = ps + CephesD::kS3;
+    ps = ps * zz; ps = ps + CephesD::kS4;
+    ps = ps * zz; ps = ps + CephesD::kS5;
+    double pc = CephesD::kC0;
+    pc = pc * zz; pc = pc + CephesD::kC1;
+    pc = pc * zz; pc = pc + CephesD::kC2;
+    pc = pc * zz; pc = pc + CephesD::kC3;
+    pc = pc * zz; pc = pc + CephesD::kC4;
+    pc = pc * zz; pc = pc + CephesD::kC5;
+    sp = r + r * zz * ps;
+    cp = 1.0 - zz * 0.5 + zz * zz * pc;
+    oct = j;
+}
+
+__device__ __forceinline__ double cephes_sin(double x) {
+    double sp, cp;
+    int j;
+    cephes_reduce(x, sp, cp, j);
+    double out = (((j - 2) & 2) == 0) ? cp : sp;
+    if (j & 4) out = -out;
+    if (x < 0.0) out = -out;
+    return out;
+}
+
+// Both functions from one reduction (the adjoint needs the cosine).
+__device__ __forceinline__ void cephes_sincos(double x, double &s, double &c) {
+    double sp, cp;
+    int j;
+    cephes_reduce(x, sp, cp, j);
+    const bool swap = ((j - 2) & 2) == 0;
+    s = swap ? cp : sp;
+    c = swap ? sp : cp;
+    if (j & 4) s = -s;
+    if (((j - 2) & 4) == 0) c = -c;
+    if (x < 0.0) s = -s;
+}
+
+// float version (sincos.h:166-233, Cephes sinf)
+__device__ __forceinline__ float cephes_sinf(float x) {
+    const float kFourOverPiF = (float)(4.0 / 3.14159265358979323846);
+    const float ax = fabsf(x);
+    int j = (int)(kFourOverPiF * ax);
+    j = (j + 1) & ~1;
+    const float y = (float)j;
+    const float r = ((ax - y * 0.78515625f) - y * 2.4187564849853515625e-4f) -
+                    y * 3.77489497744594108e-8f;
+    const float z = r * r;
+    const float s = (((-1.9515295891E-4f * z + 8.3321608736E-3f) * z -
+                      1.6666654611E-1f) * z * r) + r;
+    const float c = ((2.443315711809948E-005f * z - 1.388731625493765E-003f) * z +
+                     4.166664568298827E-002f) * z * z - 0.5f * z + 1.0f;
+    float out = (((j - 2) & 2) == 0) ? c : s;
+    if (j & 4) out = -out;
+    if (x < 0.0f) out = -out;
+    return out;
+}
+
+template <typename T> struct Trig;
+template <> struct Trig<double> {
+    __device__ __forceinline__ static double sin(double x) { return cephes_sin(x); }
+};
+template <> struct Trig<float> {
+    __device__ __forceinline__ static float sin(float x) { return cephes_sinf(x); }
+};
+
+// Agent-scope atomics (all XCDs of the device see one coherent value).
+__device__ __forceinline__ void atomic_add_agent(unsigned *p, unsigned v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_add_agent(unsigned long long *p,
+                                                 unsigned long long v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_add_agent(double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_add_agent(float *p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace blmi

@@ -1,0 +1,411 @@
+// C ABI of the MI355X tracking library (declared in include/blond_mi355x.h).
+//
+// Conventions: every array argument is a device pointer; lengths of particle
+// arrays are int64; every call takes the HIP stream to run on (NULL = the
+// default stream) and returns 0 on success or a BLMI_ERR_* code, with a
+// message available from blmi_last_error().  Calls are asynchronous with
+// respect to the host unless stated otherwise.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "blmi_common.h"
+#include "engine.h"
+#include "particles.h"
+#include "spectral.h"
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+
+int blmi_set_error(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int blmi_check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return blmi_set_error(BLMI_ERR_HIP, "%s launch failed: %s", what, hipGetErrorString(e));
+    return BLMI_OK;
+}
+
+namespace blmi {
+
+// ---------------------------------------------------------------------------
+// device properties and library workspace (per device)
+// ---------------------------------------------------------------------------
+struct DeviceInfo {
+    int cus = 256;
+    int max_lds = 65536;
+};
+static std::mutex g_mu;
+static std::map<int, DeviceInfo> g_info;
+
+static DeviceInfo &info() {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_info.find(dev);
+    if (it != g_info.end()) return it->second;
+    DeviceInfo d;
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+        d.cus = v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        v > 0)
+        d.max_lds = v;
+    return g_info[dev] = d;
+}
+
+int device_cu_count() { return info().cus; }
+int device_max_lds() { return info().max_lds; }
+
+struct Workspace {
+    void *ptr[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t size[4] = {0, 0, 0, 0};
+};
+static std::map<int, Workspace> g_ws;
+
+void *workspace(size_t bytes, int slot) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_mu);
+    Workspace &w = g_ws[dev];
+    if (w.size[slot] >= bytes && w.ptr[slot]) return w.ptr[slot];
+    if (w.ptr[slot]) (void)hipFree(w.ptr[slot]);  // synchronises the device
+    w.ptr[slot] = nullptr;
+    w.size[slot] = 0;
+    size_t want = bytes < 4096 ? 4096 : bytes;
+    if (hipMalloc(&w.ptr[slot], want) != hipSuccess) {
+        w.ptr[slot] = nullptr;
+        return nullptr;
+    }
+    w.size[slot] = want;
+    return w.ptr[slot];
+}
+
+}  // namespace blmi
+
+using namespace blmi;
+
+static inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+#define BLMI_REQUIRE(cond, ...)                                       \
+    do {                                                              \
+        if (!(cond)) return blmi_set_error(BLMI_ERR_ARG, __VA_ARGS__); \
+    } while (0)
+
+extern "C" {
+
+const char *blmi_last_error(void) { return g_err; }
+
+int blmi_version(void) { return 100; }
+
+int blmi_init(int device) {
+    BLMI_HIP_TRY(hipSetDevice(device));
+    (void)info();
+    return BLMI_OK;
+}
+
+int blmi_device_info(int *cus, int *max_lds) {
+    if (cus) *cus = device_cu_count();
+    if (max_lds) *max_lds = device_max_lds();
+    return BLMI_OK;
+}
+
+// ---------------------------------------------------------------- kick ----
+int blmi_kick(const double *dt, double *dE, int n_rf, double charge, const double *voltage,
+              const double *omega_rf, const double *phi_rf, int64_t n, double acc_kick,
+              void *stream) {
+    BLMI_REQUIRE(n >= 0 && n_rf >= 0, "kick: negative size");
+    BLMI_REQUIRE(n == 0 || (dt && dE), "kick: null particle array");
+    BLMI_REQUIRE(n_rf == 0 || (voltage && omega_rf && phi_rf), "kick: null RF array");
+    return launch_kick<double>(dt, dE, n, n_rf, charge, voltage, omega_rf, phi_rf, acc_kick,
+                               S(stream));
+}
+
+int blmi_kick_f32(const float *dt, float *dE, int n_rf, float charge, const float *voltage,
+                  const float *omega_rf, const float *phi_rf, int64_t n, float acc_kick,
+                  void *stream) {
+    BLMI_REQUIRE(n >= 0 && n_rf >= 0, "kick_f32: negative size");
+    BLMI_REQUIRE(n == 0 || (dt && dE), "kick_f32: null particle array");
+    return launch_kick<float>(dt, dE, n, n_rf, charge, voltage, omega_rf, phi_rf, acc_kick,
+                              S(stream));
+}
+
+// --------------------------------------------------------------- drift ----
+int blmi_drift(double *dt, const double *dE, int solver, double t_rev, double length_ratio,
+               double alpha_order, double eta_0, double eta_1, double eta_2, double alpha_0,
+               double alpha_1, double alpha_2, double beta, double energy, int64_t n,
+               void *stream) {
+    BLMI_REQUIRE(n >= 0, "drift: negative size");
+    BLMI_REQUIRE(solver >= 0 && solver <= 2, "drift: solver %d not in {0 simple, 1 legacy, 2 exact}",
+                 solver);
+    BLMI_REQUIRE(n == 0 || (dt && dE), "drift: null particle array");
+    DriftCoeffs<double> c = make_drift_coeffs<double>(solver, t_rev, length_ratio, alpha_order,
+                                                      eta_0, eta_1, eta_2, alpha_0, alpha_1,
+                                                      alpha_2, beta, energy);
+    return launch_drift<double>(dt, dE, n, c, S(stream));
+}
+
+int blmi_drift_f32(float *dt, const float *dE, int solver, float t_rev, float length_ratio,
+                   float alpha_order, float eta_0, float eta_1, float eta_2, float alpha_0,
+                   float alpha_1, float alpha_2, float beta, float energy, int64_t n,
+                   void *stream) {
+    BLMI_REQUIRE(n >= 0, "drift_f32: negative size");
+    BLMI_REQUIRE(solver >= 0 && solver <= 2, "drift_f32: bad solver %d", solver);
+    BLMI_REQUIRE(n == 0 || (dt && dE), "drift_f32: null particle array");
+    DriftCoeffs<float> c = make_drift_coeffs<float>(solver, t_rev, length_ratio, alpha_order,
+                                                    eta_0, eta_1, eta_2, alpha_0, alpha_1,
+                                                    alpha_2, beta, energy);
+    return launch_drift<float>(dt, dE, n, c, S(stream));
+}
+
+// ----------------------------------------------------------- histogram ----
+int blmi_histogram(const double *dt, double *profile, double cut_left, double cut_right,
+                   int n_slices, int64_t n, void *stream) {
+    BLMI_REQUIRE(n >= 0 && n_slices >= 0, "histogram: negative size");
+    BLMI_REQUIRE(n_slices == 0 || profile, "histogram: null profile");
+    BLMI_REQUIRE(n == 0 || dt, "histogram: null dt");
+    BLMI_REQUIRE(cut_right > cut_left, "histogram: cut_right must exceed cut_left");
+    return launch_histogram<double>(dt, n, profile, cut_left, cut_right, n_slices, S(stream));
+}
+
+int blmi_histogram_f32(const float *dt, float *profile, float cut_left, float cut_right,
+                       int n_slices, int64_t n, void *stream) {
+    BLMI_REQUIRE(n >= 0 && n_slices >= 0, "histogram_f32: negative size");
+    BLMI_REQUIRE(cut_right > cut_left, "histogram_f32: cut_right must exceed cut_left");
+    return launch_histogram<float>(dt, n, profile, cut_left, cut_right, n_slices, S(stream));
+}
+
+int blmi_smooth_histogram(const double *dt, double *profile, double cut_left, double cut_right,
+                          int n_slices, int64_t n, void *stream) {
+    BLMI_REQUIRE(n >= 0 && n_slices >= 0, "smooth_histogram: negative size");
+    BLMI_REQUIRE(n_slices == 0 || profile, "smooth_histogram: null profile");
+    BLMI_REQUIRE(cut_right > cut_left, "smooth_histogram: cut_right must exceed cut_left");
+    long long *scratch = (long long *)workspace(sizeof(long long) * (size_t)(n_slices + 1), 0);
+    if (!scratch) return blmi_set_error(BLMI_ERR_ALLOC, "smooth_histogram: workspace allocation");
+    return launch_smooth_histogram<double>(dt, n, profile, cut_left, cut_right, n_slices, scratch,
+                                           S(stream));
+}
+
+int blmi_smooth_histogram_f32(const float *dt, float *profile, float cut_left, float cut_right,
+                              int n_slices, int64_t n, void *stream) {
+    BLMI_REQUIRE(n >= 0 && n_slices >= 0, "smooth_histogram_f32: negative size");
+    BLMI_REQUIRE(cut_right > cut_left, "smooth_histogram_f32: cut_right must exceed cut_left");
+    long long *scratch = (long long *)workspace(sizeof(long long) * (size_t)(n_slices + 1), 0);
+    if (!scratch) return blmi_set_error(BLMI_ERR_ALLOC, "smooth_histogram_f32: workspace allocation");
+    return launch_smooth_histogram<float>(dt, n, profile, cut_left, cut_right, n_slices, scratch,
+                                          S(stream));
+}
+
+// --------------------------------------------------- linear_interp_kick ----
+int blmi_linear_interp_kick(const double *dt, double *dE, const double *voltage,
+                            const double *bin_centers, double charge, int n_slices, int64_t n,
+                            double acc_kick, void *stream) {
+    BLMI_REQUIRE(n >= 0, "linear_interp_kick: negative size");
+    BLMI_REQUIRE(n_slices >= 2, "linear_interp_kick: needs at least 2 slices (got %d)", n_slices);
+    BLMI_REQUIRE(voltage && bin_centers, "linear_interp_kick: null voltage/bin_centers");
+    double *tab = (double *)workspace(sizeof(double) * (2 * (size_t)n_slices + 2), 1);
+    if (!tab) return blmi_set_error(BLMI_ERR_ALLOC, "linear_interp_kick: workspace allocation");
+    double *vk = tab, *fac = tab + n_slices, *par = tab + 2 * n_slices;
+    int rc = launch_interp_table<double>(voltage, bin_centers, n_slices, charge, acc_kick, vk, fac,
+                                         par, S(stream));
+    if (rc) return rc;
+    return launch_interp_kick<double>(dt, dE, n, vk, fac, par, n_slices - 1, S(stream));
+}
+
+int blmi_linear_interp_kick_f32(const float *dt, float *dE, const float *voltage,
+                                const float *bin_centers, float charge, int n_slices, int64_t n,
+                                float acc_kick, void *stream) {
+    BLMI_REQUIRE(n >= 0, "linear_interp_kick_f32: negative size");
+    BLMI_REQUIRE(n_slices >= 2, "linear_interp_kick_f32: needs at least 2 slices");
+    float *tab = (float *)workspace(sizeof(float) * (2 * (size_t)n_slices + 2), 1);
+    if (!tab) return blmi_set_error(BLMI_ERR_ALLOC, "linear_interp_kick_f32: workspace allocation");
+    float *vk = tab, *fac = tab + n_slices, *par = tab + 2 * n_slices;
+    int rc = launch_interp_table<float>(voltage, bin_centers, n_slices, charge, acc_kick, vk, fac,
+                                        par, S(stream));
+    if (rc) return rc;
+    return launch_interp_kick<float>(dt, dE, n, vk, fac, par, n_slices - 1, S(stream));
+}
+
+// --------------------------------------------------------- rf_volt_comp ----
+int blmi_rf_volt_comp(const double *voltage, const double *omega_rf, const double *phi_rf,
+                      const double *bin_centers, int n_rf, int n_bins, double *rf_voltage,
+                      void *stream) {
+    BLMI_REQUIRE(n_rf >= 0 && n_bins >= 0, "rf_volt_comp: negative size");
+    return launch_rf_volt_comp<double>(voltage, omega_rf, phi_rf, bin_centers, n_rf, n_bins,
+                                       rf_voltage, S(stream));
+}
+
+int blmi_rf_volt_comp_f32(const float *voltage, const float *omega_rf, const float *phi_rf,
+                          const float *bin_centers, int n_rf, int n_bins, float *rf_voltage,
+                          void *stream) {
+    BLMI_REQUIRE(n_rf >= 0 && n_bins >= 0, "rf_volt_comp_f32: negative size");
+    return launch_rf_volt_comp<float>(voltage, omega_rf, phi_rf, bin_centers, n_rf, n_bins,
+                                      rf_voltage, S(stream));
+}
+
+// ------------------------------------------------------- fast_resonator ----
+int blmi_fast_resonator(double *re, double *im, const double *freq, const double *R_S,
+                        const double *Q, const double *f_r, int n_res, int n_freq, void *stream) {
+    BLMI_REQUIRE(n_res >= 0 && n_freq >= 0, "fast_resonator: negative size");
+    return launch_fast_resonator(re, im, freq, R_S, Q, f_r, n_res, n_freq, S(stream));
+}
+
+// ------------------------------------------------------------------ FFT ----
+int blmi_fft_max_lds_length(int64_t *n) {
+    *n = fft_lds_max_real();
+    return BLMI_OK;
+}
+
+int blmi_rfft(const double *in, int64_t n_in, double *out, int64_t n, void *stream) {
+    BLMI_REQUIRE(n_in >= 0 && n > 0, "rfft: bad sizes");
+    FftPlan p;
+    int rc = fft_plan_get(n, &p);
+    if (rc) return rc;
+    return launch_rfft(in, n_in, reinterpret_cast<double2 *>(out), p, S(stream));
+}
+
+int blmi_irfft(const double *in, int64_t m_in, double *out, int64_t n, void *stream) {
+    BLMI_REQUIRE(m_in >= 0 && n > 0, "irfft: bad sizes");
+    FftPlan p;
+    int rc = fft_plan_get(n, &p);
+    if (rc) return rc;
+    return launch_irfft(reinterpret_cast<const double2 *>(in), m_in, out, n, p, S(stream));
+}
+
+int blmi_induced_voltage(const double *profile, int n_slices, const double *Z, int64_t n_fft,
+                         double scale, double *volt, int n_out, void *stream) {
+    BLMI_REQUIRE(n_slices >= 1 && profile && Z && volt, "induced_voltage: bad arguments");
+    FftPlan p;
+    int rc = fft_plan_get(n_fft, &p);
+    if (rc) return rc;
+    InducedArgs a{};
+    a.profile = const_cast<double *>(profile);
+    a.n_slices = n_slices;
+    a.Z = reinterpret_cast<const double2 *>(Z);
+    a.scale = scale;
+    a.volt = volt;
+    a.n_out = n_out;
+    return launch_induced_voltage_lds(a, p, S(stream));
+}
+
+// --------------------------------------------------------------- engine ----
+void *blmi_engine_create(int64_t n, int n_rf, int n_turns, int solver) {
+    if (n < 0 || n_rf < 0 || n_turns < 1 || solver < 0 || solver > 2) {
+        blmi_set_error(BLMI_ERR_ARG, "engine_create: bad arguments");
+        return nullptr;
+    }
+    Engine *e = new Engine();
+    if (e->init(n, n_rf, n_turns, solver) != BLMI_OK) {
+        delete e;
+        return nullptr;
+    }
+    return e;
+}
+
+int blmi_engine_destroy(void *h) {
+    delete reinterpret_cast<Engine *>(h);
+    return BLMI_OK;
+}
+
+int blmi_engine_set_particles(void *h, double *dt, double *dE, int64_t n) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    BLMI_REQUIRE(n == e->n, "engine: particle count %lld differs from %lld", (long long)n,
+                 (long long)e->n);
+    BLMI_REQUIRE(((((uintptr_t)dt) | ((uintptr_t)dE)) & 15u) == 0,
+                 "engine: dt and dE must be 16-byte aligned");
+    e->dt = dt;
+    e->dE = dE;
+    return BLMI_OK;
+}
+
+int blmi_engine_set_programs(void *h, const double *rf, const double *acc, const double *t_rev,
+                             double length_ratio, double alpha_order, const double *eta0,
+                             const double *eta1, const double *eta2, const double *alpha0,
+                             const double *alpha1, const double *alpha2, const double *beta,
+                             const double *energy, double charge) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    e->charge = charge;
+    return e->set_programs(rf, acc, t_rev, length_ratio, alpha_order, eta0, eta1, eta2, alpha0,
+                           alpha1, alpha2, beta, energy);
+}
+
+int blmi_engine_set_slices(void *h, int n_slices, double cut_left, double cut_right,
+                           double *profile) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    BLMI_REQUIRE(n_slices >= 0 && (n_slices == 0 || profile), "engine: bad slices");
+    BLMI_REQUIRE(n_slices == 0 || cut_right > cut_left, "engine: cut_right must exceed cut_left");
+    return e->set_slices(n_slices, cut_left, cut_right, profile);
+}
+
+int blmi_engine_set_induced(void *h, int64_t n_fft, const double *Z_host, double scale,
+                            double *volt, const double *bin_centers) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    BLMI_REQUIRE(e->n_slices >= 2, "engine: set slices (>= 2) before the induced voltage");
+    BLMI_REQUIRE(volt && bin_centers && Z_host, "engine: null induced-voltage arrays");
+    return e->set_induced(n_fft, Z_host, scale, volt, bin_centers, e->charge);
+}
+
+int blmi_engine_set_launch(void *h, int block, int blocks_per_cu, int graph_chunk) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    BLMI_REQUIRE(block == 64 || block == 128 || block == 256 || block == 512,
+                 "engine: block size must be 64/128/256/512");
+    BLMI_REQUIRE(blocks_per_cu >= 1 && blocks_per_cu <= 64, "engine: blocks_per_cu in [1, 64]");
+    BLMI_REQUIRE(graph_chunk >= 1 && graph_chunk <= 1000, "engine: graph_chunk in [1, 1000]");
+    e->block_size = block;
+    e->blocks_per_cu = blocks_per_cu;
+    if (e->graph_chunk != graph_chunk) e->invalidate_graph();
+    e->graph_chunk = graph_chunk;
+    e->invalidate_graph();
+    return BLMI_OK;
+}
+
+int blmi_engine_set_turn(void *h, int turn) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    return e->set_turn(turn);
+}
+
+int blmi_engine_get_turn(void *h, int *turn) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e && turn, "engine: null handle");
+    return e->get_turn(turn);
+}
+
+int blmi_engine_prime(void *h, void *stream) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    return e->prime(S(stream));
+}
+
+int blmi_engine_track(void *h, int turns, int use_graph, void *stream) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    BLMI_REQUIRE(turns >= 0, "engine: negative turn count");
+    return e->track(turns, S(stream), use_graph != 0);
+}
+
+int blmi_engine_time_kernels(void *h, int turns, float *ms_induced, float *ms_turn,
+                             void *stream) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e && ms_induced && ms_turn, "engine: null argument");
+    return e->time_kernels(turns, S(stream), ms_induced, ms_turn);
+}
+
+}  // extern "C"

@@ -1,0 +1,507 @@
+// Slice-axis kernels: LDS Stockham FFT (radix 2/3/4/5/8), the real-FFT
+// split/merge, the one-turn induced voltage and the resonator impedance.
+//
+// A real transform of length n = 2M runs as a complex transform of length M
+// on z[m] = x[2m] + i x[2m+1] held in one workgroup's LDS (M <= 10240 complex
+// fits the 160 KiB of a CDNA4 workgroup).  Each Stockham stage reads all of
+// its butterfly inputs into registers, synchronises, and writes the outputs
+// back to the same LDS buffer, so one buffer suffices.  Twiddles come from an
+// exact table computed on the host in extended precision.
+#include <math.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "blmi_common.h"
+#include "particles.h"
+#include "spectral.h"
+
+namespace blmi {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+    return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 csub(double2 a, double2 b) {
+    return make_double2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) {
+    return make_double2(a.x * s, a.y * s);
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV> __device__ __forceinline__ double2 mul_mi(double2 a) {
+    return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+template <int R, bool INV> struct Dft;
+
+template <bool INV> struct Dft<2, INV> {
+    __device__ __forceinline__ static void run(double2 *v) {
+        const double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <bool INV> struct Dft<4, INV> {
+    __device__ __forceinline__ static void run(double2 *v) {
+        const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        const double2 t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+
+template <bool INV> struct Dft<8, INV> {
+    __device__ __forceinline__ static void run(double2 *v) {
+        double2 e[4] = {v[0], v[2], v[4], v[6]};
+        double2 o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, INV>::run(e);
+        Dft<4, INV>::run(o);
+        const double h = 0.70710678118654752440;  // sqrt(1/2)
+        // w8^k o_k, w8 = exp(-+ 2 pi i / 8)
+        const double2 o1 = INV ? make_double2(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
+                               : make_double2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+        const double2 o2 = mul_mi<INV>(o[2]);
+        const double2 o3 = INV ? make_double2(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
+                               : make_double2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
+template <bool INV> struct Dft<3, INV> {
+    __device__ __forceinline__ static void run(double2 *v) {
+        const double s3 = 0.86602540378443864676;  // sqrt(3)/2
+        const double2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+        const double2 m = csub(v[0], cscale(s, 0.5));
+        const double2 r = mul_mi<INV>(cscale(d, s3));
+        v[0] = cadd(v[0], s);
+        v[1] = cadd(m, r);
+        v[2] = csub(m, r);
+    }
+};
+
+template <bool INV> struct Dft<5, INV> {
+    __device__ __forceinline__ static void run(double2 *v) {
+        const double c1 = 0.30901699437494742410;   // cos(2pi/5)
+        const double c2 = -0.80901699437494742410;  // cos(4pi/5)
+        const double s1 = 0.95105651629515357212;   // sin(2pi/5)
+        const double s2 = 0.58778525229247312917;   // sin(4pi/5)
+        const double2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        const double2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        const double2 x0 = v[0];
+        const double2 p1 = cadd(x0, cadd(cscale(a1, c1), cscale(a2, c2)));
+        const double2 p2 = cadd(x0, cadd(cscale(a1, c2), cscale(a2, c1)));
+        const double2 q1 = mul_mi<INV>(cadd(cscale(b1, s1), cscale(b2, s2)));
+        const double2 q2 = mul_mi<INV>(csub(cscale(b1, s2), cscale(b2, s1)));
+        v[0] = cadd(x0, cadd(a1, a2));
+        v[1] = cadd(p1, q1);
+        v[4] = csub(p1, q1);
+        v[2] = cadd(p2, q2);
+        v[3] = csub(p2, q2);
+    }
+};
+
+// One Stockham stage: Ns = product of the radices already applied.
+template <int R, bool INV>
+__device__ __forceinline__ void stockham_stage(double2 *buf, int M, int Ns,
+                                               const double2 *__restrict__ tw) {
+    constexpr int BMAX = (16 / R) > 0 ? (16 / R) : 1;
+    const int nb = M / R;
+    const int T = blockDim.x;
+    const int tstride = M / (Ns * R);
+    double2 v[BMAX][R];
+#pragma unroll
+    for (int q = 0; q < BMAX; ++q) {
+        const int j = threadIdx.x + q * T;
+        if (j < nb) {
+            const int k = j % Ns;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double2 x = buf[j + r * nb];
+                if (r > 0) {
+                    double2 w = tw[r * k * tstride];
+                    if (INV) w.y = -w.y;
+                    x = cmul(x, w);
+                }
+                v[q][r] = x;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BMAX; ++q) {
+        const int j = threadIdx.x + q * T;
+        if (j < nb) {
+            Dft<R, INV>::run(v[q]);
+            const int k = j % Ns;
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+struct StageList {
+    int n;
+    int radix[kMaxStages];
+};
+
+template <bool INV>
+__device__ void fft_lds(double2 *buf, int M, const StageList &st, const double2 *__restrict__ tw) {
+    int Ns = 1;
+    for (int s = 0; s < st.n; ++s) {
+        switch (st.radix[s]) {
+            case 8: stockham_stage<8, INV>(buf, M, Ns, tw); break;
+            case 4: stockham_stage<4, INV>(buf, M, Ns, tw); break;
+            case 2: stockham_stage<2, INV>(buf, M, Ns, tw); break;
+            case 5: stockham_stage<5, INV>(buf, M, Ns, tw); break;
+            default: stockham_stage<3, INV>(buf, M, Ns, tw); break;
+        }
+        Ns *= st.radix[s];
+    }
+}
+
+// X[k] and X[M-k] of the real transform from the half-length complex
+// transform Z (k in [0, M/2]); for k == 0 returns X[0] and X[M].
+__device__ __forceinline__ void real_split(double2 Zk, double2 Zmk, double2 Wk, double2 Wmk,
+                                           double2 &Xk, double2 &Xmk) {
+    // X[k] = 1/2 [(Z[k] + conj Z[M-k]) - i W^k (Z[k] - conj Z[M-k])]
+    // (mul_mi<false>(a) = -i a)
+    const double2 s1 = cadd(Zk, cconj(Zmk)), d1 = csub(Zk, cconj(Zmk));
+    Xk = cscale(cadd(s1, mul_mi<false>(cmul(Wk, d1))), 0.5);
+    const double2 s2 = cadd(Zmk, cconj(Zk)), d2 = csub(Zmk, cconj(Zk));
+    Xmk = cscale(cadd(s2, mul_mi<false>(cmul(Wmk, d2))), 0.5);
+}
+
+// U[k] = 1/2 [(Y[k] + conj Y[M-k]) + i conj(W^k) (Y[k] - conj Y[M-k])]
+__device__ __forceinline__ double2 real_merge(double2 Yk, double2 Ymk, double2 Wk) {
+    const double2 s = cadd(Yk, cconj(Ymk)), d = csub(Yk, cconj(Ymk));
+    return cscale(cadd(s, mul_mi<true>(cmul(cconj(Wk), d))), 0.5);
+}
+
+// ---------------------------------------------------------------------------
+// standalone rfft / irfft (numpy semantics for even n)
+// ---------------------------------------------------------------------------
+__global__ void rfft_lds_kernel(const double *__restrict__ in, int64_t n_in, double2 *__restrict__ out,
+                                int M, StageList st, const double2 *__restrict__ tw,
+                                const double2 *__restrict__ tw2) {
+    extern __shared__ double2 buf[];
+    const int n = 2 * M;
+    for (int m = threadIdx.x; m < M; m += blockDim.x) {
+        const int64_t j0 = 2 * (int64_t)m, j1 = j0 + 1;
+        buf[m] = make_double2(j0 < n_in ? in[j0] : 0.0, j1 < n_in ? in[j1] : 0.0);
+    }
+    (void)n;
+    __syncthreads();
+    fft_lds<false>(buf, M, st, tw);
+    for (int k = threadIdx.x; k <= M / 2; k += blockDim.x) {
+        const double2 Zk = buf[k % M], Zmk = buf[(M - k) % M];
+        double2 Xk, Xmk;
+        real_split(Zk, Zmk, tw2[k], tw2[M - k], Xk, Xmk);
+        out[k] = Xk;
+        out[M - k] = Xmk;
+    }
+}
+
+__global__ void irfft_lds_kernel(const double2 *__restrict__ in, int64_t m_in, double *__restrict__ out,
+                                 int64_t n_out, int M, StageList st, const double2 *__restrict__ tw,
+                                 const double2 *__restrict__ tw2) {
+    extern __shared__ double2 buf[];
+    auto Y = [&](int k) -> double2 {
+        double2 y = k < m_in ? in[k] : make_double2(0.0, 0.0);
+        if (k == 0 || k == M) y.y = 0.0;  // c2r ignores imag of DC and Nyquist
+        return y;
+    };
+    for (int k = threadIdx.x; k <= M / 2; k += blockDim.x) {
+        const double2 Yk = Y(k), Ymk = Y(M - k);
+        buf[k % M] = real_merge(Yk, Ymk, tw2[k]);
+        if (k != 0 && 2 * k != M) buf[M - k] = real_merge(Ymk, Yk, tw2[M - k]);
+    }
+    __syncthreads();
+    fft_lds<true>(buf, M, st, tw);
+    // U carries the 1/2 of the split, so 1/M here gives numpy's 1/n overall
+    const double fct = 1.0 / (double)M;
+    for (int m = threadIdx.x; m < M; m += blockDim.x) {
+        const double2 u = buf[m];
+        const int64_t j0 = 2 * (int64_t)m;
+        if (j0 < n_out) out[j0] = u.x * fct;
+        if (j0 + 1 < n_out) out[j0 + 1] = u.y * fct;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one-turn induced voltage on one workgroup
+// ---------------------------------------------------------------------------
+__global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
+                                   const double2 *__restrict__ tw, const double2 *__restrict__ tw2) {
+    extern __shared__ double2 buf[];
+    const int ns = a.n_slices;
+    unsigned *counts = nullptr;
+    if (a.counts0) counts = (a.turn && (*a.turn & 1)) ? a.counts1 : a.counts0;
+    // load the profile as z[m] = p[2m] + i p[2m+1], zero padded to n = 2M
+    for (int m = threadIdx.x; m < M; m += blockDim.x) {
+        double p0 = 0.0, p1 = 0.0;
+        const int j0 = 2 * m, j1 = j0 + 1;
+        if (counts) {
+            if (j0 < ns) { p0 = (double)counts[j0]; a.profile[j0] = p0; counts[j0] = 0u; }
+            if (j1 < ns) { p1 = (double)counts[j1]; a.profile[j1] = p1; counts[j1] = 0u; }
+        } else {
+            if (j0 < ns) p0 = a.profile[j0];
+            if (j1 < ns) p1 = a.profile[j1];
+        }
+        buf[m] = make_double2(p0, p1);
+    }
+    __syncthreads();
+    fft_lds<false>(buf, M, st, tw);
+    // spectrum -> times impedance -> half-length inverse input, pairwise.
+    // All pairs are read into registers before any is overwritten.
+    constexpr int PMAX = 8;  // host guarantees M/2+1 <= PMAX * blockDim.x
+    double2 ua[PMAX], ub[PMAX];
+#pragma unroll
+    for (int q = 0; q < PMAX; ++q) {
+        const int k = threadIdx.x + q * blockDim.x;
+        if (k <= M / 2) {
+            const double2 Zk = buf[k % M], Zmk = buf[(M - k) % M];
+            double2 Xk, Xmk;
+            real_split(Zk, Zmk, tw2[k], tw2[M - k], Xk, Xmk);
+            // impedance.py:479-484: total_impedance * beam_spectrum
+            double2 Yk = cmul(a.Z[k], Xk), Ymk = cmul(a.Z[M - k], Xmk);
+            if (k == 0) {  // c2r ignores the imaginary parts of DC and Nyquist
+                Yk.y = 0.0;
+                Ymk.y = 0.0;
+            }
+            ua[q] = real_merge(Yk, Ymk, tw2[k]);
+            ub[q] = real_merge(Ymk, Yk, tw2[M - k]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PMAX; ++q) {
+        const int k = threadIdx.x + q * blockDim.x;
+        if (k <= M / 2) {
+            buf[k % M] = ua[q];
+            if (k != 0 && 2 * k != M) buf[M - k] = ub[q];
+        }
+    }
+    __syncthreads();
+    fft_lds<true>(buf, M, st, tw);
+    // V = -(charge*e*ratio * irfft(...))[:n_slices]   (impedance.py:475-489)
+    // Each thread rewrites only its own buf[m] as the two reals vl[2m], vl[2m+1].
+    // U carries the 1/2 of the split, so 1/M here gives numpy's 1/n overall
+    const double fct = 1.0 / (double)M;
+    double *vl = reinterpret_cast<double *>(buf);
+    for (int m = threadIdx.x; m < M; m += blockDim.x) {
+        const double2 u = buf[m];
+        const double v0 = -(a.scale * (u.x * fct)), v1 = -(a.scale * (u.y * fct));
+        vl[2 * m] = v0;
+        vl[2 * m + 1] = v1;
+        if (2 * m < a.n_out) a.volt[2 * m] = v0;
+        if (2 * m + 1 < a.n_out) a.volt[2 * m + 1] = v1;
+    }
+    if (a.vk) {
+        __syncthreads();
+        // linear_interp_kick.cpp:33-48 on V (acc = 0 for the induced kick)
+        const double inv_bw = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
+        if (threadIdx.x == 0) {
+            a.tab_params[0] = a.bc[0];
+            a.tab_params[1] = inv_bw;
+        }
+        for (int i = threadIdx.x; i < ns - 1; i += blockDim.x) {
+            const double k = a.charge * (vl[i + 1] - vl[i]) * inv_bw;
+            a.vk[i] = k;
+            a.fac[i] = (a.charge * vl[i] - a.bc[i] * k) + 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// resonator impedance (fast_resonator.cpp:45-71): one thread per frequency,
+// resonators accumulated in order
+// ---------------------------------------------------------------------------
+__global__ void fast_resonator_kernel(double *__restrict__ re, double *__restrict__ im,
+                                      const double *__restrict__ freq, const double *__restrict__ R,
+                                      const double *__restrict__ Q, const double *__restrict__ fr,
+                                      int n_res, int n_f) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < 1 || k >= n_f) return;
+    double zr = re[k], zi = im[k];
+    const double f = freq[k];
+    for (int r = 0; r < n_res; ++r) {
+        const double q2 = Q[r] * Q[r];
+        const double u = f / fr[r] - fr[r] / f;
+        zr = zr + R[r] / (1.0 + q2 * u * u);
+        zi = zi - R[r] * (Q[r] * u) / (1.0 + q2 * u * u);
+    }
+    re[k] = zr;
+    im[k] = zi;
+}
+
+// ---------------------------------------------------------------------------
+// plans
+// ---------------------------------------------------------------------------
+static bool factor_radices(int M, StageList &st) {
+    st.n = 0;
+    int m = M;
+    const int order[3] = {8, 4, 2};
+    for (int r : order)
+        while (m % r == 0) {
+            if (st.n >= kMaxStages) return false;
+            st.radix[st.n++] = r;
+            m /= r;
+        }
+    const int odd[2] = {5, 3};
+    for (int r : odd)
+        while (m % r == 0) {
+            if (st.n >= kMaxStages) return false;
+            st.radix[st.n++] = r;
+            m /= r;
+        }
+    return m == 1;
+}
+
+static int lds_threads(int M) {
+    int t = (M + 14) / 15;
+    t = ((t + 63) / 64) * 64;
+    if (t < 64) t = 64;
+    if (t > 1024) t = 1024;
+    return t;
+}
+
+int64_t fft_lds_max_real() {
+    const int64_t lds = device_max_lds();
+    return 2 * (lds / (int64_t)sizeof(double2));
+}
+
+struct PlanCache {
+    std::mutex mu;
+    std::map<std::pair<int, int>, FftPlan> plans;  // (device, M)
+};
+static PlanCache &plan_cache() {
+    static PlanCache c;
+    return c;
+}
+
+int fft_plan_get(int64_t n_real, FftPlan *plan) {
+    if (n_real < 2 || (n_real & 1))
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "FFT length %lld must be even and >= 2",
+                              (long long)n_real);
+    if (n_real > fft_lds_max_real())
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED,
+                              "FFT length %lld exceeds the single-workgroup LDS transform (%lld)",
+                              (long long)n_real, (long long)fft_lds_max_real());
+    const int M = (int)(n_real / 2);
+    StageList st;
+    if (!factor_radices(M, st))
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED,
+                              "FFT length %lld is not of the form 2*2^a*3^b*5^c", (long long)n_real);
+    int dev = 0;
+    BLMI_HIP_TRY(hipGetDevice(&dev));
+    PlanCache &pc = plan_cache();
+    std::lock_guard<std::mutex> lk(pc.mu);
+    auto key = std::make_pair(dev, M);
+    auto it = pc.plans.find(key);
+    if (it != pc.plans.end()) {
+        *plan = it->second;
+        return BLMI_OK;
+    }
+    FftPlan p;
+    p.M = M;
+    p.n_real = 2 * M;
+    p.n_stages = st.n;
+    for (int s = 0; s < st.n; ++s) p.radix[s] = st.radix[s];
+    p.threads = lds_threads(M);
+    // twiddles from x87 extended-precision cosl/sinl, then rounded once to
+    // double, so each entry is (nearly always) the correctly rounded value
+    auto cis = [](long double num, long double den) {
+        // exp(-2 pi i num/den)
+        const long double two_pi = 6.283185307179586476925286766559L;
+        long double a = two_pi * num / den;
+        return make_double2((double)cosl(a), (double)(-sinl(a)));
+    };
+    std::vector<double2> tw(M), tw2(M + 1);
+    for (int m = 0; m < M; ++m) tw[m] = cis(m, M);
+    for (int k = 0; k <= M; ++k) tw2[k] = cis(k, 2.0L * M);
+    BLMI_HIP_TRY(hipMalloc(&p.tw, sizeof(double2) * M));
+    BLMI_HIP_TRY(hipMalloc(&p.tw2, sizeof(double2) * (M + 1)));
+    BLMI_HIP_TRY(hipMemcpy(p.tw, tw.data(), sizeof(double2) * M, hipMemcpyHostToDevice));
+    BLMI_HIP_TRY(hipMemcpy(p.tw2, tw2.data(), sizeof(double2) * (M + 1), hipMemcpyHostToDevice));
+    pc.plans[key] = p;
+    *plan = p;
+    return BLMI_OK;
+}
+
+static StageList stages_of(const FftPlan &p) {
+    StageList st;
+    st.n = p.n_stages;
+    for (int s = 0; s < p.n_stages; ++s) st.radix[s] = p.radix[s];
+    return st;
+}
+
+static int set_lds_limit(const void *fn, size_t bytes) {
+    if (bytes > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess)
+            return blmi_set_error(BLMI_ERR_HIP, "hipFuncSetAttribute(%zu B LDS): %s", bytes,
+                                  hipGetErrorString(e));
+    }
+    return BLMI_OK;
+}
+
+int launch_rfft(const double *in, int64_t n_in, double2 *out, const FftPlan &p, hipStream_t s) {
+    const size_t lds = sizeof(double2) * p.M;
+    int rc = set_lds_limit((const void *)rfft_lds_kernel, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(rfft_lds_kernel, dim3(1), dim3(p.threads), lds, s, in, n_in, out, p.M,
+                       stages_of(p), p.tw, p.tw2);
+    return blmi_check_launch("rfft");
+}
+
+int launch_irfft(const double2 *in, int64_t m_in, double *out, int64_t n_out, const FftPlan &p,
+                 hipStream_t s) {
+    const size_t lds = sizeof(double2) * p.M;
+    int rc = set_lds_limit((const void *)irfft_lds_kernel, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(irfft_lds_kernel, dim3(1), dim3(p.threads), lds, s, in, m_in, out, n_out,
+                       p.M, stages_of(p), p.tw, p.tw2);
+    return blmi_check_launch("irfft");
+}
+
+int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream_t s) {
+    if (p.M / 2 + 1 > 8 * p.threads || p.M > 16 * p.threads)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "FFT plan M=%d too large for %d threads", p.M,
+                              p.threads);
+    if (a.n_slices > p.n_real || a.n_out < 0 || a.n_out > p.n_real)
+        return blmi_set_error(BLMI_ERR_ARG, "n_slices %d / n_out %d exceed n_fft %d", a.n_slices,
+                              a.n_out, p.n_real);
+    const size_t lds = sizeof(double2) * p.M;
+    int rc = set_lds_limit((const void *)induced_lds_kernel, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(induced_lds_kernel, dim3(1), dim3(p.threads), lds, s, a, p.M, stages_of(p),
+                       p.tw, p.tw2);
+    return blmi_check_launch("induced_voltage");
+}
+
+int launch_fast_resonator(double *re, double *im, const double *freq, const double *R,
+                          const double *Q, const double *fr, int n_res, int n_f, hipStream_t s) {
+    if (n_f <= 1) return BLMI_OK;
+    hipLaunchKernelGGL(fast_resonator_kernel, dim3((n_f + 255) / 256), dim3(256), 0, s, re, im,
+                       freq, R, Q, fr, n_res, n_f);
+    return blmi_check_launch("fast_resonator");
+}
+
+}  // namespace blmi

@@ -1,0 +1,187 @@
+"""Fused tracking turn on the MI355X.
+
+``FusedTracker`` runs the same map a BLonD main script runs object by object,
+
+    with intensity effects: [profile, total_induced_voltage, tracker]
+                            (EX_18_robinson_instability.py:226)
+    without:                [tracker, profile]  (EX_01_Acceleration.py:152-153)
+                            or [tracker]
+
+as at most two kernels per turn (induced voltage, then one streaming pass
+doing interpolated kick + RF kick + drift + histogram), replayed from a
+captured HIP graph.  After ``track(k)`` the objects hold exactly what k passes
+of the map leave in them: ``beam.dt/dE``, ``profile.n_macroparticles``,
+``total_induced_voltage.induced_voltage``, ``rf_station.counter`` and the
+beam's synchronous energy.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from .. import _lib
+from ..constants import e
+from ..impedances.impedance import InducedVoltageFreq
+from ..utils import bmath as bm
+
+_SOLVER = {"simple": 0, "legacy": 1, "exact": 2}
+
+
+def _ptr(t) -> ct.c_void_p:
+    return ct.c_void_p(t.data_ptr())
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+class FusedTracker:
+    def __init__(self, tracker, profile=None, total_induced_voltage=None, use_graph=True,
+                 block=256, blocks_per_cu=8, graph_chunk=50):
+        import torch
+
+        if bm.precision.num != 2:
+            raise NotImplementedError("the fused engine runs in double precision")
+        if tracker.interpolation:
+            raise NotImplementedError("interpolation=True is tracked object by object")
+        self.tracker = tracker
+        self.profile = profile
+        self.tiv = total_induced_voltage
+        self.beam = tracker.beam
+        self.rf = tracker.rf_params
+        self.use_graph = bool(use_graph)
+        if self.tiv is not None and self.profile is None:
+            raise ValueError("intensity effects need the profile")
+        if self.tiv is not None and self.tiv.profile is not self.profile:
+            raise ValueError("total_induced_voltage must use the same profile")
+        if self.profile is not None and self.profile.smooth:
+            raise NotImplementedError("the fused turn bins with the nearest-bin histogram")
+        tracker.to_gpu()
+        if self.profile is not None:
+            self.profile.to_gpu()
+        if self.tiv is not None:
+            self.tiv.to_gpu()
+        if self.beam.dt.dtype != torch.float64:
+            raise TypeError("beam coordinates must be float64")
+        self._torch = torch
+        rp = self.rf
+        n_turns = rp.n_turns
+        self.n_turns = n_turns
+        n = self.beam.n_macroparticles
+        lib = _lib.load()
+        h = lib.blmi_engine_create(n, rp.n_rf, n_turns, _SOLVER[tracker.solver])
+        if not h:
+            raise _lib.BlmiError(f"blmi_engine_create: {_lib.last_error()}")
+        self._h = ct.c_void_p(h)
+        _lib.call("blmi_engine_set_particles", self._h, _ptr(self.beam.dt), _ptr(self.beam.dE), n)
+        self._upload_programs()
+        if self.profile is not None:
+            p = self.profile
+            _lib.call("blmi_engine_set_slices", self._h, p.n_slices, float(p.cut_left),
+                      float(p.cut_right), _ptr(p.n_macroparticles))
+        if self.tiv is not None:
+            self._setup_induced()
+        _lib.call("blmi_engine_set_launch", self._h, int(block), int(blocks_per_cu),
+                  int(graph_chunk))
+        _lib.call("blmi_engine_set_turn", self._h, int(rp.counter[0]))
+        self._primed = False
+
+    # ------------------------------------------------------------------
+    def _phase_program(self):
+        """phi_rf for all turns with the accumulated offsets of
+        tracker.py:556-568 applied, from the current counter on (the
+        recurrence depends only on the programs, so it can be precomputed)."""
+        rp = self.rf
+        phi = np.array(rp.host_array("phi_rf"), dtype=np.float64, copy=True)
+        omega = rp.host_array("omega_rf")
+        dphi = np.array(rp.dphi_rf, dtype=np.float64, copy=True)
+        for t in range(rp.counter[0], rp.n_turns):
+            dphi = dphi + (2.0 * np.pi * rp.harmonic[:, t + 1]
+                           * (omega[:, t + 1] - rp.omega_rf_d[:, t + 1]) / rp.omega_rf_d[:, t + 1])
+            phi[:, t + 1] += dphi
+        return phi
+
+    def _upload_programs(self):
+        rp = self.rf
+        nt = rp.n_turns + 1
+        self._phi_prog = self._phase_program()
+        rf = np.empty((nt, 3, rp.n_rf), dtype=np.float64)
+        rf[:, 0, :] = np.asarray(rp.host_array("voltage"), dtype=np.float64).T
+        rf[:, 1, :] = np.asarray(rp.host_array("omega_rf"), dtype=np.float64).T
+        rf[:, 2, :] = self._phi_prog.T
+        rf = np.ascontiguousarray(rf)
+        acc = np.zeros(nt, dtype=np.float64)
+        acc[: len(self.tracker.acceleration_kick)] = self.tracker.acceleration_kick
+        arrs = [np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in
+                (rp.t_rev, rp.eta_0, rp.eta_1, rp.eta_2, rp.alpha_0, rp.alpha_1, rp.alpha_2,
+                 rp.beta, rp.energy)]
+        self._keep = (rf, acc, arrs)
+        _lib.call("blmi_engine_set_programs", self._h, _dptr(rf), _dptr(acc), _dptr(arrs[0]),
+                  float(rp.length_ratio), float(rp.alpha_order), _dptr(arrs[1]), _dptr(arrs[2]),
+                  _dptr(arrs[3]), _dptr(arrs[4]), _dptr(arrs[5]), _dptr(arrs[6]),
+                  _dptr(arrs[7]), _dptr(arrs[8]), float(rp.particle.charge))
+
+    def _setup_induced(self):
+        torch = self._torch
+        srcs = self.tiv.induced_voltage_list
+        if not srcs or not all(isinstance(s, InducedVoltageFreq) for s in srcs):
+            raise NotImplementedError("the fused turn supports InducedVoltageFreq sources")
+        n_fft = srcs[0].n_fft
+        if any(s.n_fft != n_fft for s in srcs):
+            raise NotImplementedError("all sources must share n_fft")
+        z = 0
+        for s in srcs:
+            zi = s.total_impedance
+            zi = zi.cpu().numpy() if hasattr(zi, "cpu") else np.asarray(zi)
+            z = z + zi
+        self._Z = np.ascontiguousarray(z, dtype=np.complex128)
+        scale = self.beam.particle.charge * e * self.beam.ratio
+        p = self.profile
+        iv = self.tiv.induced_voltage
+        if not (hasattr(iv, "is_cuda") and iv.is_cuda and iv.numel() == p.n_slices
+                and iv.dtype == torch.float64 and iv.is_contiguous()):
+            iv = torch.zeros(p.n_slices, dtype=torch.float64, device="cuda")
+            self.tiv.induced_voltage = iv
+        _lib.call("blmi_engine_set_induced", self._h, int(n_fft), _dptr(self._Z), float(scale),
+                  _ptr(iv), _ptr(p.bin_centers))
+
+    # ------------------------------------------------------------------
+    def invalidate(self):
+        """Call after changing beam.dt outside the tracker: the staged
+        histogram of the current turn is rebuilt on the next track()."""
+        self._primed = False
+
+    def track(self, n_turns: int = 1):
+        """Advance ``n_turns`` turns (k passes of the map)."""
+        n_turns = int(n_turns)
+        if n_turns <= 0:
+            return
+        stream = ct.c_void_p(self._torch.cuda.current_stream().cuda_stream)
+        if self.tiv is not None and not self._primed:
+            _lib.call("blmi_engine_prime", self._h, stream)
+            self._primed = True
+        start = self.rf.counter[0]
+        _lib.call("blmi_engine_track", self._h, n_turns, 1 if self.use_graph else 0, stream)
+        end = start + n_turns
+        rp = self.rf
+        # host-side state the map leaves behind (tracker.py:556-615)
+        if end > start:
+            host_phi = rp.host_array("phi_rf")
+            host_phi[:, start + 1:end + 1] = self._phi_prog[:, start + 1:end + 1]
+        rp.counter[0] = end
+        self.beam.beta = rp.beta[end]
+        self.beam.gamma = rp.gamma[end]
+        self.beam.energy = rp.energy[end]
+        self.beam.momentum = rp.momentum[end]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().blmi_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

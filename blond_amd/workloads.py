@@ -1,0 +1,194 @@
+"""Synthetic benchmark workloads (seeded, no external data).
+
+  c1  LHC-like ring (26658.883 m, 1/55.759505^2, 450 GeV/c), h=35640, 6 MV,
+      Gaussian bunch 4 sigma = 1 ns, 100 slices over [0, t_rf]; turn =
+      tracker, profile.
+  c2  PSB-like ring (157.08 m, gamma_t = 4.4, 310 MeV/c), h=1, 8 kV, phi=pi,
+      Gaussian bunch as c1, intensity 1e13, 4096 slices over [0, t_rf],
+      n_fft = 16384; broadband resonator (5 kOhm, 1 GHz, Q=1) + 20 narrowband
+      resonators drawn from default_rng(42); turn = profile, induced
+      voltage, tracker.
+  c3  c1 ring with a second RF system at 4h, V2/V1 = 0.25 in
+      bunch-lengthening phase (phi2 = 0: the two slopes cancel at the bunch
+      centre, phi = pi), 65536 slices, n_fft = 131072, resonators as c2.
+
+Bunch draws follow bigaussian(seed=1): dt from default_rng(1), dE from
+default_rng(2).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .beam.beam import Beam, Proton
+from .beam.distributions import bigaussian
+from .beam.profile import CutOptions, OtherSlicesOptions, Profile
+from .constants import e
+from .impedances.impedance import InducedVoltageFreq, TotalInducedVoltage
+from .impedances.impedance_sources import Resonators
+from .input_parameters.rf_parameters import RFStation
+from .input_parameters.ring import Ring
+from .trackers.tracker import RingAndRFTracker
+
+
+def resonator_set(n_narrow: int = 20, seed: int = 42, broadband=True):
+    """Broadband (5 kOhm, 1 GHz, Q=1) plus narrowband resonators with
+    R = 10**U(3,6), f_r = U(1e6, 2e8), Q = 10**U(2,4), drawn in that order."""
+    rng = np.random.default_rng(seed)
+    R = 10 ** rng.uniform(3, 6, n_narrow)
+    fr = rng.uniform(1e6, 2e8, n_narrow)
+    Q = 10 ** rng.uniform(2, 4, n_narrow)
+    if broadband:
+        R = np.concatenate([[5e3], R])
+        fr = np.concatenate([[1e9], fr])
+        Q = np.concatenate([[1.0], Q])
+    return R, fr, Q
+
+
+@dataclass
+class Workload:
+    name: str
+    ring: Ring
+    rf: RFStation
+    beam: Beam
+    tracker: RingAndRFTracker
+    profile: Profile | None = None
+    tiv: TotalInducedVoltage | None = None
+    intensity: bool = False
+    meta: dict = field(default_factory=dict)
+    _fused: object = None
+
+    def map(self):
+        """The per-turn map as a BLonD main script runs it."""
+        if self.intensity:
+            return [self.profile, self.tiv, self.tracker]
+        return [self.tracker] + ([self.profile] if self.profile is not None else [])
+
+    def to_gpu(self):
+        self.tracker.to_gpu()
+        if self.profile is not None:
+            self.profile.to_gpu()
+        if self.tiv is not None:
+            self.tiv.to_gpu()
+
+    def fused(self, n_turns: int = 1, **launch):
+        """Track n_turns with the fused engine (created on first use)."""
+        from .trackers.fused import FusedTracker
+
+        if self._fused is None:
+            self._fused = FusedTracker(self.tracker, self.profile, self.tiv, **launch)
+        self._fused.track(n_turns)
+        return self._fused
+
+    def snapshot(self) -> dict:
+        """Host copies of everything a CPU restatement of the turn needs."""
+        rp = self.rf
+        st = {
+            "name": self.name,
+            "intensity": self.intensity,
+            "turn": int(rp.counter[0]),
+            "dt": _h(self.beam.dt).copy(),
+            "dE": _h(self.beam.dE).copy(),
+            "n_rf": rp.n_rf,
+            "charge": float(rp.particle.charge),
+            "voltage": np.array(rp.host_array("voltage"), dtype=np.float64),
+            "omega_rf": np.array(rp.host_array("omega_rf"), dtype=np.float64),
+            "phi_rf": np.array(rp.host_array("phi_rf"), dtype=np.float64),
+            "acc": np.array(self.tracker.acceleration_kick, dtype=np.float64),
+            "solver": self.tracker.solver,
+            "t_rev": np.array(rp.t_rev), "length_ratio": float(rp.length_ratio),
+            "alpha_order": float(rp.alpha_order),
+            "eta_0": np.array(rp.eta_0), "eta_1": np.array(rp.eta_1), "eta_2": np.array(rp.eta_2),
+            "alpha_0": np.array(rp.alpha_0), "alpha_1": np.array(rp.alpha_1),
+            "alpha_2": np.array(rp.alpha_2), "beta": np.array(rp.beta),
+            "energy": np.array(rp.energy),
+        }
+        if self.profile is not None:
+            p = self.profile
+            st.update(n_slices=p.n_slices, cut_left=float(p.cut_left), cut_right=float(p.cut_right),
+                      bin_centers=_h(p.bin_centers).copy(),
+                      profile=_h(p.n_macroparticles).copy())
+        if self.tiv is not None:
+            src = self.tiv.induced_voltage_list[0]
+            z = 0
+            for s in self.tiv.induced_voltage_list:
+                z = z + _h(s.total_impedance)
+            st.update(Z=np.asarray(z, dtype=np.complex128), n_fft=int(src.n_fft),
+                      n_induced_voltage=int(src.n_induced_voltage),
+                      ratio=float(self.beam.ratio),
+                      induced_voltage=_h(self.tiv.induced_voltage).copy())
+        return st
+
+    def oracle_copy(self) -> dict:
+        return self.snapshot()
+
+
+def _h(a):
+    if hasattr(a, "detach"):
+        return a.detach().cpu().numpy()
+    return np.asarray(a)
+
+
+def _profile(beam, cut_left, cut_right, n_slices):
+    return Profile(beam, CutOptions(cut_left=cut_left, cut_right=cut_right, n_slices=n_slices),
+                   other_slices_options=OtherSlicesOptions(direct_slicing=False))
+
+
+def build_workload(name: str, n_particles: int | None = None, n_turns: int | None = None,
+                   n_slices: int | None = None, n_fft: int | None = None, seed: int = 1,
+                   n_resonators: int = 20, to_gpu: bool = True) -> Workload:
+    """Seeded synthetic workload ``c1``, ``c2`` or ``c3`` (see module doc).
+    Device residency (``to_gpu``) needs a GPU; with ``to_gpu=False`` only the
+    host setup runs."""
+    if name == "c1":
+        n_particles = n_particles or 1_000_000
+        n_turns = n_turns or 1000
+        n_slices = n_slices or 100
+        ring = Ring(26658.883, 1 / 55.759505 ** 2, 450e9, Proton(), n_turns)
+        rf = RFStation(ring, [35640], [6e6], [0.0])
+        beam = Beam(ring, n_particles, 1e9)
+        bigaussian(ring, rf, beam, sigma_dt=1.0e-9 / 4, seed=seed)
+        prof = _profile(beam, 0.0, float(rf.t_rf[0, 0]), n_slices)
+        tr = RingAndRFTracker(rf, beam)
+        w = Workload("c1", ring, rf, beam, tr, prof, None, False)
+    elif name in ("c2", "c3"):
+        if name == "c2":
+            n_particles = n_particles or 4_000_000
+            n_turns = n_turns or 2000
+            n_slices = n_slices or 4096
+            n_fft = n_fft or 16384
+            ring = Ring(157.08, 1 / 4.4 ** 2, 310e6, Proton(), n_turns)
+            rf = RFStation(ring, [1], [8e3], [np.pi])
+            intensity = 1e13
+        else:
+            n_particles = n_particles or 10_000_000
+            n_turns = n_turns or 500
+            n_slices = n_slices or 65536
+            n_fft = n_fft or 131072
+            ring = Ring(26658.883, 1 / 55.759505 ** 2, 450e9, Proton(), n_turns)
+            rf = RFStation(ring, [35640, 142560], [6e6, 1.5e6], [0.0, 0.0], n_rf=2)
+            intensity = 1e11
+        beam = Beam(ring, n_particles, intensity)
+        bigaussian(ring, rf, beam, sigma_dt=1.0e-9 / 4, seed=seed)
+        prof = _profile(beam, 0.0, float(rf.t_rf[0, 0]), n_slices)
+        R, fr, Q = resonator_set(n_resonators)
+        res = Resonators(R, fr, Q)
+        # frequency resolution chosen so that n_induced_voltage = n_fft exactly
+        f_res = (1 + 1e-9) / (n_fft * prof.bin_size)
+        ivf = InducedVoltageFreq(beam, prof, [res], frequency_resolution=f_res)
+        tiv = TotalInducedVoltage(beam, prof, [ivf])
+        tr = RingAndRFTracker(rf, beam)
+        w = Workload(name, ring, rf, beam, tr, prof, tiv, True)
+        w.meta["n_fft"] = ivf.n_fft
+    else:
+        raise ValueError(f"unknown workload {name!r}")
+    w.meta.update(n_particles=n_particles, n_turns=n_turns, n_slices=n_slices)
+    if to_gpu:
+        w.to_gpu()
+    return w
+
+
+def induced_scale(beam) -> float:
+    """charge * e * ratio (impedance.py:475-485)."""
+    return beam.particle.charge * e * beam.ratio

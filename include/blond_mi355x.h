@@ -1,0 +1,164 @@
+/*
+ * blond_mi355x.h -- C ABI of libblond_mi355x.so, the MI355X (gfx950) engine
+ * for BLonD's per-turn macroparticle tracking path.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer (HBM), except where a name ends
+ *     in _host.  Particle arrays are structure-of-arrays dt[n], dE[n].
+ *   - Particle counts are int64 (the reference passes int32, butils_wrap_cpp.py:79-80).
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Calls are asynchronous with respect to the host unless noted.
+ *   - Every function returns 0 (BLMI_OK) or a BLMI_ERR_* code; the message of
+ *     the last failure on the calling thread is blmi_last_error().
+ *   - The reference entry point each function replaces is cited as
+ *     path:line of blond-admin/BLonD.  Argument order and meaning follow it;
+ *     differences are listed per function.
+ *   - real_t: functions without suffix take double (the reference's
+ *     libblond_double.so); the _f32 variants take float (libblond_single.so,
+ *     built with -DUSEFLOAT, blond/compile.py:167).
+ */
+#ifndef BLOND_MI355X_H
+#define BLOND_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLMI_OK 0
+#define BLMI_ERR_ARG 1
+#define BLMI_ERR_HIP 2
+#define BLMI_ERR_UNSUPPORTED 3
+#define BLMI_ERR_ALLOC 4
+
+/* Message of the last failed call on this thread. */
+const char *blmi_last_error(void);
+int blmi_version(void);
+/* Select the device for subsequent calls from this thread (hipSetDevice). */
+int blmi_init(int device);
+int blmi_device_info(int *cus, int *max_lds_per_workgroup);
+
+/* RF kick: dE[i] += charge * sum_j voltage[j] sin(omega_rf[j] dt[i] + phi_rf[j]) + acc_kick.
+ * Replaces kick(), blond/cpp_routines/kick.cpp:16-35 (bound by
+ * blond/utils/butils_wrap_cpp.py:863-898).  Bit-identical to the reference's
+ * strict build (same Cephes-method sine, same operation order). */
+int blmi_kick(const double *dt, double *dE, int n_rf, double charge, const double *voltage,
+              const double *omega_rf, const double *phi_rf, int64_t n, double acc_kick,
+              void *stream);
+int blmi_kick_f32(const float *dt, float *dE, int n_rf, float charge, const float *voltage,
+                  const float *omega_rf, const float *phi_rf, int64_t n, float acc_kick,
+                  void *stream);
+
+/* Drift: dt[i] += f(dE[i]) with solver 0 simple, 1 legacy, 2 exact.
+ * Replaces drift(), blond/cpp_routines/drift.cpp:19-78 (butils_wrap_cpp.py:901-943). */
+int blmi_drift(double *dt, const double *dE, int solver, double t_rev, double length_ratio,
+               double alpha_order, double eta_0, double eta_1, double eta_2, double alpha_0,
+               double alpha_1, double alpha_2, double beta, double energy, int64_t n,
+               void *stream);
+int blmi_drift_f32(float *dt, const float *dE, int solver, float t_rev, float length_ratio,
+                   float alpha_order, float eta_0, float eta_1, float eta_2, float alpha_0,
+                   float alpha_1, float alpha_2, float beta, float energy, int64_t n,
+                   void *stream);
+
+/* Nearest-bin histogram of dt over [cut_left, cut_right) into profile[n_slices]
+ * (overwritten; counts stored as real).  Replaces histogram(),
+ * blond/cpp_routines/histogram.cpp:22-73 (slice_beam, butils_wrap_cpp.py:971-984).
+ * Counts are identical to the reference for identical dt. */
+int blmi_histogram(const double *dt, double *profile, double cut_left, double cut_right,
+                   int n_slices, int64_t n, void *stream);
+int blmi_histogram_f32(const float *dt, float *profile, float cut_left, float cut_right,
+                       int n_slices, int64_t n, void *stream);
+
+/* Two-bin "smooth" histogram with the reference's weight rule.  Replaces
+ * smooth_histogram(), histogram.cpp:75-132 (slice_smooth, butils_wrap_cpp.py:987-1000).
+ * Weights are summed in 64-bit fixed point: deterministic, within ~1e-13 of
+ * the exact sum per particle. */
+int blmi_smooth_histogram(const double *dt, double *profile, double cut_left, double cut_right,
+                          int n_slices, int64_t n, void *stream);
+int blmi_smooth_histogram_f32(const float *dt, float *profile, float cut_left, float cut_right,
+                              int n_slices, int64_t n, void *stream);
+
+/* Kick by linear interpolation of voltage[n_slices] on bin_centers.
+ * Replaces linear_interp_kick(), linear_interp_kick.cpp:21-71
+ * (butils_wrap_cpp.py:946-968).  dt is const here (the reference's pointer is
+ * non-const but never written). */
+int blmi_linear_interp_kick(const double *dt, double *dE, const double *voltage,
+                            const double *bin_centers, double charge, int n_slices, int64_t n,
+                            double acc_kick, void *stream);
+int blmi_linear_interp_kick_f32(const float *dt, float *dE, const float *voltage,
+                                const float *bin_centers, float charge, int n_slices, int64_t n,
+                                float acc_kick, void *stream);
+
+/* RF voltage on bin centres, accumulated into rf_voltage[n_bins].
+ * Replaces rf_volt_comp(), kick.cpp:37-51 (butils_wrap_cpp.py:835-860). */
+int blmi_rf_volt_comp(const double *voltage, const double *omega_rf, const double *phi_rf,
+                      const double *bin_centers, int n_rf, int n_bins, double *rf_voltage,
+                      void *stream);
+int blmi_rf_volt_comp_f32(const float *voltage, const float *omega_rf, const float *phi_rf,
+                          const float *bin_centers, int n_rf, int n_bins, float *rf_voltage,
+                          void *stream);
+
+/* Resonator impedance accumulated into (re, im)[n_freq] for freq[1:].
+ * Replaces fast_resonator_real_imag(), fast_resonator.cpp:19-73
+ * (fast_resonator, butils_wrap_cpp.py:1133-1171). */
+int blmi_fast_resonator(double *re, double *im, const double *freq, const double *R_S,
+                        const double *Q, const double *f_r, int n_res, int n_freq, void *stream);
+
+/* numpy.fft.rfft(in[:n_in], n) -> out (n/2+1 complex, interleaved re/im) and
+ * numpy.fft.irfft(in[:m_in], n) -> out[n].  These replace the numpy FFTs the
+ * reference binds as bm.rfft/bm.irfft (blond/utils/bmath_backends.py:447-449).
+ * n must be even and 5-smooth (2*2^a*3^b*5^c), at most blmi_fft_max_lds_length. */
+int blmi_fft_max_lds_length(int64_t *n);
+int blmi_rfft(const double *in, int64_t n_in, double *out, int64_t n, void *stream);
+int blmi_irfft(const double *in, int64_t m_in, double *out, int64_t n, void *stream);
+
+/* One-turn induced voltage: volt = -(scale * irfft(Z * rfft(profile, n_fft)))[:n_out]
+ * with scale = charge * e * ratio and n_out = n_induced_voltage (or n_slices
+ * for the cut of impedance.py:113-131).  Replaces
+ * _InducedVoltage.induced_voltage_1turn, blond/impedances/impedance.py:457-489.
+ * Z: n_fft/2+1 complex (interleaved), already divided by bin_size (impedance.py:890). */
+int blmi_induced_voltage(const double *profile, int n_slices, const double *Z, int64_t n_fft,
+                         double scale, double *volt, int n_out, void *stream);
+
+/* Fused turn engine.  Runs the reference's map
+ *   with intensity effects:  [Profile, TotalInducedVoltage, RingAndRFTracker]
+ *                            (EX_18_robinson_instability.py:226; tracker.py:521-615,
+ *                             profile.py:700-719, impedance.py:113-146)
+ *   without:                 [RingAndRFTracker, Profile]  (EX_01_Acceleration.py:152-153)
+ * as at most two kernels per turn, optionally replayed from a captured HIP graph.
+ * Per-turn programs are uploaded once; the turn counter lives on the device.
+ * No reference counterpart as a single call: it replaces the per-object
+ * .track() loop of the user's main script.
+ *   rf_host: [n_turns+1][3][n_rf] doubles (voltage, omega_rf, phi_rf per turn);
+ *   acc_host: [n_turns+1] acceleration kicks (-delta_E, tracker.py:331);
+ *   t_rev ... energy: [n_turns+1] ring programs (drift index turn+1, tracker.py:606). */
+void *blmi_engine_create(int64_t n, int n_rf, int n_turns, int solver);
+int blmi_engine_destroy(void *engine);
+int blmi_engine_set_particles(void *engine, double *dt, double *dE, int64_t n);
+int blmi_engine_set_programs(void *engine, const double *rf_host, const double *acc_host,
+                             const double *t_rev_host, double length_ratio, double alpha_order,
+                             const double *eta0_host, const double *eta1_host,
+                             const double *eta2_host, const double *alpha0_host,
+                             const double *alpha1_host, const double *alpha2_host,
+                             const double *beta_host, const double *energy_host, double charge);
+int blmi_engine_set_slices(void *engine, int n_slices, double cut_left, double cut_right,
+                           double *profile);
+int blmi_engine_set_induced(void *engine, int64_t n_fft, const double *Z_host, double scale,
+                            double *volt, const double *bin_centers);
+int blmi_engine_set_launch(void *engine, int block, int blocks_per_cu, int graph_chunk);
+int blmi_engine_set_turn(void *engine, int turn);   /* synchronous */
+int blmi_engine_get_turn(void *engine, int *turn);  /* synchronous */
+int blmi_engine_prime(void *engine, void *stream);
+int blmi_engine_track(void *engine, int turns, int use_graph, void *stream);
+/* Track `turns` eager turns with HIP events around each kernel on the engine's
+ * stream; returns the average device time (ms) of the induced-voltage kernel
+ * and of the particle kernel.  Synchronous. */
+int blmi_engine_time_kernels(void *engine, int turns, float *ms_induced, float *ms_turn,
+                             void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLOND_MI355X_H */

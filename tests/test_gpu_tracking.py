@@ -1,0 +1,104 @@
+"""GPU parity of whole turns: the object-by-object map (RingAndRFTracker,
+Profile, TotalInducedVoltage on the MI355X) and the fused engine against the
+oracle's turn loop (the reference's map restated on the CPU)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("no GPU", allow_module_level=True)
+
+from blond_amd import _lib  # noqa: E402
+from blond_amd.workloads import build_workload  # noqa: E402
+
+
+def host(t):
+    return t.detach().cpu().numpy() if hasattr(t, "detach") else np.asarray(t)
+
+
+def test_device_info_printed():
+    import ctypes as ct
+
+    cus, lds = ct.c_int(), ct.c_int()
+    _lib.call("blmi_device_info", ct.byref(cus), ct.byref(lds))
+    n = ct.c_int64()
+    _lib.call("blmi_fft_max_lds_length", ct.byref(n))
+    print(f"\nCUs={cus.value} max LDS/workgroup={lds.value} max LDS FFT length={n.value}")
+    assert cus.value > 0 and lds.value >= 65536
+
+
+@pytest.mark.parametrize("n_particles", [20000, 20001])
+def test_plain_turns_fused_bit_exact(n_particles):
+    w = build_workload("c1", n_particles=n_particles, n_turns=40, n_slices=100)
+    ref = orc.track(w.snapshot(), 25)
+    w.fused(25)
+    np.testing.assert_array_equal(host(w.beam.dt), ref["dt"])
+    np.testing.assert_array_equal(host(w.beam.dE), ref["dE"])
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
+    assert w.rf.counter[0] == 25
+
+
+def test_plain_turns_object_by_object_bit_exact():
+    w = build_workload("c1", n_particles=30001, n_turns=20, n_slices=100)
+    ref = orc.track(w.snapshot(), 10)
+    for _ in range(10):
+        for m in w.map():
+            m.track()
+    np.testing.assert_array_equal(host(w.beam.dt), ref["dt"])
+    np.testing.assert_array_equal(host(w.beam.dE), ref["dE"])
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
+
+
+def test_plain_turns_graph_vs_eager_identical():
+    a = build_workload("c1", n_particles=50000, n_turns=130, n_slices=100)
+    b = build_workload("c1", n_particles=50000, n_turns=130, n_slices=100)
+    a.fused(0)
+    fa = a.fused(120)  # 2 graph replays of 50 + 20 eager turns
+    from blond_amd.trackers.fused import FusedTracker
+
+    fb = FusedTracker(b.tracker, b.profile, None, use_graph=False)
+    fb.track(120)
+    np.testing.assert_array_equal(host(a.beam.dt), host(b.beam.dt))
+    np.testing.assert_array_equal(host(a.beam.dE), host(b.beam.dE))
+    np.testing.assert_array_equal(host(a.profile.n_macroparticles), host(b.profile.n_macroparticles))
+    del fa
+
+
+def _check_intensity(w, ref, rtol):
+    """Across turns the induced voltage's FFT-level rounding (1e-15 of its
+    maximum) feeds back into dE and dt, so multi-turn parity is held to
+    `rtol` (1e-9, the moment tolerance of the tracking path) per particle,
+    and the moments to 1e-9 relative."""
+    dt, dE = host(w.beam.dt), host(w.beam.dE)
+    np.testing.assert_allclose(dt, ref["dt"], rtol=rtol, atol=0)
+    sc = np.abs(ref["dE"]).max()
+    np.testing.assert_allclose(dE, ref["dE"], rtol=0, atol=rtol * sc)
+    for a, b in ((dt, ref["dt"]), (dE, ref["dE"])):
+        assert abs(a.mean() - b.mean()) <= 1e-9 * max(abs(b.mean()), b.std())
+        assert abs(a.std() - b.std()) <= 1e-9 * b.std()
+    v = host(w.tiv.induced_voltage)
+    assert np.max(np.abs(v - ref["induced_voltage"])) <= 1e-10 * np.abs(ref["induced_voltage"]).max()
+
+
+@pytest.mark.parametrize("n_slices,n_fft", [(256, 1024), (4096, 16384)])
+def test_intensity_turns_fused(n_slices, n_fft):
+    w = build_workload("c2", n_particles=40001, n_turns=30, n_slices=n_slices, n_fft=n_fft)
+    ref = orc.track(w.snapshot(), 20)
+    w.fused(20)
+    _check_intensity(w, ref, 1e-9)
+    # profile of the last turn's starting coordinates: identical counts
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
+
+
+def test_intensity_turns_object_by_object():
+    w = build_workload("c2", n_particles=40001, n_turns=12, n_slices=512, n_fft=2048)
+    ref = orc.track(w.snapshot(), 8)
+    for _ in range(8):
+        for m in w.map():
+            m.track()
+    _check_intensity(w, ref, 1e-9)
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
