@@ -43,8 +43,11 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--blocks-per-cu", type=int, default=8)
+    ap.add_argument("--block", type=int, default=1024)
+    ap.add_argument("--blocks-per-cu", type=int, default=1)
+    ap.add_argument("--groups", type=int, default=8)
+    ap.add_argument("--max-peel", type=int, default=0)
+    ap.add_argument("--method", choices=["auto", "fft", "direct"], default="auto")
     return ap.parse_args()
 
 
@@ -81,7 +84,6 @@ def main():
         sys.exit(2)
     import torch
 
-    from blond_amd import _lib
     from blond_amd.trackers.fused import FusedTracker
     from blond_amd.utils import bmath as bm
     from blond_amd.workloads import build_workload
@@ -92,7 +94,8 @@ def main():
     n = w.beam.n_macroparticles
     snap = None if a.no_cpu_baseline else w.snapshot()
     ft = FusedTracker(w.tracker, w.profile, w.tiv, use_graph=not a.no_graph, block=a.block,
-                      blocks_per_cu=a.blocks_per_cu)
+                      blocks_per_cu=a.blocks_per_cu, groups=a.groups, max_peel=a.max_peel,
+                      method=None if a.method == "auto" else a.method)
     ft.track(a.warmup)
     torch.cuda.synchronize()
     times = []
@@ -106,12 +109,8 @@ def main():
     value = n * a.steps / t_med
 
     # live per-kernel durations (HIP events on the engine's stream)
-    import ctypes as ct
-
-    ms_ind, ms_turn = ct.c_float(), ct.c_float()
-    _lib.call("blmi_engine_time_kernels", ft._h, a.timing_turns, ct.byref(ms_ind),
-              ct.byref(ms_turn), ct.c_void_p(torch.cuda.current_stream().cuda_stream))
-    achieved = BYTES_PER_PARTICLE_TURN * n / (ms_turn.value * 1e-3) / 1e9
+    ms_ind, ms_turn = ft.time_kernels(a.timing_turns)
+    achieved = BYTES_PER_PARTICLE_TURN * n / (ms_turn * 1e-3) / 1e9
     turn_gbs = BYTES_PER_PARTICLE_TURN * n / (t_med / a.steps) / 1e9
 
     line = {
@@ -132,11 +131,13 @@ def main():
                    "macroparticles": n, "n_slices": w.meta.get("n_slices"),
                    "n_fft": w.meta.get("n_fft"), "form": "plain",
                    "graph": not a.no_graph, "repeats": a.repeats,
-                   "block": a.block, "blocks_per_cu": a.blocks_per_cu},
+                   "block": a.block, "blocks_per_cu": a.blocks_per_cu, "groups": a.groups,
+                   "max_peel": a.max_peel},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "turn_kernel (interp kick + RF kick + drift + histogram)",
-                     "kernel_ms": ms_turn.value, "induced_ms": ms_ind.value,
+                     "kernel_ms": ms_turn, "induced_ms": ms_ind,
+                     "induced_method": ft.method,
                      "whole_turn_gbs": turn_gbs, "whole_turn_frac": turn_gbs / HBM_PEAK_GBS},
     }
     ft.close()

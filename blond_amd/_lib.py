@@ -53,6 +53,9 @@ SIGNATURES = {
     "blmi_engine_prime": [_p, _p],
     "blmi_engine_track": [_p, _i, _i, _p],
     "blmi_engine_time_kernels": [_p, _i, ct.POINTER(_f), ct.POINTER(_f), _p],
+    "blmi_engine_set_method": [_p, _i],
+    "blmi_engine_get_method": [_p, ct.POINTER(_i)],
+    "blmi_engine_set_hist": [_p, _i, _i],
 }
 _RESTYPES = {
     "blmi_last_error": ct.c_char_p,

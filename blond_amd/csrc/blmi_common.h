@@ -7,6 +7,7 @@
 // rounded expansions the compiler emits without fast-math flags.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -32,6 +33,23 @@ namespace blmi {
 
 // Wavefront width of CDNA4.
 constexpr int kWave = 64;
+
+// Optional start/stop events recorded by the kernel's own dispatch
+// (hipExtLaunchKernelGGL), i.e. the kernel's execution interval without the
+// queue gaps a separate hipEventRecord would add.
+struct KernelEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+template <typename F, typename... Args>
+inline void launch(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                   const KernelEvents &ev, Args... args) {
+    if (ev.start)
+        hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)lds, s, ev.start, ev.stop, 0u,
+                              args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+}
 
 // ---------------------------------------------------------------------------
 // Sine and cosine by the Cephes method (S. L. Moshier, Cephes Math Library,

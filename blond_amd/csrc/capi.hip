@@ -291,7 +291,7 @@ int blmi_induced_voltage(const double *profile, int n_slices, const double *Z, i
     int rc = fft_plan_get(n_fft, &p);
     if (rc) return rc;
     InducedArgs a{};
-    a.profile = const_cast<double *>(profile);
+    a.src = profile;
     a.n_slices = n_slices;
     a.Z = reinterpret_cast<const double2 *>(Z);
     a.scale = scale;
@@ -364,13 +364,12 @@ int blmi_engine_set_induced(void *h, int64_t n_fft, const double *Z_host, double
 int blmi_engine_set_launch(void *h, int block, int blocks_per_cu, int graph_chunk) {
     Engine *e = reinterpret_cast<Engine *>(h);
     BLMI_REQUIRE(e, "engine: null handle");
-    BLMI_REQUIRE(block == 64 || block == 128 || block == 256 || block == 512,
-                 "engine: block size must be 64/128/256/512");
+    BLMI_REQUIRE(block == 64 || block == 128 || block == 256 || block == 512 || block == 1024,
+                 "engine: block size must be 64/128/256/512/1024");
     BLMI_REQUIRE(blocks_per_cu >= 1 && blocks_per_cu <= 64, "engine: blocks_per_cu in [1, 64]");
     BLMI_REQUIRE(graph_chunk >= 1 && graph_chunk <= 1000, "engine: graph_chunk in [1, 1000]");
     e->block_size = block;
     e->blocks_per_cu = blocks_per_cu;
-    if (e->graph_chunk != graph_chunk) e->invalidate_graph();
     e->graph_chunk = graph_chunk;
     e->invalidate_graph();
     return BLMI_OK;
@@ -399,6 +398,25 @@ int blmi_engine_track(void *h, int turns, int use_graph, void *stream) {
     BLMI_REQUIRE(e, "engine: null handle");
     BLMI_REQUIRE(turns >= 0, "engine: negative turn count");
     return e->track(turns, S(stream), use_graph != 0);
+}
+
+int blmi_engine_set_method(void *h, int method) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    return e->set_method(method);
+}
+
+int blmi_engine_get_method(void *h, int *method) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e && method, "engine: null handle");
+    *method = e->intensity ? e->method : -1;
+    return BLMI_OK;
+}
+
+int blmi_engine_set_hist(void *h, int groups, int max_peel) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    return e->set_hist(groups, max_peel);
 }
 
 int blmi_engine_time_kernels(void *h, int turns, float *ms_induced, float *ms_turn,

@@ -4,10 +4,17 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "blmi_common.h"
 #include "particles.h"
 #include "spectral.h"
 
 namespace blmi {
+
+// LDS budget of the turn kernel's private histogram (u32 per bin).
+constexpr int kLdsHistBytes = 131072;
+// Largest slice count for which the induced voltage uses the direct
+// (Toeplitz) convolution: ns^2 FP64 FMAs spread over all CUs.
+constexpr int kDirectMaxSlices = 8192;
 
 struct Engine {
     int device = 0;
@@ -19,25 +26,30 @@ struct Engine {
     double charge = 1.0;
     double *d_rf = nullptr, *d_acc = nullptr;
     DriftCoeffs<double> *d_drift = nullptr;
-    int *d_turn = nullptr;
-    int host_turn = 0;             // mirror of *d_turn, kept by set_turn/track
-    unsigned *d_ticket = nullptr;
-    // slicing
+    int *d_base = nullptr;         // [0] turn base, [1..2] pending flags of the count buffers
+    int host_turn = 0;             // mirror of the turn base, kept by set_turn/track
+    // slicing: counts double buffered by turn parity
     int n_slices = 0;
     double cut_left = 0, cut_right = 0, inv_bw = 0;
-    unsigned *d_counts = nullptr;  // 2 x n_slices staging counts
+    int max_peel = 0;
+    int groups = 8;                // copies of the counts (same-address atomics / groups)
+    unsigned *d_counts = nullptr;  // [2][groups][n_slices]
     double *profile = nullptr;     // borrowed: f64 profile seen by the caller
     // induced voltage
     bool intensity = false;
+    int64_t n_fft = 0;
+    int method = 0;                // 0: LDS FFT (one workgroup), 1: direct convolution
+    bool have_fft = false, have_direct = false;
     FftPlan plan{};
     double2 *d_Z = nullptr;
+    double *d_gl = nullptr;        // direct method: -scale*irfft(Z), unrolled
     double scale = 0;
     double *volt = nullptr;        // borrowed: induced voltage seen by the caller
     const double *bc = nullptr;    // borrowed: bin centres
     double *d_table = nullptr;     // vk[n_slices], fac[n_slices], (bc0, inv_bw)
     // launch shape (tunable)
-    int block_size = 256;
-    int blocks_per_cu = 8;
+    int block_size = 1024;
+    int blocks_per_cu = 1;
     // graph replay
     int graph_chunk = 50;
     int graph_turns = 0;
@@ -57,13 +69,18 @@ struct Engine {
     int set_slices(int n_slices, double cut_left, double cut_right, double *profile);
     int set_induced(int64_t n_fft, const double *Z_host, double scale, double *volt,
                     const double *bc, double charge);
+    int set_method(int method);
+    int set_hist(int groups, int max_peel);
+    int alloc_counts();
     int set_turn(int turn);
     int get_turn(int *turn);
     int prime(hipStream_t s);
     int track(int turns, hipStream_t user, bool use_graph);
-    int launch_one_turn(hipStream_t s);
-    int launch_turn_kernel(hipStream_t s, bool interp);
-    InducedArgs induced_args() const;
+    int launch_turns(hipStream_t s, int count, bool advance);
+    int finish_chunk(hipStream_t s, int count);
+    int launch_turn_kernel(hipStream_t s, int off, bool interp, const KernelEvents &ev);
+    int launch_induced(hipStream_t s, int off, const KernelEvents &ev);
+    InducedArgs induced_args(int off) const;
     int time_kernels(int turns, hipStream_t user, float *ms_induced, float *ms_turn);
     void invalidate_graph();
 };

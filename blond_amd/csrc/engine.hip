@@ -7,22 +7,29 @@
 //   RF kick(t) -> drift(t+1) -> profile(dt_{t+1}).
 //
 // Here a turn is at most two launches:
-//   induced(t): staged counts of dt_t -> profile (f64, user visible) -> V_t
-//               (user visible) -> interpolation table (slope, intercept)
+//   induced(t): counts of dt_t -> caller's profile, V_t (caller's induced
+//               voltage) and the interpolation table (slope, intercept);
+//               direct Toeplitz convolution over all CUs for moderate slice
+//               counts, or the one-workgroup LDS FFT.
 //   turn(t):    per particle: interp kick, RF kick, drift, bin of dt_{t+1}
-//               into a workgroup-private LDS histogram; the histogram is
-//               flushed to staging counts with device-scope atomics.
+//               counted into a workgroup-private LDS histogram (wavefront-
+//               aggregated increments) and flushed to global counts with
+//               relaxed device-scope atomics.
 // dt and dE are read once and written once per turn (32 B/particle in FP64).
 //
-// Every per-turn scalar (RF voltage/frequency/phase, acceleration kick, drift
-// coefficients) lives in device tables indexed by a device-side turn counter,
-// so a captured graph of K turns replays unchanged for any turn range.  The
-// counter is advanced by the last workgroup of the turn kernel to finish
-// (ticket counter), after every workgroup has read it.
-//
-// Staging counts are double buffered by turn parity: turn(t) accumulates
-// into counts[(t+1)&1] while induced(t) reads counts[t&1] and clears
-// counts[(t+1)&1] beforehand; so no memset node is ever needed.
+// No kernel synchronises workgroups with each other (a device-scope
+// release/acquire pair writes back and invalidates the per-XCD L2 caches);
+// ordering comes from kernel boundaries only:
+//   * the turn index is t = *base + k: k is baked into each launch (and into
+//     the captured graph of a chunk of turns), *base is advanced by a
+//     one-thread kernel after each chunk;
+//   * counts are double buffered by turn parity: turn(t) accumulates into
+//     counts[(t+1)&1]; induced(t+1) reads them and clears counts[t&1] for
+//     turn(t+1); without intensity effects workgroup 0 of turn(t+1) publishes
+//     counts[(t+1)&1] as the caller's profile (and clears them), and every
+//     track() ends with a publish launch for its last turn.
+// Every per-turn scalar lives in device tables indexed by t, so a captured
+// graph replays unchanged for any turn range, and no memset node is recorded.
 #include <stdlib.h>
 #include <string.h>
 
@@ -63,31 +70,67 @@ __device__ __forceinline__ T e_drift(const DriftCoeffs<T> &c, T e, int solver) {
                    T(1));
 }
 
+// Count `bin` (>= 0; < 0 = no count) into the LDS histogram.  Lanes of the
+// wavefront holding the same bin are merged first: up to max_peel times the
+// first pending lane's bin is broadcast, the lanes sharing it are counted
+// with one LDS atomic, then the remaining lanes add one each.  A narrow
+// bunch (few distinct bins per wavefront) thus costs a handful of atomics
+// instead of 64 conflicting ones; a wide one costs at most max_peel extra
+// scalar steps.
+__device__ __forceinline__ void lds_count(unsigned *__restrict__ h, int bin, int max_peel) {
+    unsigned long long pending = __ballot(bin >= 0);
+    const int lane = (int)__lane_id();
+    for (int it = 0; it < max_peel && pending != 0ull; ++it) {
+        const int leader = __builtin_ctzll(pending);
+        const int lb = __builtin_amdgcn_readlane(bin, leader);
+        const unsigned long long same = __ballot(bin == lb) & pending;
+        if (lane == leader) atomicAdd(&h[lb], (unsigned)__popcll(same));
+        pending &= ~same;
+    }
+    if ((pending >> lane) & 1ull) atomicAdd(&h[bin], 1u);
+}
+
 struct TurnArgs {
     int n_rf;
     double charge;
     const double *rf;                  // [turn][3][n_rf]: V, omega, phi
     const double *acc;                 // [turn]
     const DriftCoeffs<double> *drift;  // [turn]
-    int *turn;                         // device turn counter
+    const int *base;                   // device turn base
+    int off;                           // this launch's turn offset
     int last_turn;                     // n_turns - 1
-    unsigned *ticket;                  // zero between launches
     // interpolation kick (intensity effects)
     const double *vk, *fac, *tab_params;
     int n_tab;
-    // slicing
+    // slicing: counts[parity][group][bin]; workgroup b adds into group b % groups
     unsigned *counts0, *counts1;
+    int groups;
     int n_slices;
     double cut_left, inv_bw;
-    // plain mode: last workgroup publishes counts as the f64 profile
+    int max_peel;
+    // plain mode: the previous turn's counts are published by all workgroups
     double *profile;
+    int *pending;                      // [2]: turn whose histogram counts[p] hold, or -1
 };
 
+// profile[b] = sum over groups of c[g][b], and the counts cleared, for b in [b0, b1)
+__device__ __forceinline__ void publish_range(unsigned *__restrict__ c, double *__restrict__ profile,
+                                              int n_slices, int groups, int b0, int b1) {
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) {
+        unsigned s = 0;
+        for (int g = 0; g < groups; ++g) {
+            s += c[(size_t)g * n_slices + b];
+            c[(size_t)g * n_slices + b] = 0u;
+        }
+        profile[b] = (double)s;
+    }
+}
+
 template <int SOLVER, bool INTERP, bool HIST, bool LDSH, bool PUBLISH>
-__global__ void __launch_bounds__(512) turn_kernel(double *__restrict__ dt, double *__restrict__ dE,
-                                                   int64_t n, TurnArgs a) {
+__global__ void __launch_bounds__(1024) turn_kernel(double *__restrict__ dt, double *__restrict__ dE,
+                                                    int64_t n, TurnArgs a) {
     extern __shared__ unsigned lds_hist[];
-    int t = *a.turn;
+    int t = *a.base + a.off;
     if (t < 0) t = 0;
     if (t > a.last_turn) t = a.last_turn;  // host checks the range; never read past the tables
     const double *V = a.rf + (size_t)t * 3 * a.n_rf;
@@ -95,8 +138,20 @@ __global__ void __launch_bounds__(512) turn_kernel(double *__restrict__ dt, doub
     const double *phi = w + a.n_rf;
     const double acc = a.acc[t];
     const DriftCoeffs<double> dc = a.drift[t + 1];
-    unsigned *counts = ((t + 1) & 1) ? a.counts1 : a.counts0;
-    if (PUBLISH) counts = a.counts0;
+    unsigned *counts = (((t + 1) & 1) ? a.counts1 : a.counts0) +
+                       (size_t)(blockIdx.x % a.groups) * a.n_slices;
+    if (PUBLISH) {
+        // counts[t&1] hold the histogram of dt_t when the previous turn ran in
+        // this chunk: every workgroup publishes its share of the bins
+        if (a.pending[t & 1] == t) {
+            unsigned *prev = (t & 1) ? a.counts1 : a.counts0;
+            const int per = (a.n_slices + (int)gridDim.x - 1) / (int)gridDim.x;
+            const int b0 = (int)blockIdx.x * per;
+            const int b1 = min(b0 + per, a.n_slices);
+            publish_range(prev, a.profile, a.n_slices, a.groups, b0, b1);
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.pending[(t + 1) & 1] = t + 1;
+    }
     double bc0 = 0.0, ibw = 0.0;
     if (INTERP) {
         bc0 = a.tab_params[0];
@@ -106,7 +161,7 @@ __global__ void __launch_bounds__(512) turn_kernel(double *__restrict__ dt, doub
         for (int b = threadIdx.x; b < a.n_slices; b += blockDim.x) lds_hist[b] = 0u;
         __syncthreads();
     }
-    auto step = [&](double &x, double &e) {
+    auto step = [&](double &x, double &e) -> int {
         if (INTERP) {
             const double f = floor((x - bc0) * ibw);
             if (f >= 0.0 && f < (double)a.n_tab) {
@@ -118,29 +173,40 @@ __global__ void __launch_bounds__(512) turn_kernel(double *__restrict__ dt, doub
         x = x + e_drift(dc, e, SOLVER);
         if (HIST) {
             const double f = floor((x - a.cut_left) * a.inv_bw);
-            if (f >= 0.0 && f < (double)a.n_slices) {
-                if (LDSH)
-                    atomicAdd(&lds_hist[(int)f], 1u);
-                else
-                    atomic_add_agent(&counts[(int)f], 1u);
-            }
+            return (f >= 0.0 && f < (double)a.n_slices) ? (int)f : -1;
         }
+        return -1;
+    };
+    auto count = [&](int bin) {
+        if (!HIST) return;
+        if (LDSH)
+            lds_count(lds_hist, bin, a.max_peel);
+        else if (bin >= 0)
+            atomic_add_agent(&counts[bin], 1u);
     };
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t n2 = n / 2;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) {
         double2 x = reinterpret_cast<const double2 *>(dt)[i];
         double2 e = reinterpret_cast<const double2 *>(dE)[i];
-        step(x.x, e.x);
-        step(x.y, e.y);
+        const int b0 = step(x.x, e.x);
+        const int b1 = step(x.y, e.y);
         reinterpret_cast<double2 *>(dt)[i] = x;
         reinterpret_cast<double2 *>(dE)[i] = e;
+        count(b0);
+        count(b1);
     }
-    if ((n & 1) && blockIdx.x == gridDim.x - 1 && threadIdx.x == blockDim.x - 1) {
-        double x = dt[n - 1], e = dE[n - 1];
-        step(x, e);
-        dt[n - 1] = x;
-        dE[n - 1] = e;
+    if ((n & 1) && blockIdx.x == gridDim.x - 1 && threadIdx.x < 64) {
+        // odd tail: the first wavefront of the last block takes part in the
+        // aggregated count, lane 0 owns the particle
+        int b = -1;
+        if (threadIdx.x == 0) {
+            double x = dt[n - 1], e = dE[n - 1];
+            b = step(x, e);
+            dt[n - 1] = x;
+            dE[n - 1] = e;
+        }
+        count(b);
     }
     if (HIST && LDSH) {
         __syncthreads();
@@ -149,39 +215,28 @@ __global__ void __launch_bounds__(512) turn_kernel(double *__restrict__ dt, doub
             if (c) atomic_add_agent(&counts[b], c);
         }
     }
-    // ticket: the last workgroup to finish advances the turn (and in plain
-    // mode publishes the profile and clears the counts)
-    __shared__ int is_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __atomic_thread_fence(__ATOMIC_RELEASE);  // this block's count updates first
-        const unsigned prev = __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_ACQ_REL,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (is_last) {
-        __atomic_thread_fence(__ATOMIC_ACQUIRE);
-        if (PUBLISH) {
-            for (int b = threadIdx.x; b < a.n_slices; b += blockDim.x) {
-                const unsigned c = __hip_atomic_load(&counts[b], __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-                a.profile[b] = (double)c;
-                __hip_atomic_store(&counts[b], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-        if (threadIdx.x == 0) {
-            __hip_atomic_store(a.turn, t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
 }
 
-// Histogram of the current dt into counts[turn & 1] (engine priming).
-__global__ void prime_hist_kernel(const double *__restrict__ dt, int64_t n, const int *turn,
+// Plain mode, end of a track() call: publish the last turn's counts (one
+// workgroup, so the pending flag is cleared after every bin is done).
+__global__ void publish_kernel(const int *base, int off, unsigned *counts0, unsigned *counts1,
+                               int groups, int *pending, double *profile, int n_slices) {
+    const int t = *base + off;  // histogram of dt_t
+    if (pending[t & 1] != t) return;
+    publish_range((t & 1) ? counts1 : counts0, profile, n_slices, groups, 0, n_slices);
+    __syncthreads();
+    if (threadIdx.x == 0) pending[t & 1] = -1;
+}
+
+// *base += k, after every launch of the chunk has read it (stream order).
+__global__ void advance_kernel(int *base, int k) { *base += k; }
+
+// Histogram of the current coordinates into counts[t & 1] (engine priming);
+// both buffers are cleared first by fill kernels.
+__global__ void prime_hist_kernel(const double *__restrict__ dt, int64_t n, const int *base,
                                   unsigned *counts0, unsigned *counts1, int n_slices,
                                   double cut_left, double inv_bw) {
-    const int t = *turn;
+    const int t = *base;
     unsigned *counts = (t & 1) ? counts1 : counts0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -196,24 +251,19 @@ __global__ void prime_hist_kernel(const double *__restrict__ dt, int64_t n, cons
 Engine::~Engine() { release(); }
 
 void Engine::release() {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    graph_exec = nullptr;
-    graph = nullptr;
+    invalidate_graph();
     (void)hipFree(d_rf);
     (void)hipFree(d_acc);
     (void)hipFree(d_drift);
-    (void)hipFree(d_turn);
-    (void)hipFree(d_ticket);
+    (void)hipFree(d_base);
     (void)hipFree(d_counts);
     (void)hipFree(d_table);
     (void)hipFree(d_Z);
-    d_rf = d_acc = nullptr;
+    (void)hipFree(d_gl);
+    d_rf = d_acc = d_table = d_gl = nullptr;
     d_drift = nullptr;
-    d_turn = nullptr;
-    d_ticket = nullptr;
+    d_base = nullptr;
     d_counts = nullptr;
-    d_table = nullptr;
     d_Z = nullptr;
     if (own_stream) (void)hipStreamDestroy(own_stream);
     own_stream = nullptr;
@@ -231,10 +281,10 @@ int Engine::init(int64_t n_particles, int n_rf_, int n_turns_, int solver_) {
     BLMI_HIP_TRY(hipMalloc(&d_rf, sizeof(double) * 3 * n_rf * (size_t)(n_turns + 1)));
     BLMI_HIP_TRY(hipMalloc(&d_acc, sizeof(double) * (size_t)(n_turns + 1)));
     BLMI_HIP_TRY(hipMalloc(&d_drift, sizeof(DriftCoeffs<double>) * (size_t)(n_turns + 1)));
-    BLMI_HIP_TRY(hipMalloc(&d_turn, sizeof(int)));
-    BLMI_HIP_TRY(hipMalloc(&d_ticket, sizeof(unsigned)));
-    BLMI_HIP_TRY(hipMemset(d_turn, 0, sizeof(int)));
-    BLMI_HIP_TRY(hipMemset(d_ticket, 0, sizeof(unsigned)));
+    // d_base[0] = turn base, d_base[1..2] = turn held by each count buffer (-1: none)
+    BLMI_HIP_TRY(hipMalloc(&d_base, sizeof(int) * 4));
+    BLMI_HIP_TRY(hipMemset(d_base, 0, sizeof(int)));
+    BLMI_HIP_TRY(hipMemset(d_base + 1, 0xff, sizeof(int) * 2));
     BLMI_HIP_TRY(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
     BLMI_HIP_TRY(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
     BLMI_HIP_TRY(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
@@ -266,33 +316,102 @@ int Engine::set_slices(int n_slices_, double cut_left_, double cut_right_, doubl
     cut_right = cut_right_;
     inv_bw = (double)n_slices / (cut_right - cut_left);  // histogram.cpp:29
     profile = profile_;
+    invalidate_graph();
+    return alloc_counts();
+}
+
+int Engine::alloc_counts() {
     (void)hipFree(d_counts);
     d_counts = nullptr;
     if (n_slices > 0) {
-        BLMI_HIP_TRY(hipMalloc(&d_counts, sizeof(unsigned) * 2 * (size_t)n_slices));
-        BLMI_HIP_TRY(hipMemset(d_counts, 0, sizeof(unsigned) * 2 * (size_t)n_slices));
+        const size_t bytes = sizeof(unsigned) * 2 * (size_t)groups * (size_t)n_slices;
+        BLMI_HIP_TRY(hipMalloc(&d_counts, bytes));
+        BLMI_HIP_TRY(hipMemset(d_counts, 0, bytes));
     }
-    invalidate_graph();
+    BLMI_HIP_TRY(hipMemset(d_base + 1, 0xff, sizeof(int) * 2));
     return BLMI_OK;
 }
 
-int Engine::set_induced(int64_t n_fft, const double *Z_host, double scale_, double *volt_,
+int Engine::set_induced(int64_t n_fft_, const double *Z_host, double scale_, double *volt_,
                         const double *bc_, double charge_) {
-    int rc = fft_plan_get(n_fft, &plan);
-    if (rc) return rc;
+    n_fft = n_fft_;
     scale = scale_;
     volt = volt_;
     bc = bc_;
     charge = charge_;
-    const size_t nz = (size_t)plan.M + 1;
+    const int64_t nz = n_fft / 2 + 1;
     (void)hipFree(d_Z);
     (void)hipFree(d_table);
-    BLMI_HIP_TRY(hipMalloc(&d_Z, sizeof(double2) * nz));
-    BLMI_HIP_TRY(hipMemcpy(d_Z, Z_host, sizeof(double2) * nz, hipMemcpyHostToDevice));
+    (void)hipFree(d_gl);
+    d_Z = nullptr;
+    d_table = d_gl = nullptr;
+    BLMI_HIP_TRY(hipMalloc(&d_Z, sizeof(double2) * (size_t)nz));
+    BLMI_HIP_TRY(hipMemcpy(d_Z, Z_host, sizeof(double2) * (size_t)nz, hipMemcpyHostToDevice));
     BLMI_HIP_TRY(hipMalloc(&d_table, sizeof(double) * (2 * (size_t)n_slices + 2)));
     BLMI_HIP_TRY(hipMemset(d_table, 0, sizeof(double) * (2 * (size_t)n_slices + 2)));
+    have_fft = fft_plan_get(n_fft, &plan) == BLMI_OK;
+    // the direct method's wake kernel g = -scale * irfft(Z) needs one
+    // transform of length n_fft at set-up
+    have_direct = false;
+    if (have_fft && n_slices <= kDirectMaxSlices) {
+        double *h = nullptr;
+        BLMI_HIP_TRY(hipMalloc(&h, sizeof(double) * (size_t)n_fft));
+        int rc = launch_irfft(d_Z, nz, h, n_fft, plan, nullptr);
+        std::vector<double> hh((size_t)n_fft);
+        if (rc == BLMI_OK) {
+            hipError_t e = hipMemcpy(hh.data(), h, sizeof(double) * (size_t)n_fft,
+                                     hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = blmi_set_error(BLMI_ERR_HIP, "wake copy: %s", hipGetErrorString(e));
+        }
+        (void)hipFree(h);
+        if (rc) return rc;
+        // gl[k] = g[(k - (ns-1)) mod n_fft], k in [0, 2 ns - 1), zero padded
+        const size_t len = 2 * (size_t)n_slices + kConvPad;
+        std::vector<double> gl(len, 0.0);
+        for (int64_t k = 0; k < 2 * (int64_t)n_slices - 1; ++k) {
+            int64_t idx = (k - (n_slices - 1)) % n_fft;
+            if (idx < 0) idx += n_fft;
+            gl[(size_t)k] = -(scale * hh[(size_t)idx]);
+        }
+        BLMI_HIP_TRY(hipMalloc(&d_gl, sizeof(double) * len));
+        BLMI_HIP_TRY(hipMemcpy(d_gl, gl.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+        have_direct = true;
+    }
+    if (!have_fft && !have_direct)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED,
+                              "induced voltage: n_fft=%lld is not supported by the LDS transform",
+                              (long long)n_fft);
+    method = have_direct ? 1 : 0;
     intensity = true;
     invalidate_graph();
+    return BLMI_OK;
+}
+
+int Engine::set_method(int method_) {
+    if (method_ == 1 && !have_direct)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "direct convolution not available (n_slices %d)",
+                              n_slices);
+    if (method_ == 0 && !have_fft)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "LDS FFT not available (n_fft %lld)",
+                              (long long)n_fft);
+    if (method_ != 0 && method_ != 1) return blmi_set_error(BLMI_ERR_ARG, "method must be 0 or 1");
+    method = method_;
+    invalidate_graph();
+    return BLMI_OK;
+}
+
+int Engine::set_hist(int groups_, int max_peel_) {
+    if (groups_ < 1 || groups_ > kMaxCountGroups)
+        return blmi_set_error(BLMI_ERR_ARG, "groups in [1, %d]", kMaxCountGroups);
+    if (max_peel_ < 0 || max_peel_ > 64) return blmi_set_error(BLMI_ERR_ARG, "max_peel in [0, 64]");
+    max_peel = max_peel_;
+    invalidate_graph();
+    if (groups_ != groups) {
+        // the staged counts of the current turn are dropped: the caller
+        // primes again (FusedTracker does so on its next track())
+        groups = groups_;
+        return alloc_counts();
+    }
     return BLMI_OK;
 }
 
@@ -307,31 +426,39 @@ void Engine::invalidate_graph() {
 int Engine::set_turn(int turn) {
     if (turn < 0 || turn >= n_turns)
         return blmi_set_error(BLMI_ERR_ARG, "turn %d outside [0, %d)", turn, n_turns);
-    BLMI_HIP_TRY(hipMemcpy(d_turn, &turn, sizeof(int), hipMemcpyHostToDevice));
+    BLMI_HIP_TRY(hipMemcpy(d_base, &turn, sizeof(int), hipMemcpyHostToDevice));
     host_turn = turn;
+    // staged counts refer to the old turn: drop them (prime rebuilds them)
+    if (d_counts)
+        BLMI_HIP_TRY(hipMemset(d_counts, 0,
+                               sizeof(unsigned) * 2 * (size_t)groups * (size_t)n_slices));
+    BLMI_HIP_TRY(hipMemset(d_base + 1, 0xff, sizeof(int) * 2));
     return BLMI_OK;
 }
 
 int Engine::get_turn(int *turn) {
-    BLMI_HIP_TRY(hipMemcpy(turn, d_turn, sizeof(int), hipMemcpyDeviceToHost));
+    BLMI_HIP_TRY(hipMemcpy(turn, d_base, sizeof(int), hipMemcpyDeviceToHost));
     return BLMI_OK;
 }
 
 int Engine::prime(hipStream_t s) {
+    // counts of the current coordinates (the first pass of the map's
+    // Profile.track()) into counts[t & 1]; the other buffer cleared
     if (!intensity || n_slices <= 0) return BLMI_OK;
-    int rc = launch_fill<unsigned>(d_counts, 2 * (int64_t)n_slices, 0u, s);
+    int rc = launch_fill<unsigned>(d_counts, 2 * (int64_t)groups * n_slices, 0u, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(prime_hist_kernel, dim3(stream_grid(n, 256)), dim3(256), 0, s, dt, n, d_turn,
-                       d_counts, d_counts + n_slices, n_slices, cut_left, inv_bw);
+    hipLaunchKernelGGL(prime_hist_kernel, dim3(stream_grid(n, 256)), dim3(256), 0, s, dt, n, d_base,
+                       d_counts, d_counts + (size_t)groups * n_slices, n_slices, cut_left, inv_bw);
     return blmi_check_launch("engine_prime");
 }
 
 template <int SOLVER>
-static int launch_turn_solver(const Engine &e, const TurnArgs &a, bool interp, bool hist, bool ldsh,
-                              bool publish, int grid, int block, size_t lds, hipStream_t s) {
-#define BLMI_TURN(I, H, L, P)                                                                   \
-    hipLaunchKernelGGL((turn_kernel<SOLVER, I, H, L, P>), dim3(grid), dim3(block), lds, s, e.dt, \
-                       e.dE, e.n, a)
+static void launch_turn_solver(const Engine &e, const TurnArgs &a, bool interp, bool hist,
+                               bool ldsh, int grid, int block, size_t lds, hipStream_t s,
+                               const KernelEvents &ev) {
+#define BLMI_TURN(I, H, L, P)                                                                  \
+    launch(turn_kernel<SOLVER, I, H, L, P>, dim3(grid), dim3(block), lds, s, ev, e.dt, e.dE, e.n, \
+           a)
     if (interp) {
         if (hist) {
             if (ldsh) BLMI_TURN(true, true, true, false); else BLMI_TURN(true, true, false, false);
@@ -340,28 +467,23 @@ static int launch_turn_solver(const Engine &e, const TurnArgs &a, bool interp, b
         }
     } else {
         if (hist) {
-            if (publish) {
-                if (ldsh) BLMI_TURN(false, true, true, true); else BLMI_TURN(false, true, false, true);
-            } else {
-                if (ldsh) BLMI_TURN(false, true, true, false); else BLMI_TURN(false, true, false, false);
-            }
+            if (ldsh) BLMI_TURN(false, true, true, true); else BLMI_TURN(false, true, false, true);
         } else {
             BLMI_TURN(false, false, false, false);
         }
     }
 #undef BLMI_TURN
-    return blmi_check_launch("turn");
 }
 
-InducedArgs Engine::induced_args() const {
-    // staged counts of dt_t -> profile, V_t and the interpolation table; the
-    // kernel picks counts[t&1] from the device turn counter and clears them
-    // after reading (turn(t) accumulates into the other buffer)
+InducedArgs Engine::induced_args(int off) const {
     InducedArgs ia{};
-    ia.turn = d_turn;
+    ia.src = nullptr;
+    ia.turn_base = d_base;
+    ia.turn_off = off;
     ia.counts0 = d_counts;
-    ia.counts1 = d_counts + n_slices;
-    ia.profile = profile;
+    ia.counts1 = d_counts + (size_t)groups * n_slices;
+    ia.groups = groups;
+    ia.profile_out = profile;
     ia.n_slices = n_slices;
     ia.Z = reinterpret_cast<const double2 *>(d_Z);
     ia.scale = scale;
@@ -372,10 +494,17 @@ InducedArgs Engine::induced_args() const {
     ia.vk = d_table;
     ia.fac = d_table + n_slices;
     ia.tab_params = d_table + 2 * n_slices;
+    ia.gl = d_gl;
     return ia;
 }
 
-int Engine::launch_turn_kernel(hipStream_t s, bool interp) {
+int Engine::launch_induced(hipStream_t s, int off, const KernelEvents &ev) {
+    InducedArgs ia = induced_args(off);
+    if (method == 1) return launch_induced_direct(ia, s, ev);
+    return launch_induced_voltage_lds(ia, plan, s, ev);
+}
+
+int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const KernelEvents &ev) {
     const bool hist = n_slices > 0 && d_counts != nullptr;
     TurnArgs a{};
     a.n_rf = n_rf;
@@ -383,81 +512,111 @@ int Engine::launch_turn_kernel(hipStream_t s, bool interp) {
     a.rf = d_rf;
     a.acc = d_acc;
     a.drift = d_drift;
-    a.turn = d_turn;
+    a.base = d_base;
+    a.off = off;
     a.last_turn = n_turns - 1;
-    a.ticket = d_ticket;
     a.vk = d_table;
     a.fac = d_table ? d_table + n_slices : nullptr;
     a.tab_params = d_table ? d_table + 2 * n_slices : nullptr;
     a.n_tab = n_slices - 1;
     a.counts0 = d_counts;
-    a.counts1 = d_counts ? d_counts + n_slices : nullptr;
+    a.counts1 = d_counts ? d_counts + (size_t)groups * n_slices : nullptr;
+    a.groups = groups;
     a.n_slices = n_slices;
     a.cut_left = cut_left;
     a.inv_bw = inv_bw;
+    a.max_peel = max_peel;
     a.profile = profile;
+    a.pending = d_base + 1;
     const size_t lds_bytes = (size_t)n_slices * sizeof(unsigned);
-    const bool ldsh = hist && lds_bytes <= 65536;
+    const bool ldsh = hist && lds_bytes <= (size_t)kLdsHistBytes;
     const int block = block_size;
     int grid = device_cu_count() * blocks_per_cu;
     const int64_t need = (n / 2 + block - 1) / block;
     if (grid > need) grid = (int)(need > 0 ? need : 1);
     const size_t lds = ldsh ? lds_bytes : 0;
-    const bool publish = hist && !interp;
-    switch (solver) {
-        case 0: return launch_turn_solver<0>(*this, a, interp, hist, ldsh, publish, grid, block, lds, s);
-        case 1: return launch_turn_solver<1>(*this, a, interp, hist, ldsh, publish, grid, block, lds, s);
-        default: return launch_turn_solver<2>(*this, a, interp, hist, ldsh, publish, grid, block, lds, s);
+    if (lds > 65536) {
+        hipError_t e = hipSuccess;
+#define BLMI_ATTR(S, I, P)                                                                       \
+    if (e == hipSuccess)                                                                         \
+        e = hipFuncSetAttribute((const void *)turn_kernel<S, I, true, true, P>,                  \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        BLMI_ATTR(0, true, false) BLMI_ATTR(1, true, false) BLMI_ATTR(2, true, false)
+        BLMI_ATTR(0, false, true) BLMI_ATTR(1, false, true) BLMI_ATTR(2, false, true)
+#undef BLMI_ATTR
+        if (e != hipSuccess)
+            return blmi_set_error(BLMI_ERR_HIP, "turn kernel LDS attribute: %s", hipGetErrorString(e));
     }
+    switch (solver) {
+        case 0: launch_turn_solver<0>(*this, a, interp, hist, ldsh, grid, block, lds, s, ev); break;
+        case 1: launch_turn_solver<1>(*this, a, interp, hist, ldsh, grid, block, lds, s, ev); break;
+        default: launch_turn_solver<2>(*this, a, interp, hist, ldsh, grid, block, lds, s, ev); break;
+    }
+    return blmi_check_launch("turn");
 }
 
-int Engine::launch_one_turn(hipStream_t s) {
-    if (intensity) {
-        InducedArgs ia = induced_args();
-        int rc = launch_induced_voltage_lds(ia, plan, s);
+// Turns base+off0 .. base+off0+count-1, then (if advance) *base += count and,
+// without intensity effects, the publish launch for the last turn.
+int Engine::launch_turns(hipStream_t s, int count, bool advance) {
+    const KernelEvents none;
+    for (int k = 0; k < count; ++k) {
+        if (intensity) {
+            int rc = launch_induced(s, k, none);
+            if (rc) return rc;
+        }
+        int rc = launch_turn_kernel(s, k, intensity, none);
         if (rc) return rc;
     }
-    return launch_turn_kernel(s, intensity);
+    if (advance) return finish_chunk(s, count);
+    return BLMI_OK;
+}
+
+int Engine::finish_chunk(hipStream_t s, int count) {
+    if (!intensity && n_slices > 0 && d_counts) {
+        hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(1024), 0, s, d_base, count, d_counts,
+                           d_counts + (size_t)groups * n_slices, groups, d_base + 1, profile,
+                           n_slices);
+        int rc = blmi_check_launch("publish");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, d_base, count);
+    return blmi_check_launch("advance");
 }
 
 int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *ms_turn) {
-    // Eager turns with HIP events around each kernel on the engine's stream:
-    // average device duration of the induced-voltage and the particle kernels.
+    // Eager turns; each kernel launched with start/stop events recorded by its
+    // own dispatch: average device duration of the induced-voltage kernel and
+    // of the particle kernel.
     if (turns <= 0) return blmi_set_error(BLMI_ERR_ARG, "time_kernels: turns must be positive");
     if (host_turn + turns > n_turns)
         return blmi_set_error(BLMI_ERR_ARG, "time_kernels: %d turns from turn %d passes n_turns=%d",
                               turns, host_turn, n_turns);
     BLMI_HIP_TRY(hipEventRecord(ev_in, user));
     BLMI_HIP_TRY(hipStreamWaitEvent(own_stream, ev_in, 0));
-    std::vector<hipEvent_t> ev(3 * (size_t)turns, nullptr);
+    std::vector<hipEvent_t> ev(4 * (size_t)turns, nullptr);
     for (auto &x : ev) BLMI_HIP_TRY(hipEventCreate(&x));
     int rc = BLMI_OK;
-    const bool was_intensity = intensity;
     for (int k = 0; k < turns && rc == BLMI_OK; ++k) {
-        BLMI_HIP_TRY(hipEventRecord(ev[3 * k], own_stream));
-        if (was_intensity) {
-            InducedArgs ia = induced_args();
-            rc = launch_induced_voltage_lds(ia, plan, own_stream);
-            if (rc) break;
-        }
-        BLMI_HIP_TRY(hipEventRecord(ev[3 * k + 1], own_stream));
-        rc = launch_turn_kernel(own_stream, was_intensity);
-        BLMI_HIP_TRY(hipEventRecord(ev[3 * k + 2], own_stream));
+        KernelEvents ei{ev[4 * k], ev[4 * k + 1]}, et{ev[4 * k + 2], ev[4 * k + 3]};
+        if (intensity) rc = launch_induced(own_stream, k, ei);
+        if (rc == BLMI_OK) rc = launch_turn_kernel(own_stream, k, intensity, et);
     }
+    if (rc == BLMI_OK) rc = finish_chunk(own_stream, turns);
     if (rc == BLMI_OK) {
-        BLMI_HIP_TRY(hipEventSynchronize(ev[3 * (size_t)turns - 1]));
+        host_turn += turns;
+        BLMI_HIP_TRY(hipStreamSynchronize(own_stream));
         double a = 0, b = 0;
         for (int k = 0; k < turns; ++k) {
             float x = 0, y = 0;
-            BLMI_HIP_TRY(hipEventElapsedTime(&x, ev[3 * k], ev[3 * k + 1]));
-            BLMI_HIP_TRY(hipEventElapsedTime(&y, ev[3 * k + 1], ev[3 * k + 2]));
+            if (intensity) BLMI_HIP_TRY(hipEventElapsedTime(&x, ev[4 * k], ev[4 * k + 1]));
+            BLMI_HIP_TRY(hipEventElapsedTime(&y, ev[4 * k + 2], ev[4 * k + 3]));
             a += x;
             b += y;
         }
         *ms_induced = (float)(a / turns);
         *ms_turn = (float)(b / turns);
-        host_turn += turns;
     }
+    BLMI_HIP_TRY(hipStreamSynchronize(own_stream));
     for (auto &x : ev) (void)hipEventDestroy(x);
     BLMI_HIP_TRY(hipEventRecord(ev_out, own_stream));
     BLMI_HIP_TRY(hipStreamWaitEvent(user, ev_out, 0));
@@ -475,12 +634,12 @@ int Engine::track(int turns, hipStream_t user, bool use_graph) {
     BLMI_HIP_TRY(hipEventRecord(ev_in, user));
     BLMI_HIP_TRY(hipStreamWaitEvent(own_stream, ev_in, 0));
     int done = 0;
+    int rc = BLMI_OK;
     if (use_graph && turns >= graph_chunk) {
         if (!graph_exec || graph_turns != graph_chunk) {
             invalidate_graph();
             BLMI_HIP_TRY(hipStreamBeginCapture(own_stream, hipStreamCaptureModeThreadLocal));
-            int rc = BLMI_OK;
-            for (int k = 0; k < graph_chunk && rc == BLMI_OK; ++k) rc = launch_one_turn(own_stream);
+            rc = launch_turns(own_stream, graph_chunk, true);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(own_stream, &g);
             if (rc) {
@@ -498,10 +657,8 @@ int Engine::track(int turns, hipStream_t user, bool use_graph) {
             done += graph_turns;
         }
     }
-    for (; done < turns; ++done) {
-        int rc = launch_one_turn(own_stream);
-        if (rc) return rc;
-    }
+    if (done < turns) rc = launch_turns(own_stream, turns - done, true);
+    if (rc) return rc;
     host_turn += turns;
     BLMI_HIP_TRY(hipEventRecord(ev_out, own_stream));
     BLMI_HIP_TRY(hipStreamWaitEvent(user, ev_out, 0));

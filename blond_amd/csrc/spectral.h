@@ -4,9 +4,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "blmi_common.h"
+
 namespace blmi {
 
 constexpr int kMaxStages = 24;
+// zero padding after the direct method's unrolled wake kernel
+constexpr int kConvPad = 64;
+// copies of the global histogram counts (workgroup b adds into copy b % groups)
+constexpr int kMaxCountGroups = 16;
 
 // Complex FFT of length M = prod(radix[s]), radices in {2,3,4,5,8}, done as a
 // Stockham autosort sequence inside one workgroup's LDS.
@@ -29,15 +35,21 @@ int launch_rfft(const double *in, int64_t n_in, double2 *out, const FftPlan &p, 
 int launch_irfft(const double2 *in, int64_t m_in, double *out, int64_t n_out, const FftPlan &p,
                  hipStream_t s);
 
-// One-turn induced voltage (impedance.py:457-489 + 113-131) on one workgroup:
-// profile -> rfft(n_fft) -> * Z -> irfft -> -(scale * x)[:n_slices] -> V.
-// When counts != nullptr the profile is read from u32 counts, written to
-// `profile` as real values, and the counts are cleared.  When vk != nullptr
-// the linear_interp_kick table of V (charge, bin_centers) is built too.
+// One-turn induced voltage (impedance.py:457-489 + 113-131):
+// profile -> rfft(n_fft) -> * Z -> irfft -> -(scale * x)[:n_out] -> V.
+// `src` is the profile read; when profile_out != nullptr it is also copied
+// there (the engine's staged profile becomes the caller's).  When
+// vk != nullptr the linear_interp_kick table of V (charge, bin_centers) is
+// built as well (linear_interp_kick.cpp:33-48, acceleration kick 0).
 struct InducedArgs {
-    const int *turn;            // optional device turn counter selecting
-    unsigned *counts0, *counts1;  // counts[turn & 1] (nullptr: read `profile`)
-    double *profile;
+    const double *src;            // profile (nullptr: read the parity counts below)
+    // engine path: t = *turn_base + turn_off; the profile is counts[t & 1]
+    // and counts[(t + 1) & 1] are cleared for the next particle pass
+    const int *turn_base;
+    int turn_off;
+    unsigned *counts0, *counts1;  // each [groups][n_slices]; the profile is the sum
+    int groups;
+    double *profile_out;
     int n_slices;
     const double2 *Z;
     double scale;     // charge * e * ratio (sign applied inside)
@@ -46,8 +58,16 @@ struct InducedArgs {
     const double *bc; // bin centres (table only)
     double charge;
     double *vk, *fac, *tab_params;  // table outputs (optional)
+    const double *gl; // direct method: g[(k - (ns-1)) mod n_fft], g = -scale * irfft(Z)
 };
-int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream_t s);
+// Single workgroup: LDS Stockham rfft, impedance product, irfft.
+int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream_t s,
+                               const KernelEvents &ev = KernelEvents());
+// All CUs: V[i] = sum_j p[j] g[(i - j) mod n_fft], the same linear map as the
+// FFT route (Z[0] = 0 and the c2r Nyquist convention included in g).  Only
+// n_out == n_slices.
+int launch_induced_direct(const InducedArgs &a, hipStream_t s,
+                          const KernelEvents &ev = KernelEvents());
 
 int launch_fast_resonator(double *re, double *im, const double *freq, const double *R,
                           const double *Q, const double *fr, int n_res, int n_f, hipStream_t s);

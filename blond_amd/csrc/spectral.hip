@@ -251,18 +251,34 @@ __global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
                                    const double2 *__restrict__ tw, const double2 *__restrict__ tw2) {
     extern __shared__ double2 buf[];
     const int ns = a.n_slices;
-    unsigned *counts = nullptr;
-    if (a.counts0) counts = (a.turn && (*a.turn & 1)) ? a.counts1 : a.counts0;
+    const unsigned *cin = nullptr;
+    unsigned *cclr = nullptr;
+    if (a.counts0) {
+        const int t = *a.turn_base + a.turn_off;
+        cin = (t & 1) ? a.counts1 : a.counts0;
+        cclr = (t & 1) ? a.counts0 : a.counts1;
+    }
     // load the profile as z[m] = p[2m] + i p[2m+1], zero padded to n = 2M
+    auto prof = [&](int j) -> double {
+        if (!cin) return a.src[j];
+        unsigned s = 0;
+        for (int g = 0; g < a.groups; ++g) s += cin[(size_t)g * ns + j];
+        return (double)s;
+    };
     for (int m = threadIdx.x; m < M; m += blockDim.x) {
         double p0 = 0.0, p1 = 0.0;
         const int j0 = 2 * m, j1 = j0 + 1;
-        if (counts) {
-            if (j0 < ns) { p0 = (double)counts[j0]; a.profile[j0] = p0; counts[j0] = 0u; }
-            if (j1 < ns) { p1 = (double)counts[j1]; a.profile[j1] = p1; counts[j1] = 0u; }
-        } else {
-            if (j0 < ns) p0 = a.profile[j0];
-            if (j1 < ns) p1 = a.profile[j1];
+        if (j0 < ns) p0 = prof(j0);
+        if (j1 < ns) p1 = prof(j1);
+        if (a.profile_out) {
+            if (j0 < ns) a.profile_out[j0] = p0;
+            if (j1 < ns) a.profile_out[j1] = p1;
+        }
+        if (cclr) {
+            for (int g = 0; g < a.groups; ++g) {
+                if (j0 < ns) cclr[(size_t)g * ns + j0] = 0u;
+                if (j1 < ns) cclr[(size_t)g * ns + j1] = 0u;
+            }
         }
         buf[m] = make_double2(p0, p1);
     }
@@ -327,6 +343,145 @@ __global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
             a.fac[i] = (a.charge * vl[i] - a.bc[i] * k) + 0.0;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// one-turn induced voltage by direct convolution over all CUs
+//   V[i] = sum_m q[m] gl[i + m],  q[m] = p[ns-1-m],  gl[k] = g[(k-(ns-1)) mod n_fft]
+// Each workgroup produces kConvC outputs plus the next one (needed by the
+// last table entry).  The profile (reversed) and the workgroup's window of gl
+// are staged in LDS with coalesced loads; lane l then takes samples
+// m = l, l + 256, ... so that every LDS read of a wavefront hits consecutive
+// words, and the partial sums are reduced in a fixed order (deterministic).
+// LDS: (2 ns + kConvC) doubles.
+// ---------------------------------------------------------------------------
+constexpr int kConvC = 16;
+constexpr int kConvThreads = 256;
+
+__global__ void __launch_bounds__(kConvThreads) induced_direct_kernel(InducedArgs a) {
+    extern __shared__ double conv_lds[];
+    __shared__ double red[kConvThreads / 64][kConvC + 1];
+    __shared__ double vout[kConvC + 1];
+    const int ns = a.n_slices;
+    const int i0 = blockIdx.x * kConvC;
+    const int lim = 2 * ns - 1;
+    const unsigned *cin = nullptr;
+    unsigned *cclr = nullptr;
+    if (a.counts0) {
+        const int t = *a.turn_base + a.turn_off;
+        cin = (t & 1) ? a.counts1 : a.counts0;
+        cclr = (t & 1) ? a.counts0 : a.counts1;
+    }
+    double *q = conv_lds;        // q[m] = p[ns - 1 - m]
+    double *wv = conv_lds + ns;  // wv[k] = gl[i0 + k], k < ns + kConvC
+    // staging: every thread issues all its global loads before its LDS
+    // stores, so the loads of a pass are in flight together
+    constexpr int kStage = 8;
+    const int T = blockDim.x;
+    for (int b0 = 0; b0 < ns; b0 += T * kStage) {
+        double v[kStage];
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int m = b0 + u * T + (int)threadIdx.x;
+            v[u] = 0.0;
+            if (m < ns) {
+                const int j = ns - 1 - m;
+                if (cin) {
+                    unsigned s = 0;
+#pragma unroll
+                    for (int g = 0; g < kMaxCountGroups; ++g)
+                        if (g < a.groups) s += cin[(size_t)g * ns + j];
+                    v[u] = (double)s;
+                } else {
+                    v[u] = a.src[j];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int m = b0 + u * T + (int)threadIdx.x;
+            if (m < ns) q[m] = v[u];
+        }
+    }
+    for (int b0 = 0; b0 < ns + kConvC; b0 += T * kStage) {
+        double v[kStage];
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int k = b0 + u * T + (int)threadIdx.x;
+            const int idx = i0 + k;
+            v[u] = (k < ns + kConvC && idx < lim) ? a.gl[idx] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int k = b0 + u * T + (int)threadIdx.x;
+            if (k < ns + kConvC) wv[k] = v[u];
+        }
+    }
+    __syncthreads();
+    double acc[kConvC + 1];
+#pragma unroll
+    for (int c = 0; c <= kConvC; ++c) acc[c] = 0.0;
+    for (int m = threadIdx.x; m < ns; m += blockDim.x) {
+        const double qm = q[m];
+#pragma unroll
+        for (int c = 0; c <= kConvC; ++c) acc[c] = fma(qm, wv[m + c], acc[c]);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c <= kConvC; ++c) {
+        double v = acc[c];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) red[wid][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x <= kConvC) {
+        double v = 0.0;
+        for (int w = 0; w < kConvThreads / 64; ++w) v += red[w][threadIdx.x];
+        vout[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kConvC) {
+        const int i = i0 + threadIdx.x;
+        if (i < ns) {
+            a.volt[i] = vout[threadIdx.x];
+            if (a.profile_out) a.profile_out[i] = q[ns - 1 - i];
+            if (cclr)
+                for (int g = 0; g < a.groups; ++g) cclr[(size_t)g * ns + i] = 0u;
+        }
+    }
+    if (a.vk) {
+        // linear_interp_kick.cpp:33-48 on V (acc = 0 for the induced kick)
+        const double inv_bw = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.tab_params[0] = a.bc[0];
+            a.tab_params[1] = inv_bw;
+        }
+        if (threadIdx.x < kConvC) {
+            const int i = i0 + threadIdx.x;
+            if (i < ns - 1) {
+                const double vi = vout[threadIdx.x], vi1 = vout[threadIdx.x + 1];
+                const double k = a.charge * (vi1 - vi) * inv_bw;
+                a.vk[i] = k;
+                a.fac[i] = (a.charge * vi - a.bc[i] * k) + 0.0;
+            }
+        }
+    }
+}
+
+static int set_lds_limit(const void *fn, size_t bytes);
+
+int launch_induced_direct(const InducedArgs &a, hipStream_t s, const KernelEvents &ev) {
+    if (!a.gl || a.n_out != a.n_slices || a.n_slices < 1)
+        return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: bad arguments");
+    const int blocks = (a.n_slices + kConvC - 1) / kConvC;
+    const size_t lds = sizeof(double) * (2 * (size_t)a.n_slices + kConvC);
+    if (lds + sizeof(double) * (kConvThreads / 64 + 1) * (kConvC + 1) > (size_t)device_max_lds())
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "direct induced voltage: %d slices exceed LDS",
+                              a.n_slices);
+    int rc = set_lds_limit((const void *)induced_direct_kernel, lds);
+    if (rc) return rc;
+    launch(induced_direct_kernel, dim3(blocks), dim3(kConvThreads), lds, s, ev, a);
+    return blmi_check_launch("induced_direct");
 }
 
 // ---------------------------------------------------------------------------
@@ -481,7 +636,8 @@ int launch_irfft(const double2 *in, int64_t m_in, double *out, int64_t n_out, co
     return blmi_check_launch("irfft");
 }
 
-int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream_t s) {
+int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream_t s,
+                               const KernelEvents &ev) {
     if (p.M / 2 + 1 > 8 * p.threads || p.M > 16 * p.threads)
         return blmi_set_error(BLMI_ERR_UNSUPPORTED, "FFT plan M=%d too large for %d threads", p.M,
                               p.threads);
@@ -491,8 +647,8 @@ int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream
     const size_t lds = sizeof(double2) * p.M;
     int rc = set_lds_limit((const void *)induced_lds_kernel, lds);
     if (rc) return rc;
-    hipLaunchKernelGGL(induced_lds_kernel, dim3(1), dim3(p.threads), lds, s, a, p.M, stages_of(p),
-                       p.tw, p.tw2);
+    launch(induced_lds_kernel, dim3(1), dim3(p.threads), lds, s, ev, a, p.M, stages_of(p), p.tw,
+           p.tw2);
     return blmi_check_launch("induced_voltage");
 }
 

@@ -38,7 +38,8 @@ def _dptr(a: np.ndarray):
 
 class FusedTracker:
     def __init__(self, tracker, profile=None, total_induced_voltage=None, use_graph=True,
-                 block=256, blocks_per_cu=8, graph_chunk=50):
+                 block=1024, blocks_per_cu=1, graph_chunk=50, groups=8, max_peel=0,
+                 method=None):
         import torch
 
         if bm.precision.num != 2:
@@ -84,8 +85,32 @@ class FusedTracker:
             self._setup_induced()
         _lib.call("blmi_engine_set_launch", self._h, int(block), int(blocks_per_cu),
                   int(graph_chunk))
+        _lib.call("blmi_engine_set_hist", self._h, int(groups), int(max_peel))
+        if method is not None and self.tiv is not None:
+            _lib.call("blmi_engine_set_method", self._h, {"fft": 0, "direct": 1}[method])
         _lib.call("blmi_engine_set_turn", self._h, int(rp.counter[0]))
         self._primed = False
+
+    @property
+    def method(self) -> str | None:
+        """Induced-voltage method in use: 'fft', 'direct' or None."""
+        m = ct.c_int()
+        _lib.call("blmi_engine_get_method", self._h, ct.byref(m))
+        return {0: "fft", 1: "direct"}.get(m.value)
+
+    def time_kernels(self, turns: int):
+        """Track ``turns`` eager turns timing each kernel with HIP events on
+        the engine's stream; returns (induced_ms, particle_kernel_ms) averages
+        and advances the host state like ``track``."""
+        a, b = ct.c_float(), ct.c_float()
+        stream = ct.c_void_p(self._torch.cuda.current_stream().cuda_stream)
+        if self.tiv is not None and not self._primed:
+            _lib.call("blmi_engine_prime", self._h, stream)
+            self._primed = True
+        start = self.rf.counter[0]
+        _lib.call("blmi_engine_time_kernels", self._h, int(turns), ct.byref(a), ct.byref(b), stream)
+        self._advance(start, start + int(turns))
+        return a.value, b.value
 
     # ------------------------------------------------------------------
     def _phase_program(self):
@@ -96,10 +121,14 @@ class FusedTracker:
         phi = np.array(rp.host_array("phi_rf"), dtype=np.float64, copy=True)
         omega = rp.host_array("omega_rf")
         dphi = np.array(rp.dphi_rf, dtype=np.float64, copy=True)
+        self._dphi_prog = np.zeros((rp.n_rf, rp.n_turns + 1))
+        self._dphi_prog[:, rp.counter[0]] = dphi
         for t in range(rp.counter[0], rp.n_turns):
             dphi = dphi + (2.0 * np.pi * rp.harmonic[:, t + 1]
                            * (omega[:, t + 1] - rp.omega_rf_d[:, t + 1]) / rp.omega_rf_d[:, t + 1])
             phi[:, t + 1] += dphi
+            self._dphi_prog[:, t + 1] = dphi
+        self._phase_moves = bool(np.any(self._dphi_prog != 0))
         return phi
 
     def _upload_programs(self):
@@ -163,12 +192,20 @@ class FusedTracker:
             self._primed = True
         start = self.rf.counter[0]
         _lib.call("blmi_engine_track", self._h, n_turns, 1 if self.use_graph else 0, stream)
-        end = start + n_turns
+        self._advance(start, start + n_turns)
+
+    def _advance(self, start: int, end: int):
         rp = self.rf
         # host-side state the map leaves behind (tracker.py:556-615)
-        if end > start:
+        if end > start and self._phase_moves:
             host_phi = rp.host_array("phi_rf")
             host_phi[:, start + 1:end + 1] = self._phi_prog[:, start + 1:end + 1]
+            rp.dphi_rf[:] = self._dphi_prog[:, end]
+            if rp._device == "GPU":
+                import torch
+
+                rp.phi_rf[:, start + 1:end + 1] = torch.from_numpy(
+                    np.ascontiguousarray(self._phi_prog[:, start + 1:end + 1])).to(rp.phi_rf.device)
         rp.counter[0] = end
         self.beam.beta = rp.beta[end]
         self.beam.gamma = rp.gamma[end]

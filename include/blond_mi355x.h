@@ -146,7 +146,19 @@ int blmi_engine_set_slices(void *engine, int n_slices, double cut_left, double c
                            double *profile);
 int blmi_engine_set_induced(void *engine, int64_t n_fft, const double *Z_host, double scale,
                             double *volt, const double *bin_centers);
+/* Launch shape of the particle kernel (block in {64..1024}, workgroups per CU)
+ * and the number of turns per captured graph. */
 int blmi_engine_set_launch(void *engine, int block, int blocks_per_cu, int graph_chunk);
+/* Induced-voltage method: 0 = one-workgroup LDS FFT (rfft, x Z, irfft),
+ * 1 = direct convolution over all CUs (n_slices <= 8192).  set_induced picks
+ * 1 when available. */
+int blmi_engine_set_method(void *engine, int method);
+int blmi_engine_get_method(void *engine, int *method);
+/* Histogram: `groups` copies of the global counts (1..16; workgroup b adds into
+ * copy b % groups, the consumer sums them) and `max_peel` wavefront-aggregation
+ * steps (0..64) before plain LDS atomics.  Changing `groups` drops the staged
+ * counts of the current turn (prime again). */
+int blmi_engine_set_hist(void *engine, int groups, int max_peel);
 int blmi_engine_set_turn(void *engine, int turn);   /* synchronous */
 int blmi_engine_get_turn(void *engine, int *turn);  /* synchronous */
 int blmi_engine_prime(void *engine, void *stream);
