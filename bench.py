@@ -45,9 +45,9 @@ def parse():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--blocks-per-cu", type=int, default=1)
-    ap.add_argument("--groups", type=int, default=8)
+    ap.add_argument("--groups", type=int, default=0, help="count copies (0: automatic)")
     ap.add_argument("--max-peel", type=int, default=0)
-    ap.add_argument("--method", choices=["auto", "fft", "direct"], default="auto")
+    ap.add_argument("--method", choices=["auto", "fft", "direct", "fourstep"], default="auto")
     return ap.parse_args()
 
 

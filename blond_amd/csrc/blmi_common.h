@@ -20,6 +20,7 @@
 // Host-side error plumbing (defined in capi.hip).
 int blmi_set_error(int code, const char *fmt, ...);
 int blmi_check_launch(const char *what);
+extern "C" const char *blmi_last_error(void);
 
 #define BLMI_HIP_TRY(expr)                                                   \
     do {                                                                     \
@@ -44,7 +45,8 @@ struct KernelEvents {
 template <typename F, typename... Args>
 inline void launch(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s,
                    const KernelEvents &ev, Args... args) {
-    if (ev.start)
+    // either event may be null (a span over several launches)
+    if (ev.start || ev.stop)
         hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)lds, s, ev.start, ev.stop, 0u,
                               args...);
     else

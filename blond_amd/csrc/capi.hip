@@ -11,6 +11,7 @@
 
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "blmi_common.h"
@@ -287,9 +288,11 @@ int blmi_irfft(const double *in, int64_t m_in, double *out, int64_t n, void *str
 int blmi_induced_voltage(const double *profile, int n_slices, const double *Z, int64_t n_fft,
                          double scale, double *volt, int n_out, void *stream) {
     BLMI_REQUIRE(n_slices >= 1 && profile && Z && volt, "induced_voltage: bad arguments");
-    FftPlan p;
-    int rc = fft_plan_get(n_fft, &p);
-    if (rc) return rc;
+    BLMI_REQUIRE(n_fft >= 2 && (n_fft & 1) == 0, "induced_voltage: n_fft %lld must be even",
+                 (long long)n_fft);
+    BLMI_REQUIRE(n_slices <= n_fft && n_out >= 0 && n_out <= n_fft,
+                 "induced_voltage: n_slices %d / n_out %d exceed n_fft %lld", n_slices, n_out,
+                 (long long)n_fft);
     InducedArgs a{};
     a.src = profile;
     a.n_slices = n_slices;
@@ -297,7 +300,31 @@ int blmi_induced_voltage(const double *profile, int n_slices, const double *Z, i
     a.scale = scale;
     a.volt = volt;
     a.n_out = n_out;
-    return launch_induced_voltage_lds(a, p, S(stream));
+    if (n_fft <= fft_lds_max_real()) {
+        FftPlan p;
+        int rc = fft_plan_get(n_fft, &p);
+        if (rc == BLMI_OK) return launch_induced_voltage_lds(a, p, S(stream));
+    }
+    // beyond one workgroup's LDS: the four-step transform; its plan holds
+    // scratch, so plans are kept per (device, n_fft, stream)
+    static std::mutex mu;
+    static std::map<std::tuple<int, int64_t, void *>, Fft2Plan> plans;
+    int dev = 0;
+    BLMI_HIP_TRY(hipGetDevice(&dev));
+    Fft2Plan p2;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto key = std::make_tuple(dev, n_fft, stream);
+        auto it = plans.find(key);
+        if (it == plans.end()) {
+            int rc = fft2_plan_create(n_fft, &p2);
+            if (rc) return rc;
+            plans[key] = p2;
+        } else {
+            p2 = it->second;
+        }
+    }
+    return launch_induced_fourstep(a, p2, S(stream));
 }
 
 // --------------------------------------------------------------- engine ----

@@ -10,8 +10,11 @@
 
 namespace blmi {
 
-// LDS budget of the turn kernel's private histogram (u32 per bin).
+// LDS budget of the turn kernel's private histogram: u32 per bin when it
+// fits, else two u16 counters per u32 word (a workgroup then takes at most
+// 65535 particles, so no counter can carry into its neighbour).
 constexpr int kLdsHistBytes = 131072;
+constexpr int kMaxParticlesPerGroup16 = 65535;
 // Largest slice count for which the induced voltage uses the direct
 // (Toeplitz) convolution: ns^2 FP64 FMAs spread over all CUs.
 constexpr int kDirectMaxSlices = 8192;
@@ -32,15 +35,19 @@ struct Engine {
     int n_slices = 0;
     double cut_left = 0, cut_right = 0, inv_bw = 0;
     int max_peel = 0;
-    int groups = 8;                // copies of the counts (same-address atomics / groups)
+    int groups = 1;                // copies of the counts (same-address atomics / groups)
+    bool groups_auto = true;       // groups from the slice count (4 up to 1024 slices, else 1)
     unsigned *d_counts = nullptr;  // [2][groups][n_slices]
     double *profile = nullptr;     // borrowed: f64 profile seen by the caller
     // induced voltage
     bool intensity = false;
     int64_t n_fft = 0;
-    int method = 0;                // 0: LDS FFT (one workgroup), 1: direct convolution
-    bool have_fft = false, have_direct = false;
+    // 0: LDS FFT (one workgroup), 1: direct convolution, 2: four-step FFT
+    // over many workgroups (the particle kernel then interpolates V itself)
+    int method = 0;
+    bool have_fft = false, have_direct = false, have_fourstep = false;
     FftPlan plan{};
+    Fft2Plan plan2{};
     double2 *d_Z = nullptr;
     double *d_gl = nullptr;        // direct method: -scale*irfft(Z), unrolled
     double scale = 0;
@@ -79,6 +86,9 @@ struct Engine {
     int launch_turns(hipStream_t s, int count, bool advance);
     int finish_chunk(hipStream_t s, int count);
     int launch_turn_kernel(hipStream_t s, int off, bool interp, const KernelEvents &ev);
+    // histogram mode of the particle kernel (0 global atomics, 1 LDS u32,
+    // 2 LDS u16 pairs) and its grid and LDS bytes for `block` threads
+    int hist_layout(int block, int *grid, size_t *lds) const;
     int launch_induced(hipStream_t s, int off, const KernelEvents &ev);
     InducedArgs induced_args(int off) const;
     int time_kernels(int turns, hipStream_t user, float *ms_induced, float *ms_turn);

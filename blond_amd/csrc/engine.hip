@@ -33,6 +33,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <string>
 #include <vector>
 
 #include "blmi_common.h"
@@ -76,7 +77,8 @@ __device__ __forceinline__ T e_drift(const DriftCoeffs<T> &c, T e, int solver) {
 // with one LDS atomic, then the remaining lanes add one each.  A narrow
 // bunch (few distinct bins per wavefront) thus costs a handful of atomics
 // instead of 64 conflicting ones; a wide one costs at most max_peel extra
-// scalar steps.
+// scalar steps.  PACK16: bin b is the (b & 1) half of word b >> 1.
+template <bool PACK16>
 __device__ __forceinline__ void lds_count(unsigned *__restrict__ h, int bin, int max_peel) {
     unsigned long long pending = __ballot(bin >= 0);
     const int lane = (int)__lane_id();
@@ -84,10 +86,44 @@ __device__ __forceinline__ void lds_count(unsigned *__restrict__ h, int bin, int
         const int leader = __builtin_ctzll(pending);
         const int lb = __builtin_amdgcn_readlane(bin, leader);
         const unsigned long long same = __ballot(bin == lb) & pending;
-        if (lane == leader) atomicAdd(&h[lb], (unsigned)__popcll(same));
+        if (lane == leader) {
+            const unsigned c = (unsigned)__popcll(same);
+            if (PACK16)
+                atomicAdd(&h[lb >> 1], c << (16 * (lb & 1)));
+            else
+                atomicAdd(&h[lb], c);
+        }
         pending &= ~same;
     }
-    if ((pending >> lane) & 1ull) atomicAdd(&h[bin], 1u);
+    if ((pending >> lane) & 1ull) {
+        if (PACK16)
+            atomicAdd(&h[bin >> 1], 1u << (16 * (bin & 1)));
+        else
+            atomicAdd(&h[bin], 1u);
+    }
+}
+
+// LDS words of a histogram of n_slices bins in mode 1 (u32) or 2 (u16 pairs)
+__device__ __forceinline__ int hist_words(int mode, int n_slices) {
+    return mode == 2 ? (n_slices + 1) / 2 : n_slices;
+}
+
+// Add the workgroup's LDS histogram into the global counts (non-zero bins).
+template <int LDSH>
+__device__ __forceinline__ void hist_flush(const unsigned *__restrict__ h,
+                                           unsigned *__restrict__ counts, int n_slices) {
+    const int words = hist_words(LDSH, n_slices);
+    for (int w = threadIdx.x; w < words; w += blockDim.x) {
+        const unsigned c = h[w];
+        if (!c) continue;
+        if (LDSH == 1) {
+            atomic_add_agent(&counts[w], c);
+        } else {
+            const unsigned lo = c & 0xffffu, hi = c >> 16;
+            if (lo) atomic_add_agent(&counts[2 * w], lo);
+            if (hi) atomic_add_agent(&counts[2 * w + 1], hi);
+        }
+    }
 }
 
 struct TurnArgs {
@@ -99,8 +135,11 @@ struct TurnArgs {
     const int *base;                   // device turn base
     int off;                           // this launch's turn offset
     int last_turn;                     // n_turns - 1
-    // interpolation kick (intensity effects)
+    // interpolation kick (intensity effects): table vk/fac (mode 1) or the
+    // induced voltage and bin centres themselves (mode 2); tab_params =
+    // (bin_centers[0], inv_bw)
     const double *vk, *fac, *tab_params;
+    const double *volt, *bc;
     int n_tab;
     // slicing: counts[parity][group][bin]; workgroup b adds into group b % groups
     unsigned *counts0, *counts1;
@@ -126,7 +165,7 @@ __device__ __forceinline__ void publish_range(unsigned *__restrict__ c, double *
     }
 }
 
-template <int SOLVER, bool INTERP, bool HIST, bool LDSH, bool PUBLISH>
+template <int SOLVER, int INTERP, bool HIST, int LDSH, bool PUBLISH>
 __global__ void __launch_bounds__(1024) turn_kernel(double *__restrict__ dt, double *__restrict__ dE,
                                                     int64_t n, TurnArgs a) {
     extern __shared__ unsigned lds_hist[];
@@ -158,15 +197,24 @@ __global__ void __launch_bounds__(1024) turn_kernel(double *__restrict__ dt, dou
         ibw = a.tab_params[1];
     }
     if (HIST && LDSH) {
-        for (int b = threadIdx.x; b < a.n_slices; b += blockDim.x) lds_hist[b] = 0u;
+        const int words = hist_words(LDSH, a.n_slices);
+        for (int b = threadIdx.x; b < words; b += blockDim.x) lds_hist[b] = 0u;
         __syncthreads();
     }
     auto step = [&](double &x, double &e) -> int {
         if (INTERP) {
+            // linear_interp_kick.cpp:44-64 (acceleration kick 0)
             const double f = floor((x - bc0) * ibw);
             if (f >= 0.0 && f < (double)a.n_tab) {
                 const int k = (int)f;
-                e = e + (x * a.vk[k] + a.fac[k]);
+                if (INTERP == 1) {
+                    e = e + (x * a.vk[k] + a.fac[k]);
+                } else {
+                    const double v0 = a.volt[k], v1 = a.volt[k + 1];
+                    const double vk = a.charge * (v1 - v0) * ibw;
+                    const double fac = (a.charge * v0 - a.bc[k] * vk) + 0.0;
+                    e = e + (x * vk + fac);
+                }
             }
         }
         e = e + e_rf_kick(x, a.n_rf, a.charge, V, w, phi, acc);
@@ -180,7 +228,7 @@ __global__ void __launch_bounds__(1024) turn_kernel(double *__restrict__ dt, dou
     auto count = [&](int bin) {
         if (!HIST) return;
         if (LDSH)
-            lds_count(lds_hist, bin, a.max_peel);
+            lds_count<LDSH == 2>(lds_hist, bin, a.max_peel);
         else if (bin >= 0)
             atomic_add_agent(&counts[bin], 1u);
     };
@@ -210,10 +258,7 @@ __global__ void __launch_bounds__(1024) turn_kernel(double *__restrict__ dt, dou
     }
     if (HIST && LDSH) {
         __syncthreads();
-        for (int b = threadIdx.x; b < a.n_slices; b += blockDim.x) {
-            const unsigned c = lds_hist[b];
-            if (c) atomic_add_agent(&counts[b], c);
-        }
+        hist_flush<LDSH>(lds_hist, counts, a.n_slices);
     }
 }
 
@@ -232,16 +277,36 @@ __global__ void publish_kernel(const int *base, int off, unsigned *counts0, unsi
 __global__ void advance_kernel(int *base, int k) { *base += k; }
 
 // Histogram of the current coordinates into counts[t & 1] (engine priming);
-// both buffers are cleared first by fill kernels.
-__global__ void prime_hist_kernel(const double *__restrict__ dt, int64_t n, const int *base,
-                                  unsigned *counts0, unsigned *counts1, int n_slices,
-                                  double cut_left, double inv_bw) {
+// both buffers are cleared first by a fill kernel.
+template <int LDSH>
+__global__ void __launch_bounds__(1024) prime_hist_kernel(const double *__restrict__ dt, int64_t n,
+                                                          const int *base, unsigned *counts0,
+                                                          unsigned *counts1, int n_slices,
+                                                          double cut_left, double inv_bw) {
+    extern __shared__ unsigned lds_hist[];
     const int t = *base;
     unsigned *counts = (t & 1) ? counts1 : counts0;
+    if (LDSH) {
+        const int words = hist_words(LDSH, n_slices);
+        for (int b = threadIdx.x; b < words; b += blockDim.x) lds_hist[b] = 0u;
+        __syncthreads();
+    }
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const double f = floor((dt[i] - cut_left) * inv_bw);
-        if (f >= 0.0 && f < (double)n_slices) atomic_add_agent(&counts[(int)f], 1u);
+    const int64_t n_pad = ((n + blockDim.x - 1) / blockDim.x) * blockDim.x;  // whole wavefronts
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += stride) {
+        int bin = -1;
+        if (i < n) {
+            const double f = floor((dt[i] - cut_left) * inv_bw);
+            if (f >= 0.0 && f < (double)n_slices) bin = (int)f;
+        }
+        if (LDSH)
+            lds_count<LDSH == 2>(lds_hist, bin, 0);
+        else if (bin >= 0)
+            atomic_add_agent(&counts[bin], 1u);
+    }
+    if (LDSH) {
+        __syncthreads();
+        hist_flush<LDSH>(lds_hist, counts, n_slices);
     }
 }
 
@@ -260,6 +325,7 @@ void Engine::release() {
     (void)hipFree(d_table);
     (void)hipFree(d_Z);
     (void)hipFree(d_gl);
+    fft2_plan_destroy(&plan2);
     d_rf = d_acc = d_table = d_gl = nullptr;
     d_drift = nullptr;
     d_base = nullptr;
@@ -316,6 +382,8 @@ int Engine::set_slices(int n_slices_, double cut_left_, double cut_right_, doubl
     cut_right = cut_right_;
     inv_bw = (double)n_slices / (cut_right - cut_left);  // histogram.cpp:29
     profile = profile_;
+    // copies of the counts: few bins take many same-address atomics
+    if (groups_auto) groups = n_slices <= 1024 ? 4 : 1;
     invalidate_graph();
     return alloc_counts();
 }
@@ -350,6 +418,8 @@ int Engine::set_induced(int64_t n_fft_, const double *Z_host, double scale_, dou
     BLMI_HIP_TRY(hipMalloc(&d_table, sizeof(double) * (2 * (size_t)n_slices + 2)));
     BLMI_HIP_TRY(hipMemset(d_table, 0, sizeof(double) * (2 * (size_t)n_slices + 2)));
     have_fft = fft_plan_get(n_fft, &plan) == BLMI_OK;
+    fft2_plan_destroy(&plan2);
+    have_fourstep = n_slices <= n_fft && fft2_plan_create(n_fft, &plan2) == BLMI_OK;
     // the direct method's wake kernel g = -scale * irfft(Z) needs one
     // transform of length n_fft at set-up
     have_direct = false;
@@ -377,11 +447,12 @@ int Engine::set_induced(int64_t n_fft_, const double *Z_host, double scale_, dou
         BLMI_HIP_TRY(hipMemcpy(d_gl, gl.data(), sizeof(double) * len, hipMemcpyHostToDevice));
         have_direct = true;
     }
-    if (!have_fft && !have_direct)
-        return blmi_set_error(BLMI_ERR_UNSUPPORTED,
-                              "induced voltage: n_fft=%lld is not supported by the LDS transform",
-                              (long long)n_fft);
-    method = have_direct ? 1 : 0;
+    if (!have_fft && !have_direct && !have_fourstep) {
+        const std::string why = blmi_last_error();  // the plan's own message
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "induced voltage: n_fft=%lld is not supported (%s)",
+                              (long long)n_fft, why.c_str());
+    }
+    method = have_direct ? 1 : have_fft ? 0 : 2;
     intensity = true;
     invalidate_graph();
     return BLMI_OK;
@@ -394,18 +465,23 @@ int Engine::set_method(int method_) {
     if (method_ == 0 && !have_fft)
         return blmi_set_error(BLMI_ERR_UNSUPPORTED, "LDS FFT not available (n_fft %lld)",
                               (long long)n_fft);
-    if (method_ != 0 && method_ != 1) return blmi_set_error(BLMI_ERR_ARG, "method must be 0 or 1");
+    if (method_ == 2 && !have_fourstep)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "four-step FFT not available (n_fft %lld)",
+                              (long long)n_fft);
+    if (method_ < 0 || method_ > 2) return blmi_set_error(BLMI_ERR_ARG, "method must be 0, 1 or 2");
     method = method_;
     invalidate_graph();
     return BLMI_OK;
 }
 
 int Engine::set_hist(int groups_, int max_peel_) {
-    if (groups_ < 1 || groups_ > kMaxCountGroups)
-        return blmi_set_error(BLMI_ERR_ARG, "groups in [1, %d]", kMaxCountGroups);
+    if (groups_ < 0 || groups_ > kMaxCountGroups)
+        return blmi_set_error(BLMI_ERR_ARG, "groups in [0 (automatic), %d]", kMaxCountGroups);
     if (max_peel_ < 0 || max_peel_ > 64) return blmi_set_error(BLMI_ERR_ARG, "max_peel in [0, 64]");
     max_peel = max_peel_;
     invalidate_graph();
+    groups_auto = groups_ == 0;
+    if (groups_auto) groups_ = n_slices <= 1024 ? 4 : 1;
     if (groups_ != groups) {
         // the staged counts of the current turn are dropped: the caller
         // primes again (FusedTracker does so on its next track())
@@ -441,38 +517,89 @@ int Engine::get_turn(int *turn) {
     return BLMI_OK;
 }
 
+int Engine::hist_layout(int block, int *grid_out, size_t *lds_out) const {
+    int grid = device_cu_count() * blocks_per_cu;
+    const int64_t n2 = n / 2;
+    const int64_t need = (n2 + block - 1) / block;
+    if (grid > need) grid = (int)(need > 0 ? need : 1);
+    int mode = 0;
+    size_t lds = 0;
+    if ((size_t)n_slices * sizeof(unsigned) <= (size_t)kLdsHistBytes) {
+        mode = 1;
+        lds = (size_t)n_slices * sizeof(unsigned);
+    } else if ((size_t)((n_slices + 1) / 2) * sizeof(unsigned) <= (size_t)kLdsHistBytes) {
+        // a thread takes ceil(n2 / (grid*block)) pairs: keep a workgroup's
+        // particles (+1 for an odd tail) within the u16 counters
+        const int64_t max_pairs = (kMaxParticlesPerGroup16 - 1) / (2 * (int64_t)block);
+        if (max_pairs >= 1) {
+            const int64_t g = (n2 + block * max_pairs - 1) / (block * max_pairs);
+            if (g > grid) grid = (int)g;
+            mode = 2;
+            lds = (size_t)((n_slices + 1) / 2) * sizeof(unsigned);
+        }
+    }
+    *grid_out = grid;
+    *lds_out = lds;
+    return mode;
+}
+
+static int lds_attribute(const void *fn, size_t lds) {
+    if (lds <= 65536) return BLMI_OK;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess)
+        return blmi_set_error(BLMI_ERR_HIP, "kernel LDS attribute (%zu B): %s", lds,
+                              hipGetErrorString(e));
+    return BLMI_OK;
+}
+
 int Engine::prime(hipStream_t s) {
     // counts of the current coordinates (the first pass of the map's
     // Profile.track()) into counts[t & 1]; the other buffer cleared
     if (!intensity || n_slices <= 0) return BLMI_OK;
     int rc = launch_fill<unsigned>(d_counts, 2 * (int64_t)groups * n_slices, 0u, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(prime_hist_kernel, dim3(stream_grid(n, 256)), dim3(256), 0, s, dt, n, d_base,
-                       d_counts, d_counts + (size_t)groups * n_slices, n_slices, cut_left, inv_bw);
+    int grid = 0;
+    size_t lds = 0;
+    const int mode = hist_layout(512, &grid, &lds);
+    // hist_layout counts pairs per thread; this kernel takes single particles
+    grid *= 2;
+    const void *fn = mode == 2   ? (const void *)prime_hist_kernel<2>
+                     : mode == 1 ? (const void *)prime_hist_kernel<1>
+                                 : (const void *)prime_hist_kernel<0>;
+    rc = lds_attribute(fn, lds);
+    if (rc) return rc;
+    unsigned *c1 = d_counts + (size_t)groups * n_slices;
+    if (mode == 2)
+        hipLaunchKernelGGL(prime_hist_kernel<2>, dim3(grid), dim3(256), lds, s, dt, n, d_base,
+                           d_counts, c1, n_slices, cut_left, inv_bw);
+    else if (mode == 1)
+        hipLaunchKernelGGL(prime_hist_kernel<1>, dim3(grid), dim3(256), lds, s, dt, n, d_base,
+                           d_counts, c1, n_slices, cut_left, inv_bw);
+    else
+        hipLaunchKernelGGL(prime_hist_kernel<0>, dim3(grid), dim3(256), 0, s, dt, n, d_base,
+                           d_counts, c1, n_slices, cut_left, inv_bw);
     return blmi_check_launch("engine_prime");
 }
 
-template <int SOLVER>
-static void launch_turn_solver(const Engine &e, const TurnArgs &a, bool interp, bool hist,
-                               bool ldsh, int grid, int block, size_t lds, hipStream_t s,
-                               const KernelEvents &ev) {
-#define BLMI_TURN(I, H, L, P)                                                                  \
-    launch(turn_kernel<SOLVER, I, H, L, P>, dim3(grid), dim3(block), lds, s, ev, e.dt, e.dE, e.n, \
-           a)
-    if (interp) {
-        if (hist) {
-            if (ldsh) BLMI_TURN(true, true, true, false); else BLMI_TURN(true, true, false, false);
-        } else {
-            BLMI_TURN(true, false, false, false);
-        }
-    } else {
-        if (hist) {
-            if (ldsh) BLMI_TURN(false, true, true, true); else BLMI_TURN(false, true, false, true);
-        } else {
-            BLMI_TURN(false, false, false, false);
-        }
+using TurnFn = void (*)(double *, double *, int64_t, TurnArgs);
+
+template <int S>
+static TurnFn pick_turn(int interp, bool hist, int ldsh) {
+    if (!hist)
+        return interp == 0 ? turn_kernel<S, 0, false, 0, false>
+               : interp == 1 ? turn_kernel<S, 1, false, 0, false>
+                             : turn_kernel<S, 2, false, 0, false>;
+    switch (interp * 3 + ldsh) {
+        case 0: return turn_kernel<S, 0, true, 0, true>;
+        case 1: return turn_kernel<S, 0, true, 1, true>;
+        case 2: return turn_kernel<S, 0, true, 2, true>;
+        case 3: return turn_kernel<S, 1, true, 0, false>;
+        case 4: return turn_kernel<S, 1, true, 1, false>;
+        case 5: return turn_kernel<S, 1, true, 2, false>;
+        case 6: return turn_kernel<S, 2, true, 0, false>;
+        case 7: return turn_kernel<S, 2, true, 1, false>;
+        default: return turn_kernel<S, 2, true, 2, false>;
     }
-#undef BLMI_TURN
 }
 
 InducedArgs Engine::induced_args(int off) const {
@@ -491,8 +618,9 @@ InducedArgs Engine::induced_args(int off) const {
     ia.n_out = n_slices;
     ia.bc = bc;
     ia.charge = charge;
-    ia.vk = d_table;
-    ia.fac = d_table + n_slices;
+    // the four-step path leaves the slope/intercept to the particle kernel
+    ia.vk = method == 2 ? nullptr : d_table;
+    ia.fac = method == 2 ? nullptr : d_table + n_slices;
     ia.tab_params = d_table + 2 * n_slices;
     ia.gl = d_gl;
     return ia;
@@ -501,6 +629,7 @@ InducedArgs Engine::induced_args(int off) const {
 int Engine::launch_induced(hipStream_t s, int off, const KernelEvents &ev) {
     InducedArgs ia = induced_args(off);
     if (method == 1) return launch_induced_direct(ia, s, ev);
+    if (method == 2) return launch_induced_fourstep(ia, plan2, s, ev);
     return launch_induced_voltage_lds(ia, plan, s, ev);
 }
 
@@ -518,6 +647,8 @@ int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const Kernel
     a.vk = d_table;
     a.fac = d_table ? d_table + n_slices : nullptr;
     a.tab_params = d_table ? d_table + 2 * n_slices : nullptr;
+    a.volt = volt;
+    a.bc = bc;
     a.n_tab = n_slices - 1;
     a.counts0 = d_counts;
     a.counts1 = d_counts ? d_counts + (size_t)groups * n_slices : nullptr;
@@ -528,30 +659,21 @@ int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const Kernel
     a.max_peel = max_peel;
     a.profile = profile;
     a.pending = d_base + 1;
-    const size_t lds_bytes = (size_t)n_slices * sizeof(unsigned);
-    const bool ldsh = hist && lds_bytes <= (size_t)kLdsHistBytes;
     const int block = block_size;
-    int grid = device_cu_count() * blocks_per_cu;
-    const int64_t need = (n / 2 + block - 1) / block;
-    if (grid > need) grid = (int)(need > 0 ? need : 1);
-    const size_t lds = ldsh ? lds_bytes : 0;
-    if (lds > 65536) {
-        hipError_t e = hipSuccess;
-#define BLMI_ATTR(S, I, P)                                                                       \
-    if (e == hipSuccess)                                                                         \
-        e = hipFuncSetAttribute((const void *)turn_kernel<S, I, true, true, P>,                  \
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        BLMI_ATTR(0, true, false) BLMI_ATTR(1, true, false) BLMI_ATTR(2, true, false)
-        BLMI_ATTR(0, false, true) BLMI_ATTR(1, false, true) BLMI_ATTR(2, false, true)
-#undef BLMI_ATTR
-        if (e != hipSuccess)
-            return blmi_set_error(BLMI_ERR_HIP, "turn kernel LDS attribute: %s", hipGetErrorString(e));
+    int grid = 0;
+    size_t lds = 0;
+    int ldsh = hist_layout(block, &grid, &lds);
+    if (!hist) {
+        ldsh = 0;
+        lds = 0;
     }
-    switch (solver) {
-        case 0: launch_turn_solver<0>(*this, a, interp, hist, ldsh, grid, block, lds, s, ev); break;
-        case 1: launch_turn_solver<1>(*this, a, interp, hist, ldsh, grid, block, lds, s, ev); break;
-        default: launch_turn_solver<2>(*this, a, interp, hist, ldsh, grid, block, lds, s, ev); break;
-    }
+    const int imode = interp ? (method == 2 ? 2 : 1) : 0;
+    TurnFn fn = solver == 0   ? pick_turn<0>(imode, hist, ldsh)
+                : solver == 1 ? pick_turn<1>(imode, hist, ldsh)
+                              : pick_turn<2>(imode, hist, ldsh);
+    int rc = lds_attribute((const void *)fn, lds);
+    if (rc) return rc;
+    launch(fn, dim3(grid), dim3(block), lds, s, ev, dt, dE, n, a);
     return blmi_check_launch("turn");
 }
 

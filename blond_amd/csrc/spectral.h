@@ -72,4 +72,28 @@ int launch_induced_direct(const InducedArgs &a, hipStream_t s,
 int launch_fast_resonator(double *re, double *im, const double *freq, const double *R,
                           const double *Q, const double *fr, int n_res, int n_f, hipStream_t s);
 
+// Multi-workgroup real-FFT convolution for n_fft beyond one workgroup's LDS:
+// complex length M = n_fft/2 = M1 * M2 as a four-step transform in three
+// launches -- (A) M1-point column transforms with twiddles, (B) per pair of
+// rows (k1, M1-k1): M2-point transforms, real split, impedance product, real
+// merge, inverse M2-point transforms and twiddles, (C) inverse M1-point
+// column transforms writing V.  The plan owns its twiddles and the two
+// M-complex scratch arrays (one plan per user: launches on one stream).
+struct Fft2Plan {
+    int M = 0, M1 = 0, M2 = 0, n_real = 0;
+    int threads_b = 0;            // pass B workgroup size (passes A, C: 256)
+    int n1 = 0, n2 = 0;           // stage counts of the M1- and M2-point transforms
+    int r1[kMaxStages], r2[kMaxStages];
+    double2 *twM = nullptr;       // exp(-2 pi i j / M), j < M (all sub-transforms)
+    double2 *twN = nullptr;       // exp(-2 pi i k / 2M), k <= M (real split / merge)
+    double2 *T = nullptr, *S = nullptr;  // M-complex intermediates
+};
+int fft2_plan_create(int64_t n_real, Fft2Plan *plan);
+void fft2_plan_destroy(Fft2Plan *plan);
+// Same contract as launch_induced_voltage_lds (InducedArgs; the table
+// outputs vk/fac are not produced here).  `ev`, when given, spans the three
+// launches (start at pass A, stop at pass C).
+int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t s,
+                            const KernelEvents &ev = KernelEvents());
+
 }  // namespace blmi

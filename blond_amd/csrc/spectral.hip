@@ -115,23 +115,30 @@ template <bool INV> struct Dft<5, INV> {
     }
 };
 
-// One Stockham stage: Ns = product of the radices already applied.
+// One Stockham stage over `batch` transforms of length L, transform f at
+// buf + f * ld.  Ns = product of the radices already applied.  Twiddles come
+// from a table of exp(-2 pi i j / tw_len), L dividing tw_len.  The host
+// guarantees batch * L <= 15 * blockDim.x (at most 16/R butterflies a thread).
 template <int R, bool INV>
-__device__ __forceinline__ void stockham_stage(double2 *buf, int M, int Ns,
-                                               const double2 *__restrict__ tw) {
+__device__ __forceinline__ void stockham_stage(double2 *buf, int L, int batch, int ld, int Ns,
+                                               const double2 *__restrict__ tw, int tw_len) {
     constexpr int BMAX = (16 / R) > 0 ? (16 / R) : 1;
-    const int nb = M / R;
+    const int nb = L / R;
+    const int total = nb * batch;
     const int T = blockDim.x;
-    const int tstride = M / (Ns * R);
+    const int tstride = tw_len / (Ns * R);
     double2 v[BMAX][R];
 #pragma unroll
     for (int q = 0; q < BMAX; ++q) {
         const int j = threadIdx.x + q * T;
-        if (j < nb) {
-            const int k = j % Ns;
+        if (j < total) {
+            const int f = batch == 1 ? 0 : j / nb;
+            const int jj = j - f * nb;
+            const double2 *b = buf + f * ld;
+            const int k = jj % Ns;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                double2 x = buf[j + r * nb];
+                double2 x = b[jj + r * nb];
                 if (r > 0) {
                     double2 w = tw[r * k * tstride];
                     if (INV) w.y = -w.y;
@@ -145,12 +152,15 @@ __device__ __forceinline__ void stockham_stage(double2 *buf, int M, int Ns,
 #pragma unroll
     for (int q = 0; q < BMAX; ++q) {
         const int j = threadIdx.x + q * T;
-        if (j < nb) {
+        if (j < total) {
             Dft<R, INV>::run(v[q]);
-            const int k = j % Ns;
-            const int base = (j - k) * R + k;
+            const int f = batch == 1 ? 0 : j / nb;
+            const int jj = j - f * nb;
+            double2 *b = buf + f * ld;
+            const int k = jj % Ns;
+            const int base = (jj - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[q][r];
+            for (int r = 0; r < R; ++r) b[base + r * Ns] = v[q][r];
         }
     }
     __syncthreads();
@@ -162,18 +172,24 @@ struct StageList {
 };
 
 template <bool INV>
-__device__ void fft_lds(double2 *buf, int M, const StageList &st, const double2 *__restrict__ tw) {
+__device__ void fft_lds_batch(double2 *buf, int L, int batch, int ld, const StageList &st,
+                              const double2 *__restrict__ tw, int tw_len) {
     int Ns = 1;
     for (int s = 0; s < st.n; ++s) {
         switch (st.radix[s]) {
-            case 8: stockham_stage<8, INV>(buf, M, Ns, tw); break;
-            case 4: stockham_stage<4, INV>(buf, M, Ns, tw); break;
-            case 2: stockham_stage<2, INV>(buf, M, Ns, tw); break;
-            case 5: stockham_stage<5, INV>(buf, M, Ns, tw); break;
-            default: stockham_stage<3, INV>(buf, M, Ns, tw); break;
+            case 8: stockham_stage<8, INV>(buf, L, batch, ld, Ns, tw, tw_len); break;
+            case 4: stockham_stage<4, INV>(buf, L, batch, ld, Ns, tw, tw_len); break;
+            case 2: stockham_stage<2, INV>(buf, L, batch, ld, Ns, tw, tw_len); break;
+            case 5: stockham_stage<5, INV>(buf, L, batch, ld, Ns, tw, tw_len); break;
+            default: stockham_stage<3, INV>(buf, L, batch, ld, Ns, tw, tw_len); break;
         }
         Ns *= st.radix[s];
     }
+}
+
+template <bool INV>
+__device__ void fft_lds(double2 *buf, int M, const StageList &st, const double2 *__restrict__ tw) {
+    fft_lds_batch<INV>(buf, M, 1, M, st, tw, M);
 }
 
 // X[k] and X[M-k] of the real transform from the half-length complex
@@ -350,13 +366,13 @@ __global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
 //   V[i] = sum_m q[m] gl[i + m],  q[m] = p[ns-1-m],  gl[k] = g[(k-(ns-1)) mod n_fft]
 // Each workgroup produces kConvC outputs plus the next one (needed by the
 // last table entry).  The profile (reversed) and the workgroup's window of gl
-// are staged in LDS with coalesced loads; lane l then takes samples
-// m = l, l + 256, ... so that every LDS read of a wavefront hits consecutive
+// are staged in LDS with coalesced loads; thread l then takes samples
+// m = l, l + blockDim, ... so that every LDS read of a wavefront hits consecutive
 // words, and the partial sums are reduced in a fixed order (deterministic).
 // LDS: (2 ns + kConvC) doubles.
 // ---------------------------------------------------------------------------
 constexpr int kConvC = 16;
-constexpr int kConvThreads = 256;
+constexpr int kConvThreads = 1024;
 
 __global__ void __launch_bounds__(kConvThreads) induced_direct_kernel(InducedArgs a) {
     extern __shared__ double conv_lds[];
@@ -658,6 +674,274 @@ int launch_fast_resonator(double *re, double *im, const double *freq, const doub
     hipLaunchKernelGGL(fast_resonator_kernel, dim3((n_f + 255) / 256), dim3(256), 0, s, re, im,
                        freq, R, Q, fr, n_res, n_f);
     return blmi_check_launch("fast_resonator");
+}
+
+
+// ---------------------------------------------------------------------------
+// four-step real-FFT convolution for n_fft beyond one workgroup's LDS
+//   complex length M = M1 * M2, z[m] = p[2m] + i p[2m+1], m = M2 m1 + m2
+//   A: per column m2, T[k1][m2] = W_M^(m2 k1) sum_m1 z[M2 m1 + m2] W_M1^(m1 k1)
+//   B: per row k1, Zc[k1 + M1 k2] = sum_m2 T[k1][m2] W_M2^(m2 k2); rows k1 and
+//      M1 - k1 hold each other's mirror bins M - k, so a workgroup owns such a
+//      pair, does the real split, the impedance product and the real merge,
+//      then the inverse M2-point transform and the conjugate twiddle -> S
+//   C: per column, inverse M1-point transform of S -> u[M2 m1 + m2], and
+//      V[2m], V[2m+1] = -(scale * (u / M))   (the merge carries the 1/2)
+// Columns are processed kColBatch at a time so that each row access is a
+// contiguous run (coalesced); LDS rows are padded by one element.
+// ---------------------------------------------------------------------------
+constexpr int kColBatch = 8;
+constexpr int kColThreads = 256;
+
+__device__ __forceinline__ void count_buffers(const InducedArgs &a, const unsigned *&cin,
+                                              unsigned *&cclr) {
+    cin = nullptr;
+    cclr = nullptr;
+    if (a.counts0) {
+        const int t = *a.turn_base + a.turn_off;
+        cin = (t & 1) ? a.counts1 : a.counts0;
+        cclr = (t & 1) ? a.counts0 : a.counts1;
+    }
+}
+
+__global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int M1, int M2,
+                                                             StageList st1,
+                                                             const double2 *__restrict__ twM,
+                                                             double2 *__restrict__ T) {
+    extern __shared__ double2 buf[];
+    const int ld = M1 + 1;
+    const int M = M1 * M2;
+    const int ns = a.n_slices;
+    const int c0 = blockIdx.x * kColBatch;
+    const unsigned *cin;
+    unsigned *cclr;
+    count_buffers(a, cin, cclr);
+    for (int j = threadIdx.x; j < kColBatch * M1; j += blockDim.x) {
+        const int c = j % kColBatch, m1 = j / kColBatch;
+        const int m = M2 * m1 + c0 + c;
+        const int j0 = 2 * m, j1 = j0 + 1;
+        double p[2] = {0.0, 0.0};
+        const int jj[2] = {j0, j1};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (jj[h] >= ns) continue;
+            if (cin) {
+                unsigned sum = 0;
+                for (int g = 0; g < a.groups; ++g) sum += cin[(size_t)g * ns + jj[h]];
+                p[h] = (double)sum;
+                for (int g = 0; g < a.groups; ++g) cclr[(size_t)g * ns + jj[h]] = 0u;
+            } else {
+                p[h] = a.src[jj[h]];
+            }
+            if (a.profile_out) a.profile_out[jj[h]] = p[h];
+        }
+        buf[c * ld + m1] = make_double2(p[0], p[1]);
+    }
+    __syncthreads();
+    fft_lds_batch<false>(buf, M1, kColBatch, ld, st1, twM, M);
+    for (int j = threadIdx.x; j < kColBatch * M1; j += blockDim.x) {
+        const int c = j % kColBatch, k1 = j / kColBatch;
+        const int m2 = c0 + c;
+        T[(size_t)k1 * M2 + m2] = cmul(buf[c * ld + k1], twM[(size_t)m2 * k1]);
+    }
+}
+
+__global__ void fft2_b_kernel(InducedArgs a, int M1, int M2, StageList st2,
+                              const double2 *__restrict__ twM, const double2 *__restrict__ twN,
+                              const double2 *__restrict__ T, double2 *__restrict__ S) {
+    extern __shared__ double2 buf[];
+    const int M = M1 * M2;
+    const int q = blockIdx.x;  // rows q and M1 - q
+    const int r0 = q, r1 = (M1 - q) % M1;
+    const int nrow = r1 == r0 ? 1 : 2;
+    const int ne = nrow * M2;
+    for (int j = threadIdx.x; j < ne; j += blockDim.x) {
+        const int row = j >= M2, k2 = j - row * M2;
+        buf[j] = T[(size_t)(row ? r1 : r0) * M2 + k2];
+    }
+    __syncthreads();
+    fft_lds_batch<false>(buf, M2, nrow, M2, st2, twM, M);
+    // buf[row][k2] = Zc[r + M1 k2]; the mirror bin M - k sits at
+    // row 0, (M2 - k2) % M2 for r = 0 and at row of M1 - r, M2 - 1 - k2 otherwise
+    constexpr int EMAX = 8;  // host guarantees ne <= EMAX * blockDim.x
+    double2 u[EMAX];
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+        const int j = threadIdx.x + e * blockDim.x;
+        if (j < ne) {
+            const int row = j >= M2, k2 = j - row * M2;
+            const int r = row ? r1 : r0;
+            const int k = r + M1 * k2;
+            int pj;
+            if (r == 0) {
+                pj = (M2 - k2) % M2;
+            } else {
+                const int pr = M1 - r;
+                pj = (pr == r0 ? 0 : M2) + (M2 - 1 - k2);
+            }
+            double2 Xk, Xmk;
+            real_split(buf[j], buf[pj], twN[k], twN[M - k], Xk, Xmk);
+            // impedance.py:479-484: total_impedance * beam_spectrum
+            double2 Yk = cmul(a.Z[k], Xk), Ymk = cmul(a.Z[M - k], Xmk);
+            if (k == 0) {  // c2r ignores the imaginary parts of DC and Nyquist
+                Yk.y = 0.0;
+                Ymk.y = 0.0;
+            }
+            u[e] = real_merge(Yk, Ymk, twN[k]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+        const int j = threadIdx.x + e * blockDim.x;
+        if (j < ne) buf[j] = u[e];
+    }
+    __syncthreads();
+    fft_lds_batch<true>(buf, M2, nrow, M2, st2, twM, M);
+    for (int j = threadIdx.x; j < ne; j += blockDim.x) {
+        const int row = j >= M2, m2 = j - row * M2;
+        const int r = row ? r1 : r0;
+        S[(size_t)r * M2 + m2] = cmul(buf[j], cconj(twM[(size_t)r * m2]));
+    }
+}
+
+__global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int M1, int M2,
+                                                             StageList st1,
+                                                             const double2 *__restrict__ twM,
+                                                             const double2 *__restrict__ S) {
+    extern __shared__ double2 buf[];
+    const int ld = M1 + 1;
+    const int M = M1 * M2;
+    const int c0 = blockIdx.x * kColBatch;
+    for (int j = threadIdx.x; j < kColBatch * M1; j += blockDim.x) {
+        const int c = j % kColBatch, k1 = j / kColBatch;
+        buf[c * ld + k1] = S[(size_t)k1 * M2 + c0 + c];
+    }
+    __syncthreads();
+    fft_lds_batch<true>(buf, M1, kColBatch, ld, st1, twM, M);
+    // V = -(charge*e*ratio * irfft(...))[:n_out]   (impedance.py:475-489)
+    const double fct = 1.0 / (double)M;
+    for (int j = threadIdx.x; j < kColBatch * M1; j += blockDim.x) {
+        const int c = j % kColBatch, m1 = j / kColBatch;
+        const int m = M2 * m1 + c0 + c;
+        const double2 v = buf[c * ld + m1];
+        if (2 * m < a.n_out) a.volt[2 * m] = -(a.scale * (v.x * fct));
+        if (2 * m + 1 < a.n_out) a.volt[2 * m + 1] = -(a.scale * (v.y * fct));
+    }
+    if (a.tab_params && blockIdx.x == 0 && threadIdx.x == 0) {
+        // linear_interp_kick.cpp:33-35: the kick reads V and the bin centres
+        const int ns = a.n_slices;
+        a.tab_params[0] = a.bc[0];
+        a.tab_params[1] = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
+    }
+}
+
+static bool factor_list(int L, int *radix, int &n) {
+    StageList st;
+    if (!factor_radices(L, st)) return false;
+    n = st.n;
+    for (int s = 0; s < st.n; ++s) radix[s] = st.radix[s];
+    return true;
+}
+
+static StageList stage_list(const int *radix, int n) {
+    StageList st;
+    st.n = n;
+    for (int s = 0; s < n; ++s) st.radix[s] = radix[s];
+    return st;
+}
+
+int fft2_plan_create(int64_t n_real, Fft2Plan *plan) {
+    *plan = Fft2Plan();
+    if (n_real < 4 || (n_real & 1))
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "FFT length %lld must be even and >= 4",
+                              (long long)n_real);
+    if (n_real > (int64_t)1 << 30)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "FFT length %lld too large", (long long)n_real);
+    const int M = (int)(n_real / 2);
+    // M2: the largest divisor <= 2048 that is a multiple of kColBatch with an
+    // M1-point column batch fitting the pass A/C workgroup
+    int M1 = 0, M2 = 0;
+    for (int d = 2048; d >= kColBatch; d -= kColBatch) {
+        if (M % d) continue;
+        const int m1 = M / d;
+        if (m1 < 2 || kColBatch * m1 > 15 * kColThreads) continue;
+        int r[kMaxStages], n1, n2;
+        if (!factor_list(m1, r, n1) || !factor_list(d, r, n2)) continue;
+        M1 = m1;
+        M2 = d;
+        break;
+    }
+    if (!M1)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED,
+                              "FFT length %lld has no four-step split (needs 2*2^a*3^b*5^c with a "
+                              "factor 8)", (long long)n_real);
+    Fft2Plan p;
+    p.M = M;
+    p.M1 = M1;
+    p.M2 = M2;
+    p.n_real = 2 * M;
+    factor_list(M1, p.r1, p.n1);
+    factor_list(M2, p.r2, p.n2);
+    int tb = (2 * M2 + 7) / 8;
+    tb = ((tb + 63) / 64) * 64;
+    p.threads_b = tb < 256 ? 256 : tb;
+    auto cis = [](long double num, long double den) {
+        const long double two_pi = 6.283185307179586476925286766559L;
+        long double x = two_pi * num / den;
+        return make_double2((double)cosl(x), (double)(-sinl(x)));
+    };
+    std::vector<double2> twM(M), twN(M + 1);
+    for (int j = 0; j < M; ++j) twM[j] = cis(j, M);
+    for (int k = 0; k <= M; ++k) twN[k] = cis(k, 2.0L * M);
+    hipError_t e = hipMalloc(&p.twM, sizeof(double2) * M);
+    if (e == hipSuccess) e = hipMalloc(&p.twN, sizeof(double2) * (M + 1));
+    if (e == hipSuccess) e = hipMalloc(&p.T, sizeof(double2) * M);
+    if (e == hipSuccess) e = hipMalloc(&p.S, sizeof(double2) * M);
+    if (e == hipSuccess)
+        e = hipMemcpy(p.twM, twM.data(), sizeof(double2) * M, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p.twN, twN.data(), sizeof(double2) * (M + 1), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        fft2_plan_destroy(&p);
+        return blmi_set_error(BLMI_ERR_ALLOC, "four-step plan: %s", hipGetErrorString(e));
+    }
+    *plan = p;
+    return BLMI_OK;
+}
+
+void fft2_plan_destroy(Fft2Plan *p) {
+    (void)hipFree(p->twM);
+    (void)hipFree(p->twN);
+    (void)hipFree(p->T);
+    (void)hipFree(p->S);
+    *p = Fft2Plan();
+}
+
+int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t s,
+                            const KernelEvents &ev) {
+    if (!p.M) return blmi_set_error(BLMI_ERR_ARG, "four-step plan not created");
+    if (a.n_slices > p.n_real || a.n_out < 0 || a.n_out > p.n_real)
+        return blmi_set_error(BLMI_ERR_ARG, "n_slices %d / n_out %d exceed n_fft %d", a.n_slices,
+                              a.n_out, p.n_real);
+    if (a.counts0 && a.groups > kMaxCountGroups)
+        return blmi_set_error(BLMI_ERR_ARG, "too many count groups");
+    const size_t lds_ac = sizeof(double2) * kColBatch * (size_t)(p.M1 + 1);
+    const size_t lds_b = sizeof(double2) * 2 * (size_t)p.M2;
+    int rc = set_lds_limit((const void *)fft2_a_kernel, lds_ac);
+    if (!rc) rc = set_lds_limit((const void *)fft2_c_kernel, lds_ac);
+    if (!rc) rc = set_lds_limit((const void *)fft2_b_kernel, lds_b);
+    if (rc) return rc;
+    const StageList st1 = stage_list(p.r1, p.n1), st2 = stage_list(p.r2, p.n2);
+    const int cols = p.M2 / kColBatch;
+    const KernelEvents first{ev.start, nullptr}, last{nullptr, ev.stop};
+    launch(fft2_a_kernel, dim3(cols), dim3(kColThreads), lds_ac, s, first, a, p.M1, p.M2, st1,
+           p.twM, p.T);
+    hipLaunchKernelGGL(fft2_b_kernel, dim3(p.M1 / 2 + 1), dim3(p.threads_b), lds_b, s, a, p.M1,
+                       p.M2, st2, p.twM, p.twN, p.T, p.S);
+    launch(fft2_c_kernel, dim3(cols), dim3(kColThreads), lds_ac, s, last, a, p.M1, p.M2, st1,
+           p.twM, p.S);
+    return blmi_check_launch("induced_fourstep");
 }
 
 }  // namespace blmi

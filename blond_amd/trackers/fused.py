@@ -26,6 +26,7 @@ from ..impedances.impedance import InducedVoltageFreq
 from ..utils import bmath as bm
 
 _SOLVER = {"simple": 0, "legacy": 1, "exact": 2}
+_METHODS = {"fft": 0, "direct": 1, "fourstep": 2}
 
 
 def _ptr(t) -> ct.c_void_p:
@@ -38,7 +39,7 @@ def _dptr(a: np.ndarray):
 
 class FusedTracker:
     def __init__(self, tracker, profile=None, total_induced_voltage=None, use_graph=True,
-                 block=1024, blocks_per_cu=1, graph_chunk=50, groups=8, max_peel=0,
+                 block=1024, blocks_per_cu=1, graph_chunk=50, groups=0, max_peel=0,
                  method=None):
         import torch
 
@@ -87,16 +88,20 @@ class FusedTracker:
                   int(graph_chunk))
         _lib.call("blmi_engine_set_hist", self._h, int(groups), int(max_peel))
         if method is not None and self.tiv is not None:
-            _lib.call("blmi_engine_set_method", self._h, {"fft": 0, "direct": 1}[method])
+            _lib.call("blmi_engine_set_method", self._h, _METHODS[method])
         _lib.call("blmi_engine_set_turn", self._h, int(rp.counter[0]))
         self._primed = False
 
     @property
     def method(self) -> str | None:
-        """Induced-voltage method in use: 'fft', 'direct' or None."""
+        """Induced-voltage method in use: 'fft' (one-workgroup LDS FFT),
+        'direct' (convolution over all CUs), 'fourstep' (multi-workgroup FFT)
+        or None without intensity effects."""
+        if self.tiv is None:
+            return None
         m = ct.c_int()
         _lib.call("blmi_engine_get_method", self._h, ct.byref(m))
-        return {0: "fft", 1: "direct"}.get(m.value)
+        return {v: k for k, v in _METHODS.items()}.get(m.value)
 
     def time_kernels(self, turns: int):
         """Track ``turns`` eager turns timing each kernel with HIP events on

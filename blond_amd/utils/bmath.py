@@ -241,8 +241,8 @@ def rfftfreq(n: int, d: float = 1.0):
 
 
 def induced_voltage_1turn(profile, total_impedance, n_fft: int, scale: float, n_out: int = 0):
-    """-(scale * irfft(Z * rfft(profile, n_fft)))[:n_out] in one launch
-    (impedance.py:457-489 with scale = charge * e * ratio; n_out defaults to
+    """-(scale * irfft(Z * rfft(profile, n_fft)))[:n_out]: one launch on one
+    workgroup, or three (four-step FFT) beyond its LDS (impedance.py:457-489 with scale = charge * e * ratio; n_out defaults to
     len(profile))."""
     torch = _torch()
     p = _Staged(torch, profile, False, torch.float64, "profile")

@@ -117,7 +117,10 @@ int blmi_irfft(const double *in, int64_t m_in, double *out, int64_t n, void *str
  * with scale = charge * e * ratio and n_out = n_induced_voltage (or n_slices
  * for the cut of impedance.py:113-131).  Replaces
  * _InducedVoltage.induced_voltage_1turn, blond/impedances/impedance.py:457-489.
- * Z: n_fft/2+1 complex (interleaved), already divided by bin_size (impedance.py:890). */
+ * Z: n_fft/2+1 complex (interleaved), already divided by bin_size (impedance.py:890).
+ * n_fft up to blmi_fft_max_lds_length runs on one workgroup; longer ones run
+ * as a four-step transform whose scratch is kept per (device, n_fft, stream):
+ * calls with the same n_fft on one stream are ordered, so they never race. */
 int blmi_induced_voltage(const double *profile, int n_slices, const double *Z, int64_t n_fft,
                          double scale, double *volt, int n_out, void *stream);
 
@@ -150,12 +153,15 @@ int blmi_engine_set_induced(void *engine, int64_t n_fft, const double *Z_host, d
  * and the number of turns per captured graph. */
 int blmi_engine_set_launch(void *engine, int block, int blocks_per_cu, int graph_chunk);
 /* Induced-voltage method: 0 = one-workgroup LDS FFT (rfft, x Z, irfft),
- * 1 = direct convolution over all CUs (n_slices <= 8192).  set_induced picks
- * 1 when available. */
+ * 1 = direct convolution over all CUs (n_slices <= 8192), 2 = four-step FFT
+ * over many workgroups (any 5-smooth n_fft with a factor 16; the particle
+ * kernel then evaluates the interpolation slope and intercept from V and the
+ * bin centres itself).  set_induced picks 1, else 0, else 2. */
 int blmi_engine_set_method(void *engine, int method);
 int blmi_engine_get_method(void *engine, int *method);
-/* Histogram: `groups` copies of the global counts (1..16; workgroup b adds into
- * copy b % groups, the consumer sums them) and `max_peel` wavefront-aggregation
+/* Histogram: `groups` copies of the global counts (1..16, or 0 = automatic:
+ * 4 up to 1024 slices, else 1; workgroup b adds into copy b % groups, the
+ * consumer sums them) and `max_peel` wavefront-aggregation
  * steps (0..64) before plain LDS atomics.  Changing `groups` drops the staged
  * counts of the current turn (prime again). */
 int blmi_engine_set_hist(void *engine, int groups, int max_peel);

@@ -90,12 +90,48 @@ def _check_intensity(w, ref, rtol):
     assert np.max(np.abs(v - ref["induced_voltage"])) <= 1e-10 * np.abs(ref["induced_voltage"]).max()
 
 
-@pytest.mark.parametrize("n_slices,n_fft", [(256, 1024), (4096, 16384)])
-def test_intensity_turns_fused(n_slices, n_fft):
+@pytest.mark.parametrize("n_slices,n_fft,method", [(256, 1024, None), (4096, 16384, None),
+                                                   (4096, 16384, "fft"),
+                                                   (4096, 16384, "fourstep")])
+def test_intensity_turns_fused(n_slices, n_fft, method):
     w = build_workload("c2", n_particles=40001, n_turns=30, n_slices=n_slices, n_fft=n_fft)
     ref = orc.track(w.snapshot(), 20)
-    w.fused(20)
+    ft = w.fused(20, method=method)
+    assert ft.method == (method or "direct")
     _check_intensity(w, ref, 1e-9)
+
+
+def test_intensity_turns_fused_2e16_slices():
+    """The large-bunch shape: 65536 slices (u16-pair LDS histogram), FFT of
+    131072 (four-step transform), interpolation evaluated from V itself."""
+    w = build_workload("c3", n_particles=200001, n_turns=12, n_resonators=20)
+    ref = orc.track(w.snapshot(), 8)
+    ft = w.fused(8)
+    assert ft.method == "fourstep"
+    _check_intensity(w, ref, 1e-9)
+
+
+def test_intensity_first_turn_exact_2e16_slices():
+    w = build_workload("c3", n_particles=300001, n_turns=4)
+    ref = orc.track(w.snapshot(), 1)
+    w.fused(1)
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
+    v = host(w.tiv.induced_voltage)
+    assert np.max(np.abs(v - ref["induced_voltage"])) <= 1e-10 * np.abs(ref["induced_voltage"]).max()
+    sc = np.abs(ref["dE"]).max()
+    np.testing.assert_allclose(host(w.beam.dE), ref["dE"], rtol=0, atol=1e-12 * sc)
+
+
+@pytest.mark.parametrize("n_particles", [100001, 17_000_001])
+def test_plain_turns_2e16_slices_bit_exact(n_particles):
+    """65536 slices count in u16 pairs in LDS; above 65535 particles per
+    workgroup (17e6 on 256 CUs) the grid grows so no counter can overflow."""
+    w = build_workload("c1", n_particles=n_particles, n_turns=4, n_slices=65536)
+    ref = orc.track(w.snapshot(), 2)
+    w.fused(2)
+    np.testing.assert_array_equal(host(w.beam.dt), ref["dt"])
+    np.testing.assert_array_equal(host(w.beam.dE), ref["dE"])
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
 
 
 def test_intensity_turns_object_by_object():
