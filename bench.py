@@ -14,6 +14,15 @@ Prints one JSON line with the contract keys plus
   cpu_baseline  the reference's own C++ kernels (oracle/_ref, built with the
                 flags of `blond-compile --parallel --optimize`) on the host
                 cores, same turn loop with numpy FFTs, bounded sample
+  forms         the other forms of the metric, each on its own workload:
+                  large   c3: 1e7 macroparticles, 2 RF systems, 65536 slices,
+                          FFT 131072 (four-step), fused intensity turn
+                  batched c4: 256 bunches (1e4..1e6 each), one launch pair per
+                          turn for all bunches, autotuned, against 256
+                          sequential fused runs of the same bunches
+                  adjoint c5: 1e6 macroparticles x 500 turns, forward +
+                          checkpointed reverse pass, 112 B per
+                          macroparticle-turn
 """
 from __future__ import annotations
 
@@ -48,7 +57,106 @@ def parse():
     ap.add_argument("--groups", type=int, default=0, help="count copies (0: automatic)")
     ap.add_argument("--max-peel", type=int, default=0)
     ap.add_argument("--method", choices=["auto", "fft", "direct", "fourstep"], default="auto")
+    ap.add_argument("--forms", default="large,batched,adjoint",
+                    help="comma list of extra forms to measure ('' for none)")
     return ap.parse_args()
+
+
+def _timed(fn, sync):
+    sync()
+    t0 = time.perf_counter()
+    fn()
+    sync()
+    return time.perf_counter() - t0
+
+
+def form_large(torch, turns=200) -> dict:
+    """c3: 1e7 macroparticles, 65536 slices, FFT 131072, fused intensity turn."""
+    from blond_amd.trackers.fused import FusedTracker
+    from blond_amd.workloads import build_workload
+
+    w = build_workload("c3", n_particles=10_000_000, n_turns=turns + 80)
+    n = w.beam.n_macroparticles
+    ft = FusedTracker(w.tracker, w.profile, w.tiv)
+    ft.track(10)
+    el = _timed(lambda: ft.track(turns), torch.cuda.synchronize)
+    ms_ind, ms_turn = ft.time_kernels(50)
+    out = {"workload": "c3: 1e7 macroparticles, h=35640+142560, 65536 slices, n_fft 131072",
+           "value": n * turns / el, "unit": "macroparticle-turns/s", "turns": turns,
+           "ms_per_turn": 1e3 * el / turns, "induced_method": ft.method,
+           "particle_kernel_ms": ms_turn, "induced_ms": ms_ind,
+           "particle_kernel_frac": BYTES_PER_PARTICLE_TURN * n / (ms_turn * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "whole_turn_frac": BYTES_PER_PARTICLE_TURN * n / (el / turns) / 1e9 / HBM_PEAK_GBS}
+    ft.close()
+    return out
+
+
+def form_batched(torch, turns=100, seq_turns=20) -> dict:
+    """c4: 256 bunches in one launch pair per turn vs 256 sequential fused runs."""
+    from blond_amd.trackers.fused import FusedTracker
+    from blond_amd.workloads import build_ensemble
+
+    E = build_ensemble(n_bunches=256, n_turns=turns + seq_turns + 40)
+    n = E.n_particles
+    et = E.tracker()
+    blk, wpc, _ = et.autotune(turns=3)
+    et.track(5)
+    el = _timed(lambda: et.track(turns), torch.cuda.synchronize)
+    ms_ind, ms_turn = et.time_turns(10)
+    et.close()
+    # sequential: each bunch alone through the single-bunch fused engine
+    # (LDS FFT, the batched kernel's method), seq_turns turns each
+    fts = []
+    for k in range(len(E.trackers)):
+        w = E.bunch(k)
+        fts.append(FusedTracker(w.tracker, w.profile, w.tiv, method="fft", graph_chunk=seq_turns))
+    for f in fts:
+        f.track(2)
+
+    def seq():
+        for f in fts:
+            f.track(seq_turns)
+    el_seq = _timed(seq, torch.cuda.synchronize)
+    for f in fts:
+        f.close()
+    batched = n * turns / el
+    sequential = n * seq_turns / el_seq
+    return {"workload": "c4: 256 bunches, N_b = 10**U(4,6), 1024 slices, 5 resonators each",
+            "macroparticles": n, "value": batched, "unit": "macroparticle-turns/s",
+            "turns": turns, "ms_per_turn": 1e3 * el / turns,
+            "autotuned": {"block": blk, "wg_per_cu": wpc},
+            "particle_kernel_ms": ms_turn, "induced_ms": ms_ind,
+            "particle_kernel_frac": BYTES_PER_PARTICLE_TURN * n / (ms_turn * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "sequential_value": sequential, "sequential_turns": seq_turns,
+            "speedup_vs_sequential": batched / sequential}
+
+
+def form_adjoint(torch, turns=500) -> dict:
+    """c5: forward + checkpointed adjoint, 1e6 macroparticles x 500 turns."""
+    from blond_amd.trackers.differentiable import DifferentiableTracker
+    from blond_amd.workloads import build_adjoint_case
+
+    c = build_adjoint_case(n_particles=1_000_000, n_turns=turns)
+    dtr = DifferentiableTracker(c.tracker, c.profile, checkpoint_every=50)
+    ts = c.theta_star
+    mk = lambda th, g: [torch.tensor(th[k], dtype=torch.float64, device="cuda", requires_grad=g)
+                        for k in ("V", "phi", "R", "fr", "Q")]
+    with torch.no_grad():
+        target = dtr.profile_after(*mk(ts, False)).clone()
+
+    def step():
+        ps = mk(c.theta0, True)
+        loss = torch.sum((dtr.profile_after(*ps) - target) ** 2)
+        loss.backward()
+    step()
+    el = _timed(step, torch.cuda.synchronize)
+    n = c.beam.n_macroparticles
+    bytes_ = 112 * n * turns
+    dtr.close()
+    return {"workload": "c5: 1e6 macroparticles x 500 turns, 256 slices, 1 broadband resonator",
+            "value": n * turns / el, "unit": "macroparticle-turns/s (forward + reverse)",
+            "seconds": el, "bytes_per_macroparticle_turn": 112,
+            "whole_frac": bytes_ / el / 1e9 / HBM_PEAK_GBS}
 
 
 def cpu_baseline(snapshot: dict, target_s: float) -> dict:
@@ -141,8 +249,19 @@ def main():
                      "whole_turn_gbs": turn_gbs, "whole_turn_frac": turn_gbs / HBM_PEAK_GBS},
     }
     ft.close()
+    del w
     if snap is not None:
         line["cpu_baseline"] = cpu_baseline(snap, a.cpu_seconds)
+    forms = {}
+    for name in [f for f in a.forms.split(",") if f]:
+        fn = {"large": form_large, "batched": form_batched, "adjoint": form_adjoint}[name]
+        try:
+            forms[name] = fn(torch)
+        except Exception as exc:  # a form that cannot run is reported, not hidden
+            forms[name] = {"error": f"{type(exc).__name__}: {exc}"}
+        print(f"form {name} done", file=sys.stderr, flush=True)
+    if forms:
+        line["forms"] = forms
     print(json.dumps(line), flush=True)
 
 

@@ -56,10 +56,33 @@ SIGNATURES = {
     "blmi_engine_set_method": [_p, _i],
     "blmi_engine_get_method": [_p, ct.POINTER(_i)],
     "blmi_engine_set_hist": [_p, _i, _i],
+    "blmi_ensemble_create": [_i, _i, _i, _i],
+    "blmi_ensemble_destroy": [_p],
+    "blmi_ensemble_set_particles": [_p, _p, _p, _p, _p],
+    "blmi_ensemble_set_programs": [_p, _p, _p, _p, _d, _d, _p, _p, _p, _p, _p, _p, _p, _p, _d],
+    "blmi_ensemble_set_slices": [_p, _i, _p, _p, _p],
+    "blmi_ensemble_set_induced": [_p, _i64, _p, _p, _p, _p],
+    "blmi_ensemble_set_launch": [_p, _i, _i, _i, _i],
+    "blmi_ensemble_get_launch": [_p, ct.POINTER(_i), ct.POINTER(_i)],
+    "blmi_ensemble_set_turn": [_p, _i],
+    "blmi_ensemble_prime": [_p, _p],
+    "blmi_ensemble_track": [_p, _i, _i, _p],
+    "blmi_ensemble_time_turns": [_p, _i, ct.POINTER(_f), ct.POINTER(_f), _p],
+    "blmi_ensemble_autotune": [_p, _i, ct.POINTER(_i), ct.POINTER(_i), ct.POINTER(_f), _p],
+    "blmi_adjoint_create": [_i64, _i, _i, _i, _i],
+    "blmi_adjoint_destroy": [_p],
+    "blmi_adjoint_set_particles": [_p, _p, _p],
+    "blmi_adjoint_set_programs": [_p, _p, _p, _p, _d, _d, _p, _p, _p, _p, _p, _p, _p, _p, _d],
+    "blmi_adjoint_set_slices": [_p, _d, _d, _p],
+    "blmi_adjoint_set_wake": [_p, _i64, _p, _d],
+    "blmi_adjoint_forward": [_p, _i, _p, _p],
+    "blmi_adjoint_backward": [_p, _p, _p, _p, _p],
 }
 _RESTYPES = {
     "blmi_last_error": ct.c_char_p,
     "blmi_engine_create": ct.c_void_p,
+    "blmi_ensemble_create": ct.c_void_p,
+    "blmi_adjoint_create": ct.c_void_p,
 }
 
 

@@ -1,0 +1,504 @@
+// Differentiable tracking on the MI355X: forward turns, checkpoints and the
+// hand-written reverse (adjoint) pass.
+//
+// The map per turn t (the intensity map of the fused engine, with the
+// profile deposited continuously so the loss is differentiable):
+//   p_t     = D(dt_t)        cloud-in-cell deposit on bin centres: a
+//                            particle at fractional bin position x - 1/2 =
+//                            k + d adds 1-d to bin k and d to bin k+1
+//   V_t     = -scale * (g conv p_t)[:ns],  g = irfft(Z, n_fft) (circular,
+//                            as the FFT route of impedance.py:457-489)
+//   e1      = dE + (dt * slope_k + intercept_k)   (linear_interp_kick.cpp:44-64)
+//   e2      = e1 + q sum_j V_j sin(w_j dt + phi_j) + acc   (kick.cpp:16-35)
+//   dt_{t+1} = dt + coeff_{t+1} e2,  dE_{t+1} = e2          (drift.cpp "simple")
+// and the loss depends on p_T.  The reference's smooth_histogram
+// (histogram.cpp:75-132) moves a particle's whole weight between bins k-1
+// and k+1 when it crosses the centre of bin k, so it has no derivative
+// there; the deposit here is the continuous linear one.
+//
+// Reverse turn t, per particle (state at t recomputed from a checkpoint):
+//   lx <- lx + (lp_{t+1}[k'+1] - lp_{t+1}[k']) inv_bw      deposit of p_{t+1}
+//   le2 = le + coeff lx
+//   lx <- lx + le2 (q sum_j V_j w_j cos(.) + slope_k)       kick + interp
+//   lV_t[k] += le2 q (1-w), lV_t[k+1] += le2 q w             interp
+//   dL/dV_j += le2 q sin(.),  dL/dphi_j += le2 q V_j cos(.)
+//   le <- le2
+// then, once per turn on one workgroup: lp_t = -scale * (g corr lV_t) and
+// corr[m] += sum_i lV_t[i] p_t[(i - m) mod n_fft], from which the host
+// forms dL/dZ-parameters as -scale * sum_m irfft(dZ/dtheta)[m] corr[m].
+//
+// Traffic per macroparticle-turn: 32 B forward, 32 B recompute (slot to
+// slot), 48 B reverse (state read, two adjoints read and written).
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "adjoint.h"
+#include "blmi_common.h"
+#include "particles.h"
+
+namespace blmi {
+
+int smooth_scale_bits(int64_t n);
+
+constexpr int kAdjMaxRf = 4;
+constexpr int kAdjBlock = 256;
+constexpr int kAdjOneBlock = 1024;
+
+struct AdjArgs {
+    int n_rf;
+    double charge;
+    const double *rf;  // [turn][3][n_rf]
+    const double *acc;
+    const DriftCoeffs<double> *drift;
+    int ns;
+    double cut_left, inv_bw_cut, fix_scale;
+    const double2 *tab;  // [turn][ns]
+    double bc0, inv_bw_bc;
+    const double *bc;
+    unsigned long long *dep;  // [copies][ns]
+    double *lv;               // [copies][ns]
+    const double *lp;         // [ns]
+    double *gpart;            // [turn][grid][2 n_rf]
+};
+
+// cloud-in-cell bin and weight: x - 1/2 = k + d in bin units
+__device__ __forceinline__ void cic(double x, const AdjArgs &a, int &k, double &d) {
+    const double xf = (x - a.cut_left) * a.inv_bw_cut - 0.5;
+    const double f = floor(xf);
+    d = xf - f;
+    k = (f >= -1.0 && f < (double)a.ns) ? (int)f : -2;  // -2: no bin touched
+}
+
+__device__ __forceinline__ void deposit_lds(unsigned long long *h, double x, const AdjArgs &a) {
+    int k;
+    double d;
+    cic(x, a, k, d);
+    if (k == -2) return;
+    if (k >= 0)
+        atomicAdd(&h[k], (unsigned long long)__double2ll_rn((1.0 - d) * a.fix_scale));
+    if (k + 1 < a.ns)
+        atomicAdd(&h[k + 1], (unsigned long long)__double2ll_rn(d * a.fix_scale));
+}
+
+__device__ __forceinline__ void deposit_flush(unsigned long long *h, const AdjArgs &a) {
+    __syncthreads();
+    unsigned long long *out = a.dep + (size_t)(blockIdx.x % kAdjCopies) * a.ns;
+    for (int b = threadIdx.x; b < a.ns; b += blockDim.x)
+        if (h[b]) atomic_add_agent(&out[b], h[b]);
+}
+
+__global__ void __launch_bounds__(kAdjBlock) adj_deposit_kernel(const double *__restrict__ x,
+                                                                int64_t n, AdjArgs a) {
+    extern __shared__ unsigned long long hdep[];
+    for (int b = threadIdx.x; b < a.ns; b += blockDim.x) hdep[b] = 0ull;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        deposit_lds(hdep, x[i], a);
+    deposit_flush(hdep, a);
+}
+
+// Forward turn t: (xi, ei) -> (xo, eo) (may alias); DEPOSIT adds p_{t+1}.
+template <bool DEPOSIT>
+__global__ void __launch_bounds__(kAdjBlock) adj_turn_kernel(const double *xi, const double *ei,
+                                                             double *xo, double *eo, int64_t n,
+                                                             int t, AdjArgs a) {
+    extern __shared__ unsigned long long hdep[];
+    if (DEPOSIT) {
+        for (int b = threadIdx.x; b < a.ns; b += blockDim.x) hdep[b] = 0ull;
+        __syncthreads();
+    }
+    const double *V = a.rf + (size_t)t * 3 * a.n_rf;
+    const double *w = V + a.n_rf;
+    const double *phi = w + a.n_rf;
+    const double acc = a.acc[t];
+    const double coeff = a.drift[t + 1].coeff;
+    const double2 *tab = a.tab + (size_t)t * a.ns;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double x = xi[i], e = ei[i];
+        const double f = floor((x - a.bc0) * a.inv_bw_bc);
+        if (f >= 0.0 && f < (double)(a.ns - 1)) {
+            const double2 tk = tab[(int)f];
+            e = e + (x * tk.x + tk.y);
+        }
+        double s = 0.0;
+        for (int j = 0; j < a.n_rf; ++j) s = s + V[j] * cephes_sin(w[j] * x + phi[j]);
+        e = e + (a.charge * s + acc);
+        x = x + coeff * e;
+        xo[i] = x;
+        eo[i] = e;
+        if (DEPOSIT) deposit_lds(hdep, x, a);
+    }
+    if (DEPOSIT) deposit_flush(hdep, a);
+}
+
+// One workgroup: p_t from the deposit copies (cleared), stored; with a table,
+// V_t = -scale * (g conv p_t) and its interpolation table.
+__global__ void __launch_bounds__(kAdjOneBlock) adj_profile_kernel(AdjArgs a, int t, bool table,
+                                                                   double *prof, const double *g,
+                                                                   int n_fft, double scale,
+                                                                   double2 *tab_out) {
+    extern __shared__ double sh[];
+    double *p = sh, *V = sh + a.ns, *gs = sh + 2 * a.ns;
+    const double inv = 1.0 / a.fix_scale;
+    for (int j = threadIdx.x; j < a.ns; j += blockDim.x) {
+        unsigned long long s = 0;
+        for (int c = 0; c < kAdjCopies; ++c) {
+            s += a.dep[(size_t)c * a.ns + j];
+            a.dep[(size_t)c * a.ns + j] = 0ull;
+        }
+        p[j] = (double)(long long)s * inv;
+        prof[(size_t)t * a.ns + j] = p[j];
+    }
+    if (!table) return;
+    for (int m = threadIdx.x; m < n_fft; m += blockDim.x) gs[m] = g[m];
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.ns; i += blockDim.x) {
+        double v = 0.0;
+        for (int j = 0; j < a.ns; ++j) {
+            int m = i - j;
+            if (m < 0) m += n_fft;
+            v = fma(gs[m], p[j], v);
+        }
+        V[i] = -(scale * v);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.ns - 1; i += blockDim.x) {
+        const double k = a.charge * (V[i + 1] - V[i]) * a.inv_bw_bc;
+        tab_out[(size_t)t * a.ns + i] = make_double2(k, (a.charge * V[i] - a.bc[i] * k) + 0.0);
+    }
+}
+
+// Reverse turn t (state (x, e) at turn t): see the file comment.
+__global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__restrict__ xs,
+                                                                const double *__restrict__ es,
+                                                                double *__restrict__ lam_x,
+                                                                double *__restrict__ lam_e,
+                                                                int64_t n, int t, AdjArgs a) {
+    extern __shared__ double shr[];
+    double *lp = shr, *lvs = shr + a.ns;
+    __shared__ double red[kAdjBlock / 64][2 * kAdjMaxRf];
+    for (int b = threadIdx.x; b < a.ns; b += blockDim.x) {
+        lp[b] = a.lp[b];
+        lvs[b] = 0.0;
+    }
+    __syncthreads();
+    const double *V = a.rf + (size_t)t * 3 * a.n_rf;
+    const double *w = V + a.n_rf;
+    const double *phi = w + a.n_rf;
+    const double acc = a.acc[t];
+    const double coeff = a.drift[t + 1].coeff;
+    const double2 *tab = a.tab + (size_t)t * a.ns;
+    double gsum[2 * kAdjMaxRf];
+#pragma unroll
+    for (int j = 0; j < 2 * kAdjMaxRf; ++j) gsum[j] = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double x = xs[i], e = es[i];
+        // forward step t, as adj_turn_kernel
+        const double f = floor((x - a.bc0) * a.inv_bw_bc);
+        const bool in = f >= 0.0 && f < (double)(a.ns - 1);
+        const int k = in ? (int)f : 0;
+        double e1 = e;
+        double2 tk = make_double2(0.0, 0.0);
+        if (in) {
+            tk = tab[k];
+            e1 = e + (x * tk.x + tk.y);
+        }
+        double sn[kAdjMaxRf], cs[kAdjMaxRf];
+        double s = 0.0, ds = 0.0;
+        for (int j = 0; j < a.n_rf && j < kAdjMaxRf; ++j) {
+            cephes_sincos(w[j] * x + phi[j], sn[j], cs[j]);
+            s = s + V[j] * sn[j];
+            ds = ds + V[j] * w[j] * cs[j];
+        }
+        const double e2 = e1 + (a.charge * s + acc);
+        const double x1 = x + coeff * e2;
+        // deposit adjoint of p_{t+1} at x1
+        int kk;
+        double d;
+        cic(x1, a, kk, d);
+        double lx = lam_x[i];
+        if (kk != -2) {
+            const double l0 = kk >= 0 ? lp[kk] : 0.0;
+            const double l1 = kk + 1 < a.ns ? lp[kk + 1] : 0.0;
+            lx = lx + (l1 - l0) * a.inv_bw_cut;
+        }
+        const double le2 = lam_e[i] + coeff * lx;
+        lx = lx + le2 * (a.charge * ds);
+        for (int j = 0; j < a.n_rf && j < kAdjMaxRf; ++j) {
+            gsum[2 * j] += le2 * a.charge * sn[j];
+            gsum[2 * j + 1] += le2 * a.charge * V[j] * cs[j];
+        }
+        if (in) {
+            lx = lx + le2 * tk.x;
+            const double wt = (x - a.bc[k]) * a.inv_bw_bc;
+            atomicAdd(&lvs[k], le2 * a.charge * (1.0 - wt));
+            atomicAdd(&lvs[k + 1], le2 * a.charge * wt);
+        }
+        lam_x[i] = lx;
+        lam_e[i] = le2;
+    }
+    // per-workgroup RF gradient sums (fixed order: wavefront tree, then waves)
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int j = 0; j < 2 * a.n_rf && j < 2 * kAdjMaxRf; ++j) {
+        double v = gsum[j];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) red[wid][j] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < 2 * a.n_rf) {
+        double v = 0.0;
+        for (int q = 0; q < kAdjBlock / 64; ++q) v += red[q][threadIdx.x];
+        a.gpart[((size_t)t * gridDim.x + blockIdx.x) * 2 * a.n_rf + threadIdx.x] = v;
+    }
+    double *out = a.lv + (size_t)(blockIdx.x % kAdjCopies) * a.ns;
+    for (int b = threadIdx.x; b < a.ns; b += blockDim.x)
+        if (lvs[b] != 0.0) atomic_add_agent(&out[b], lvs[b]);
+}
+
+// One workgroup: lV_t (copies summed, cleared) -> lp_t = -scale (g corr lV_t),
+// corr[m] += sum_i lV_t[i] p_t[(i - m) mod n_fft].
+__global__ void __launch_bounds__(kAdjOneBlock) adj_iv_kernel(AdjArgs a, int t, const double *prof,
+                                                              const double *g, int n_fft,
+                                                              double scale, double *lp_out,
+                                                              double *corr) {
+    extern __shared__ double sh[];
+    double *lvs = sh, *p = sh + a.ns, *gs = sh + 2 * a.ns;
+    for (int i = threadIdx.x; i < a.ns; i += blockDim.x) {
+        double s = 0.0;
+        for (int c = 0; c < kAdjCopies; ++c) {
+            s += a.lv[(size_t)c * a.ns + i];
+            a.lv[(size_t)c * a.ns + i] = 0.0;
+        }
+        lvs[i] = s;
+        p[i] = prof[(size_t)t * a.ns + i];
+    }
+    for (int m = threadIdx.x; m < n_fft; m += blockDim.x) gs[m] = g[m];
+    __syncthreads();
+    for (int j = threadIdx.x; j < a.ns; j += blockDim.x) {
+        double v = 0.0;
+        for (int i = 0; i < a.ns; ++i) {
+            int m = i - j;
+            if (m < 0) m += n_fft;
+            v = fma(gs[m], lvs[i], v);
+        }
+        lp_out[j] = -(scale * v);
+    }
+    for (int m = threadIdx.x; m < n_fft; m += blockDim.x) {
+        double v = 0.0;
+        for (int i = 0; i < a.ns; ++i) {
+            int j = i - m;
+            if (j < 0) j += n_fft;
+            if (j < a.ns) v = fma(lvs[i], p[j], v);
+        }
+        corr[m] += v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+Adjoint::~Adjoint() { release(); }
+
+void Adjoint::release() {
+    void *bufs[] = {d_rf, d_acc, d_drift, d_x0, d_work, d_ckpt, d_slots, d_lam, d_dep,
+                    d_prof, d_tab, d_bc, d_g, d_lv, d_lp, d_corr, d_gpart};
+    for (void *p : bufs) (void)hipFree(p);
+    d_rf = d_acc = d_x0 = d_work = d_ckpt = d_slots = d_lam = nullptr;
+    d_prof = d_bc = d_g = d_lv = d_lp = d_corr = d_gpart = nullptr;
+    d_drift = nullptr;
+    d_dep = nullptr;
+    d_tab = nullptr;
+}
+
+int Adjoint::init(int64_t n_, int n_rf_, int n_turns_, int n_slices_, int every_) {
+    n = n_;
+    n_rf = n_rf_;
+    n_turns = n_turns_;
+    n_slices = n_slices_;
+    every = every_;
+    BLMI_HIP_TRY(hipGetDevice(&device));
+    grid = device_cu_count() * 4;
+    const size_t nt = (size_t)n_turns + 1;
+    const size_t pn = 2 * (size_t)n;
+    const int n_ckpt = (n_turns + every - 1) / every;
+    BLMI_HIP_TRY(hipMalloc(&d_rf, sizeof(double) * 3 * n_rf * nt));
+    BLMI_HIP_TRY(hipMalloc(&d_acc, sizeof(double) * nt));
+    BLMI_HIP_TRY(hipMalloc(&d_drift, sizeof(DriftCoeffs<double>) * nt));
+    BLMI_HIP_TRY(hipMalloc(&d_x0, sizeof(double) * pn));
+    BLMI_HIP_TRY(hipMalloc(&d_work, sizeof(double) * pn));
+    BLMI_HIP_TRY(hipMalloc(&d_ckpt, sizeof(double) * pn * n_ckpt));
+    BLMI_HIP_TRY(hipMalloc(&d_slots, sizeof(double) * pn * every));
+    BLMI_HIP_TRY(hipMalloc(&d_lam, sizeof(double) * pn));
+    BLMI_HIP_TRY(hipMalloc(&d_dep, sizeof(unsigned long long) * kAdjCopies * n_slices));
+    BLMI_HIP_TRY(hipMemset(d_dep, 0, sizeof(unsigned long long) * kAdjCopies * n_slices));
+    BLMI_HIP_TRY(hipMalloc(&d_prof, sizeof(double) * nt * n_slices));
+    BLMI_HIP_TRY(hipMalloc(&d_tab, sizeof(double2) * nt * n_slices));
+    BLMI_HIP_TRY(hipMemset(d_tab, 0, sizeof(double2) * nt * n_slices));
+    BLMI_HIP_TRY(hipMalloc(&d_bc, sizeof(double) * n_slices));
+    BLMI_HIP_TRY(hipMalloc(&d_lv, sizeof(double) * kAdjCopies * n_slices));
+    BLMI_HIP_TRY(hipMemset(d_lv, 0, sizeof(double) * kAdjCopies * n_slices));
+    BLMI_HIP_TRY(hipMalloc(&d_lp, sizeof(double) * n_slices));
+    BLMI_HIP_TRY(hipMalloc(&d_gpart, sizeof(double) * (size_t)n_turns * grid * 2 * n_rf));
+    fixed_bits = smooth_scale_bits(n);
+    return BLMI_OK;
+}
+
+int Adjoint::set_particles(const double *dt, const double *dE) {
+    BLMI_HIP_TRY(hipMemcpy(d_x0, dt, sizeof(double) * n, hipMemcpyDeviceToDevice));
+    BLMI_HIP_TRY(hipMemcpy(d_x0 + n, dE, sizeof(double) * n, hipMemcpyDeviceToDevice));
+    forward_done = false;
+    return BLMI_OK;
+}
+
+int Adjoint::set_programs(const double *rf_host, const double *acc_host, const double *t_rev,
+                          double length_ratio, double alpha_order, const double *eta0,
+                          const double *eta1, const double *eta2, const double *alpha0,
+                          const double *alpha1, const double *alpha2, const double *beta,
+                          const double *energy, double charge_) {
+    const size_t nt = (size_t)n_turns + 1;
+    charge = charge_;
+    BLMI_HIP_TRY(hipMemcpy(d_rf, rf_host, sizeof(double) * 3 * n_rf * nt, hipMemcpyHostToDevice));
+    BLMI_HIP_TRY(hipMemcpy(d_acc, acc_host, sizeof(double) * nt, hipMemcpyHostToDevice));
+    std::vector<DriftCoeffs<double>> dc(nt);
+    for (size_t i = 0; i < nt; ++i)
+        dc[i] = make_drift_coeffs<double>(0, t_rev[i], length_ratio, alpha_order, eta0[i], eta1[i],
+                                          eta2[i], alpha0[i], alpha1[i], alpha2[i], beta[i],
+                                          energy[i]);
+    BLMI_HIP_TRY(hipMemcpy(d_drift, dc.data(), sizeof(DriftCoeffs<double>) * nt,
+                           hipMemcpyHostToDevice));
+    forward_done = false;
+    return BLMI_OK;
+}
+
+int Adjoint::set_slices(double cl, double cr, const double *bc_host) {
+    cut_left = cl;
+    inv_bw_cut = (double)n_slices / (cr - cl);
+    BLMI_HIP_TRY(hipMemcpy(d_bc, bc_host, sizeof(double) * n_slices, hipMemcpyHostToDevice));
+    bc0 = bc_host[0];
+    inv_bw_bc = (double)(n_slices - 1) / (bc_host[n_slices - 1] - bc_host[0]);
+    forward_done = false;
+    return BLMI_OK;
+}
+
+int Adjoint::set_wake(int64_t n_fft_, const double *g_host, double scale_) {
+    if (n_fft_ < n_slices || n_fft_ > 8192)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "adjoint: n_fft %lld outside [n_slices, 8192]",
+                              (long long)n_fft_);
+    if (n_fft_ != n_fft) {
+        (void)hipFree(d_g);
+        (void)hipFree(d_corr);
+        d_g = d_corr = nullptr;
+        BLMI_HIP_TRY(hipMalloc(&d_g, sizeof(double) * n_fft_));
+        BLMI_HIP_TRY(hipMalloc(&d_corr, sizeof(double) * n_fft_));
+    }
+    n_fft = n_fft_;
+    scale = scale_;
+    BLMI_HIP_TRY(hipMemcpy(d_g, g_host, sizeof(double) * n_fft, hipMemcpyHostToDevice));
+    forward_done = false;
+    return BLMI_OK;
+}
+
+static AdjArgs adj_args(const Adjoint &j) {
+    AdjArgs a{};
+    a.n_rf = j.n_rf;
+    a.charge = j.charge;
+    a.rf = j.d_rf;
+    a.acc = j.d_acc;
+    a.drift = j.d_drift;
+    a.ns = j.n_slices;
+    a.cut_left = j.cut_left;
+    a.inv_bw_cut = j.inv_bw_cut;
+    a.fix_scale = ldexp(1.0, j.fixed_bits);
+    a.tab = j.d_tab;
+    a.bc0 = j.bc0;
+    a.inv_bw_bc = j.inv_bw_bc;
+    a.bc = j.d_bc;
+    a.dep = j.d_dep;
+    a.lv = j.d_lv;
+    a.lp = j.d_lp;
+    a.gpart = j.d_gpart;
+    return a;
+}
+
+int Adjoint::forward(int turns, double *profile_out, hipStream_t s) {
+    if (!d_g) return blmi_set_error(BLMI_ERR_ARG, "adjoint: set the wake first");
+    if (turns < 1 || turns > n_turns)
+        return blmi_set_error(BLMI_ERR_ARG, "adjoint: turns %d outside [1, %d]", turns, n_turns);
+    const AdjArgs a = adj_args(*this);
+    const size_t pn = 2 * (size_t)n;
+    const size_t lds_dep = sizeof(unsigned long long) * n_slices;
+    const size_t lds_one = sizeof(double) * (2 * (size_t)n_slices + n_fft);
+    double *x = d_work, *e = d_work + n;
+    BLMI_HIP_TRY(hipMemcpyAsync(d_work, d_x0, sizeof(double) * pn, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(adj_deposit_kernel, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, n, a);
+    hipLaunchKernelGGL(adj_profile_kernel, dim3(1), dim3(kAdjOneBlock), lds_one, s, a, 0, true,
+                       d_prof, d_g, (int)n_fft, scale, d_tab);
+    for (int t = 0; t < turns; ++t) {
+        if (t % every == 0)
+            BLMI_HIP_TRY(hipMemcpyAsync(d_ckpt + (size_t)(t / every) * pn, d_work,
+                                        sizeof(double) * pn, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(adj_turn_kernel<true>, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, e, x,
+                           e, n, t, a);
+        hipLaunchKernelGGL(adj_profile_kernel, dim3(1), dim3(kAdjOneBlock), lds_one, s, a, t + 1,
+                           t + 1 < turns, d_prof, d_g, (int)n_fft, scale, d_tab);
+    }
+    int rc = blmi_check_launch("adjoint_forward");
+    if (rc) return rc;
+    BLMI_HIP_TRY(hipMemcpyAsync(profile_out, d_prof + (size_t)turns * n_slices,
+                                sizeof(double) * n_slices, hipMemcpyDeviceToDevice, s));
+    forward_done = true;
+    forward_turns = turns;
+    return BLMI_OK;
+}
+
+int Adjoint::backward(const double *dprofile, double *grad_rf_host, double *corr_host,
+                      hipStream_t s) {
+    if (!forward_done) return blmi_set_error(BLMI_ERR_ARG, "adjoint: run forward first");
+    const int T = forward_turns;
+    const size_t pn = 2 * (size_t)n;
+    const size_t lds_dep = sizeof(unsigned long long) * n_slices;
+    const size_t lds_rev = sizeof(double) * 2 * (size_t)n_slices;
+    const size_t lds_one = sizeof(double) * (2 * (size_t)n_slices + n_fft);
+    AdjArgs a = adj_args(*this);
+    BLMI_HIP_TRY(hipMemsetAsync(d_lam, 0, sizeof(double) * pn, s));
+    BLMI_HIP_TRY(hipMemsetAsync(d_corr, 0, sizeof(double) * n_fft, s));
+    BLMI_HIP_TRY(hipMemcpyAsync(d_lp, dprofile, sizeof(double) * n_slices, hipMemcpyDeviceToDevice,
+                                s));
+    for (int seg = (T - 1) / every; seg >= 0; --seg) {
+        const int t0 = seg * every, t1 = (t0 + every < T) ? t0 + every : T;
+        // states t0 .. t1-1 into slots 0 .. t1-t0-1
+        BLMI_HIP_TRY(hipMemcpyAsync(d_slots, d_ckpt + (size_t)seg * pn, sizeof(double) * pn,
+                                    hipMemcpyDeviceToDevice, s));
+        for (int t = t0; t < t1 - 1; ++t) {
+            double *si = d_slots + (size_t)(t - t0) * pn, *so = si + pn;
+            hipLaunchKernelGGL(adj_turn_kernel<false>, dim3(grid), dim3(kAdjBlock), lds_dep, s, si,
+                               si + n, so, so + n, n, t, a);
+        }
+        for (int t = t1 - 1; t >= t0; --t) {
+            const double *st = d_slots + (size_t)(t - t0) * pn;
+            hipLaunchKernelGGL(adj_reverse_kernel, dim3(grid), dim3(kAdjBlock), lds_rev, s, st,
+                               st + n, d_lam, d_lam + n, n, t, a);
+            hipLaunchKernelGGL(adj_iv_kernel, dim3(1), dim3(kAdjOneBlock), lds_one, s, a, t, d_prof,
+                               d_g, (int)n_fft, scale, d_lp, d_corr);
+        }
+    }
+    int rc = blmi_check_launch("adjoint_backward");
+    if (rc) return rc;
+    std::vector<double> gp((size_t)T * grid * 2 * n_rf);
+    BLMI_HIP_TRY(hipMemcpyAsync(gp.data(), d_gpart, sizeof(double) * gp.size(),
+                                hipMemcpyDeviceToHost, s));
+    BLMI_HIP_TRY(hipMemcpyAsync(corr_host, d_corr, sizeof(double) * n_fft, hipMemcpyDeviceToHost, s));
+    BLMI_HIP_TRY(hipStreamSynchronize(s));
+    for (int j = 0; j < 2 * n_rf; ++j) {
+        double v = 0.0;
+        for (int t = 0; t < T; ++t)
+            for (int b = 0; b < grid; ++b) v += gp[((size_t)t * grid + b) * 2 * n_rf + j];
+        grad_rf_host[j] = v;
+    }
+    return BLMI_OK;
+}
+
+}  // namespace blmi

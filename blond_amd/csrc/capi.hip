@@ -16,6 +16,8 @@
 
 #include "blmi_common.h"
 #include "engine.h"
+#include "ensemble.h"
+#include "adjoint.h"
 #include "particles.h"
 #include "spectral.h"
 
@@ -451,6 +453,172 @@ int blmi_engine_time_kernels(void *h, int turns, float *ms_induced, float *ms_tu
     Engine *e = reinterpret_cast<Engine *>(h);
     BLMI_REQUIRE(e && ms_induced && ms_turn, "engine: null argument");
     return e->time_kernels(turns, S(stream), ms_induced, ms_turn);
+}
+
+// ------------------------------------------------------------- ensemble ----
+void *blmi_ensemble_create(int n_bunches, int n_rf, int n_turns, int solver) {
+    if (n_bunches < 1 || n_rf < 0 || n_turns < 1 || solver < 0 || solver > 2) {
+        blmi_set_error(BLMI_ERR_ARG, "ensemble_create: bad arguments");
+        return nullptr;
+    }
+    Ensemble *e = new Ensemble();
+    if (e->init(n_bunches, n_rf, n_turns, solver) != BLMI_OK) {
+        delete e;
+        return nullptr;
+    }
+    return e;
+}
+
+int blmi_ensemble_destroy(void *h) {
+    delete reinterpret_cast<Ensemble *>(h);
+    return BLMI_OK;
+}
+
+int blmi_ensemble_set_particles(void *h, double *dt, double *dE, const int64_t *offsets,
+                                const int64_t *counts) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(h);
+    BLMI_REQUIRE(e, "ensemble: null handle");
+    BLMI_REQUIRE(dt && dE && offsets && counts, "ensemble: null particle arrays");
+    return e->set_particles(dt, dE, offsets, counts);
+}
+
+int blmi_ensemble_set_programs(void *h, const double *rf, const double *acc, const double *t_rev,
+                               double length_ratio, double alpha_order, const double *eta0,
+                               const double *eta1, const double *eta2, const double *alpha0,
+                               const double *alpha1, const double *alpha2, const double *beta,
+                               const double *energy, double charge) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(h);
+    BLMI_REQUIRE(e, "ensemble: null handle");
+    return e->set_programs(rf, acc, t_rev, length_ratio, alpha_order, eta0, eta1, eta2, alpha0,
+                           alpha1, alpha2, beta, energy, charge);
+}
+
+int blmi_ensemble_set_slices(void *h, int n_slices, const double *cut_left,
+                             const double *cut_right, double *profiles) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(h);
+    BLMI_REQUIRE(e, "ensemble: null handle");
+    BLMI_REQUIRE(n_slices >= 2 && cut_left && cut_right && profiles, "ensemble: bad slices");
+    for (int b = 0; b < e->n_bunches; ++b)
+        BLMI_REQUIRE(cut_right[b] > cut_left[b], "ensemble: bunch %d: cut_right <= cut_left", b);
+    return e->set_slices(n_slices, cut_left, cut_right, profiles);
+}
+
+int blmi_ensemble_set_induced(void *h, int64_t n_fft, const double *Z_host, const double *scale,
+                              double *volt, const double *bin_centers) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(h);
+    BLMI_REQUIRE(e, "ensemble: null handle");
+    BLMI_REQUIRE(Z_host && scale && volt && bin_centers, "ensemble: null induced-voltage arrays");
+    return e->set_induced(n_fft, Z_host, scale, volt, bin_centers);
+}
+
+int blmi_ensemble_set_launch(void *h, int block, int wg_per_cu, int graph_chunk, int max_peel) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(h);
+    BLMI_REQUIRE(e, "ensemble: null handle");
+    return e->set_launch(block, wg_per_cu, graph_chunk, max_peel);
+}
+
+int blmi_ensemble_get_launch(void *h, int *block, int *wg_per_cu) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(h);
+    BLMI_REQUIRE(e && block && wg_per_cu, "ensemble: null argument");
+    *block = e->block;
+    *wg_per_cu = e->wg_per_cu;
+    return BLMI_OK;
+}
+
+int blmi_ensemble_set_turn(void *h, int turn) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(h);
+    BLMI_REQUIRE(e, "ensemble: null handle");
+    return e->set_turn(turn);
+}
+
+int blmi_ensemble_prime(void *h, void *stream) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(h);
+    BLMI_REQUIRE(e, "ensemble: null handle");
+    return e->prime(S(stream));
+}
+
+int blmi_ensemble_track(void *h, int turns, int use_graph, void *stream) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(h);
+    BLMI_REQUIRE(e, "ensemble: null handle");
+    BLMI_REQUIRE(turns >= 0, "ensemble: negative turn count");
+    return e->track(turns, S(stream), use_graph != 0);
+}
+
+int blmi_ensemble_time_turns(void *h, int turns, float *ms_induced, float *ms_turn, void *stream) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(h);
+    BLMI_REQUIRE(e && ms_induced && ms_turn, "ensemble: null argument");
+    return e->time_turns(turns, S(stream), ms_induced, ms_turn);
+}
+
+int blmi_ensemble_autotune(void *h, int turns, int *block, int *wg_per_cu, float *ms_per_turn,
+                           void *stream) {
+    Ensemble *e = reinterpret_cast<Ensemble *>(h);
+    BLMI_REQUIRE(e && block && wg_per_cu && ms_per_turn, "ensemble: null argument");
+    return e->autotune(turns, S(stream), block, wg_per_cu, ms_per_turn);
+}
+
+// ------------------------------------------------------------- adjoint -----
+void *blmi_adjoint_create(int64_t n, int n_rf, int n_turns, int n_slices, int checkpoint_every) {
+    if (n < 1 || n_rf < 1 || n_rf > 4 || n_turns < 1 || n_slices < 2 ||
+        n_slices > kAdjMaxSlices || checkpoint_every < 1) {
+        blmi_set_error(BLMI_ERR_ARG, "adjoint_create: bad arguments (n_rf 1..4, 2..%d slices)",
+                       kAdjMaxSlices);
+        return nullptr;
+    }
+    Adjoint *a = new Adjoint();
+    if (a->init(n, n_rf, n_turns, n_slices, checkpoint_every) != BLMI_OK) {
+        delete a;
+        return nullptr;
+    }
+    return a;
+}
+
+int blmi_adjoint_destroy(void *h) {
+    delete reinterpret_cast<Adjoint *>(h);
+    return BLMI_OK;
+}
+
+int blmi_adjoint_set_particles(void *h, const double *dt, const double *dE) {
+    Adjoint *a = reinterpret_cast<Adjoint *>(h);
+    BLMI_REQUIRE(a && dt && dE, "adjoint: null argument");
+    return a->set_particles(dt, dE);
+}
+
+int blmi_adjoint_set_programs(void *h, const double *rf, const double *acc, const double *t_rev,
+                              double length_ratio, double alpha_order, const double *eta0,
+                              const double *eta1, const double *eta2, const double *alpha0,
+                              const double *alpha1, const double *alpha2, const double *beta,
+                              const double *energy, double charge) {
+    Adjoint *a = reinterpret_cast<Adjoint *>(h);
+    BLMI_REQUIRE(a, "adjoint: null handle");
+    return a->set_programs(rf, acc, t_rev, length_ratio, alpha_order, eta0, eta1, eta2, alpha0,
+                           alpha1, alpha2, beta, energy, charge);
+}
+
+int blmi_adjoint_set_slices(void *h, double cut_left, double cut_right, const double *bin_centers) {
+    Adjoint *a = reinterpret_cast<Adjoint *>(h);
+    BLMI_REQUIRE(a && bin_centers, "adjoint: null argument");
+    BLMI_REQUIRE(cut_right > cut_left, "adjoint: cut_right must exceed cut_left");
+    return a->set_slices(cut_left, cut_right, bin_centers);
+}
+
+int blmi_adjoint_set_wake(void *h, int64_t n_fft, const double *g, double scale) {
+    Adjoint *a = reinterpret_cast<Adjoint *>(h);
+    BLMI_REQUIRE(a && g, "adjoint: null argument");
+    return a->set_wake(n_fft, g, scale);
+}
+
+int blmi_adjoint_forward(void *h, int turns, double *profile_out, void *stream) {
+    Adjoint *a = reinterpret_cast<Adjoint *>(h);
+    BLMI_REQUIRE(a && profile_out, "adjoint: null argument");
+    return a->forward(turns, profile_out, S(stream));
+}
+
+int blmi_adjoint_backward(void *h, const double *dprofile, double *grad_rf, double *corr,
+                          void *stream) {
+    Adjoint *a = reinterpret_cast<Adjoint *>(h);
+    BLMI_REQUIRE(a && dprofile && grad_rf && corr, "adjoint: null argument");
+    return a->backward(dprofile, grad_rf, corr, S(stream));
 }
 
 }  // extern "C"

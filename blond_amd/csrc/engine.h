@@ -38,6 +38,8 @@ struct Engine {
     int groups = 1;                // copies of the counts (same-address atomics / groups)
     bool groups_auto = true;       // groups from the slice count (4 up to 1024 slices, else 1)
     unsigned *d_counts = nullptr;  // [2][groups][n_slices]
+    unsigned *d_partials = nullptr;  // u16-pair mode: [grid][words] per-workgroup histograms
+    size_t partials_bytes = 0;
     double *profile = nullptr;     // borrowed: f64 profile seen by the caller
     // induced voltage
     bool intensity = false;
@@ -53,7 +55,7 @@ struct Engine {
     double scale = 0;
     double *volt = nullptr;        // borrowed: induced voltage seen by the caller
     const double *bc = nullptr;    // borrowed: bin centres
-    double *d_table = nullptr;     // vk[n_slices], fac[n_slices], (bc0, inv_bw)
+    double *d_table = nullptr;     // (slope, intercept)[n_slices] as double2, then (bc0, inv_bw)
     // launch shape (tunable)
     int block_size = 1024;
     int blocks_per_cu = 1;
@@ -89,6 +91,8 @@ struct Engine {
     // histogram mode of the particle kernel (0 global atomics, 1 LDS u32,
     // 2 LDS u16 pairs) and its grid and LDS bytes for `block` threads
     int hist_layout(int block, int *grid, size_t *lds) const;
+    int ensure_partials();
+    int launch_reduce(hipStream_t s, int off);
     int launch_induced(hipStream_t s, int off, const KernelEvents &ev);
     InducedArgs induced_args(int off) const;
     int time_kernels(int turns, hipStream_t user, float *ms_induced, float *ms_turn);

@@ -6,15 +6,20 @@
 // and without them (EX_01_Acceleration.py:152-153: tracker, profile):
 //   RF kick(t) -> drift(t+1) -> profile(dt_{t+1}).
 //
-// Here a turn is at most two launches:
+// Here a turn is the induced-voltage launch(es) and one particle launch:
 //   induced(t): counts of dt_t -> caller's profile, V_t (caller's induced
 //               voltage) and the interpolation table (slope, intercept);
 //               direct Toeplitz convolution over all CUs for moderate slice
-//               counts, or the one-workgroup LDS FFT.
+//               counts, the one-workgroup LDS FFT, or the three-launch
+//               four-step FFT for long transforms.
 //   turn(t):    per particle: interp kick, RF kick, drift, bin of dt_{t+1}
 //               counted into a workgroup-private LDS histogram (wavefront-
 //               aggregated increments) and flushed to global counts with
-//               relaxed device-scope atomics.
+//               relaxed device-scope atomics -- or, for slice counts whose
+//               u32 histogram exceeds LDS, counted as u16 pairs and stored
+//               as the workgroup's row of partial histograms, summed by one
+//               more launch (plain loads and stores instead of millions of
+//               scattered atomics).
 // dt and dE are read once and written once per turn (32 B/particle in FP64).
 //
 // No kernel synchronises workgroups with each other (a device-scope
@@ -40,91 +45,9 @@
 #include "engine.h"
 #include "particles.h"
 #include "spectral.h"
+#include "track_device.h"
 
 namespace blmi {
-
-// ---- shared per-particle arithmetic (same order as the single kernels) ----
-template <typename T>
-__device__ __forceinline__ T e_rf_kick(T t, int n_rf, T charge, const T *__restrict__ V,
-                                       const T *__restrict__ w, const T *__restrict__ phi, T acc) {
-    T s = T(0);
-    for (int j = 0; j < n_rf; ++j) s = s + V[j] * Trig<T>::sin(w[j] * t + phi[j]);
-    return charge * s + acc;
-}
-
-template <typename T>
-__device__ __forceinline__ T e_drift(const DriftCoeffs<T> &c, T e, int solver) {
-    if (solver == 0) return c.coeff * e;
-    if (solver == 1) {
-        T den;
-        if (c.alpha_order == 0)
-            den = T(1) - c.e0 * e;
-        else if (c.alpha_order == 1)
-            den = T(1) - c.e0 * e - c.e1 * e * e;
-        else
-            den = T(1) - c.e0 * e - c.e1 * e * e - c.e2 * e * e * e;
-        return c.tl * (T(1) / den - T(1));
-    }
-    const T delta = sqrt(T(1) + c.ib2 * (e * e * c.ie2 + T(2) * e / c.energy)) - T(1);
-    return c.tl * ((T(1) + c.a0 * delta + c.a1 * (delta * delta) + c.a2 * (delta * delta * delta)) *
-                       (T(1) + e / c.energy) / (T(1) + delta) -
-                   T(1));
-}
-
-// Count `bin` (>= 0; < 0 = no count) into the LDS histogram.  Lanes of the
-// wavefront holding the same bin are merged first: up to max_peel times the
-// first pending lane's bin is broadcast, the lanes sharing it are counted
-// with one LDS atomic, then the remaining lanes add one each.  A narrow
-// bunch (few distinct bins per wavefront) thus costs a handful of atomics
-// instead of 64 conflicting ones; a wide one costs at most max_peel extra
-// scalar steps.  PACK16: bin b is the (b & 1) half of word b >> 1.
-template <bool PACK16>
-__device__ __forceinline__ void lds_count(unsigned *__restrict__ h, int bin, int max_peel) {
-    unsigned long long pending = __ballot(bin >= 0);
-    const int lane = (int)__lane_id();
-    for (int it = 0; it < max_peel && pending != 0ull; ++it) {
-        const int leader = __builtin_ctzll(pending);
-        const int lb = __builtin_amdgcn_readlane(bin, leader);
-        const unsigned long long same = __ballot(bin == lb) & pending;
-        if (lane == leader) {
-            const unsigned c = (unsigned)__popcll(same);
-            if (PACK16)
-                atomicAdd(&h[lb >> 1], c << (16 * (lb & 1)));
-            else
-                atomicAdd(&h[lb], c);
-        }
-        pending &= ~same;
-    }
-    if ((pending >> lane) & 1ull) {
-        if (PACK16)
-            atomicAdd(&h[bin >> 1], 1u << (16 * (bin & 1)));
-        else
-            atomicAdd(&h[bin], 1u);
-    }
-}
-
-// LDS words of a histogram of n_slices bins in mode 1 (u32) or 2 (u16 pairs)
-__device__ __forceinline__ int hist_words(int mode, int n_slices) {
-    return mode == 2 ? (n_slices + 1) / 2 : n_slices;
-}
-
-// Add the workgroup's LDS histogram into the global counts (non-zero bins).
-template <int LDSH>
-__device__ __forceinline__ void hist_flush(const unsigned *__restrict__ h,
-                                           unsigned *__restrict__ counts, int n_slices) {
-    const int words = hist_words(LDSH, n_slices);
-    for (int w = threadIdx.x; w < words; w += blockDim.x) {
-        const unsigned c = h[w];
-        if (!c) continue;
-        if (LDSH == 1) {
-            atomic_add_agent(&counts[w], c);
-        } else {
-            const unsigned lo = c & 0xffffu, hi = c >> 16;
-            if (lo) atomic_add_agent(&counts[2 * w], lo);
-            if (hi) atomic_add_agent(&counts[2 * w + 1], hi);
-        }
-    }
-}
 
 struct TurnArgs {
     int n_rf;
@@ -135,11 +58,10 @@ struct TurnArgs {
     const int *base;                   // device turn base
     int off;                           // this launch's turn offset
     int last_turn;                     // n_turns - 1
-    // interpolation kick (intensity effects): table vk/fac (mode 1) or the
-    // induced voltage and bin centres themselves (mode 2); tab_params =
-    // (bin_centers[0], inv_bw)
-    const double *vk, *fac, *tab_params;
-    const double *volt, *bc;
+    // interpolation kick (intensity effects): tab[k] = (slope, intercept)
+    // of bin k, tab_params = (bin_centers[0], inv_bw)
+    const double2 *tab;
+    const double *tab_params;
     int n_tab;
     // slicing: counts[parity][group][bin]; workgroup b adds into group b % groups
     unsigned *counts0, *counts1;
@@ -150,7 +72,62 @@ struct TurnArgs {
     // plain mode: the previous turn's counts are published by all workgroups
     double *profile;
     int *pending;                      // [2]: turn whose histogram counts[p] hold, or -1
+    // u16-pair histograms: workgroup b stores its LDS words at partials + b * part_words
+    unsigned *partials;
+    int part_words;
 };
+
+// Copy the workgroup's LDS words to its row of partial histograms (16-byte
+// stores; part_words is a multiple of 4).
+__device__ __forceinline__ void store_partial(const unsigned *__restrict__ h,
+                                              unsigned *__restrict__ row, int words) {
+    for (int w = 4 * (int)threadIdx.x; w < words; w += 4 * (int)blockDim.x)
+        *reinterpret_cast<uint4 *>(row + w) = make_uint4(h[w], h[w + 1], h[w + 2], h[w + 3]);
+}
+
+// counts[(t) & 1][bin] for t = *base + off: sum over `rows` partial
+// histograms of u16 pairs (plain stores: every bin is written).  Thread
+// (q, g) adds words 4q..4q+3 of rows g, g + 16, ... with 16-byte loads; the
+// 16 row groups are then summed in LDS.
+constexpr int kRedWords = 64;
+constexpr int kRedGroups = 16;
+__global__ void __launch_bounds__(256) partials_reduce_kernel(const unsigned *__restrict__ P,
+                                                              int rows, int W, const int *base,
+                                                              int off, unsigned *counts0,
+                                                              unsigned *counts1, int n_slices) {
+    __shared__ unsigned red[kRedGroups][2 * kRedWords];
+    const int t = *base + off;
+    unsigned *out = (t & 1) ? counts1 : counts0;
+    const int q = threadIdx.x % (kRedWords / 4), g = threadIdx.x / (kRedWords / 4);
+    const int w = blockIdx.x * kRedWords + 4 * q;
+    unsigned lo[4] = {0u, 0u, 0u, 0u}, hi[4] = {0u, 0u, 0u, 0u};
+    if (w < W) {
+#pragma unroll 8
+        for (int r = g; r < rows; r += kRedGroups) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(P + (size_t)r * W + w);
+            lo[0] += v.x & 0xffffu;
+            hi[0] += v.x >> 16;
+            lo[1] += v.y & 0xffffu;
+            hi[1] += v.y >> 16;
+            lo[2] += v.z & 0xffffu;
+            hi[2] += v.z >> 16;
+            lo[3] += v.w & 0xffffu;
+            hi[3] += v.w >> 16;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        red[g][2 * (4 * q + i)] = lo[i];
+        red[g][2 * (4 * q + i) + 1] = hi[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kRedWords) {
+        unsigned s = 0;
+        for (int k = 0; k < kRedGroups; ++k) s += red[k][threadIdx.x];
+        const int bin = 2 * blockIdx.x * kRedWords + (int)threadIdx.x;
+        if (bin < n_slices) out[bin] = s;
+    }
+}
 
 // profile[b] = sum over groups of c[g][b], and the counts cleared, for b in [b0, b1)
 __device__ __forceinline__ void publish_range(unsigned *__restrict__ c, double *__restrict__ profile,
@@ -207,14 +184,8 @@ __global__ void __launch_bounds__(1024) turn_kernel(double *__restrict__ dt, dou
             const double f = floor((x - bc0) * ibw);
             if (f >= 0.0 && f < (double)a.n_tab) {
                 const int k = (int)f;
-                if (INTERP == 1) {
-                    e = e + (x * a.vk[k] + a.fac[k]);
-                } else {
-                    const double v0 = a.volt[k], v1 = a.volt[k + 1];
-                    const double vk = a.charge * (v1 - v0) * ibw;
-                    const double fac = (a.charge * v0 - a.bc[k] * vk) + 0.0;
-                    e = e + (x * vk + fac);
-                }
+                const double2 tk = a.tab[k];
+                e = e + (x * tk.x + tk.y);
             }
         }
         e = e + e_rf_kick(x, a.n_rf, a.charge, V, w, phi, acc);
@@ -258,7 +229,10 @@ __global__ void __launch_bounds__(1024) turn_kernel(double *__restrict__ dt, dou
     }
     if (HIST && LDSH) {
         __syncthreads();
-        hist_flush<LDSH>(lds_hist, counts, a.n_slices);
+        if (LDSH == 2)
+            store_partial(lds_hist, a.partials + (size_t)blockIdx.x * a.part_words, a.part_words);
+        else
+            hist_flush<LDSH>(lds_hist, counts, a.n_slices);
     }
 }
 
@@ -276,13 +250,15 @@ __global__ void publish_kernel(const int *base, int off, unsigned *counts0, unsi
 // *base += k, after every launch of the chunk has read it (stream order).
 __global__ void advance_kernel(int *base, int k) { *base += k; }
 
-// Histogram of the current coordinates into counts[t & 1] (engine priming);
-// both buffers are cleared first by a fill kernel.
+// Histogram of the current coordinates (engine priming), in the particle
+// kernel's layout: counts[t & 1] (cleared first by a fill kernel) for modes
+// 0 and 1, the workgroups' partial rows for mode 2 (reduced afterwards).
 template <int LDSH>
 __global__ void __launch_bounds__(1024) prime_hist_kernel(const double *__restrict__ dt, int64_t n,
                                                           const int *base, unsigned *counts0,
                                                           unsigned *counts1, int n_slices,
-                                                          double cut_left, double inv_bw) {
+                                                          double cut_left, double inv_bw,
+                                                          unsigned *partials, int part_words) {
     extern __shared__ unsigned lds_hist[];
     const int t = *base;
     unsigned *counts = (t & 1) ? counts1 : counts0;
@@ -291,22 +267,31 @@ __global__ void __launch_bounds__(1024) prime_hist_kernel(const double *__restri
         for (int b = threadIdx.x; b < words; b += blockDim.x) lds_hist[b] = 0u;
         __syncthreads();
     }
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t n_pad = ((n + blockDim.x - 1) / blockDim.x) * blockDim.x;  // whole wavefronts
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += stride) {
-        int bin = -1;
-        if (i < n) {
-            const double f = floor((dt[i] - cut_left) * inv_bw);
-            if (f >= 0.0 && f < (double)n_slices) bin = (int)f;
-        }
+    auto bin_of = [&](double x) -> int {
+        const double f = floor((x - cut_left) * inv_bw);
+        return (f >= 0.0 && f < (double)n_slices) ? (int)f : -1;
+    };
+    auto count = [&](int bin) {
         if (LDSH)
             lds_count<LDSH == 2>(lds_hist, bin, 0);
         else if (bin >= 0)
             atomic_add_agent(&counts[bin], 1u);
+    };
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n2 = n / 2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) {
+        const double2 x = reinterpret_cast<const double2 *>(dt)[i];
+        count(bin_of(x.x));
+        count(bin_of(x.y));
     }
+    if ((n & 1) && blockIdx.x == gridDim.x - 1 && threadIdx.x < 64)
+        count(threadIdx.x == 0 ? bin_of(dt[n - 1]) : -1);
     if (LDSH) {
         __syncthreads();
-        hist_flush<LDSH>(lds_hist, counts, n_slices);
+        if (LDSH == 2)
+            store_partial(lds_hist, partials + (size_t)blockIdx.x * part_words, part_words);
+        else
+            hist_flush<LDSH>(lds_hist, counts, n_slices);
     }
 }
 
@@ -325,6 +310,9 @@ void Engine::release() {
     (void)hipFree(d_table);
     (void)hipFree(d_Z);
     (void)hipFree(d_gl);
+    (void)hipFree(d_partials);
+    d_partials = nullptr;
+    partials_bytes = 0;
     fft2_plan_destroy(&plan2);
     d_rf = d_acc = d_table = d_gl = nullptr;
     d_drift = nullptr;
@@ -527,7 +515,7 @@ int Engine::hist_layout(int block, int *grid_out, size_t *lds_out) const {
     if ((size_t)n_slices * sizeof(unsigned) <= (size_t)kLdsHistBytes) {
         mode = 1;
         lds = (size_t)n_slices * sizeof(unsigned);
-    } else if ((size_t)((n_slices + 1) / 2) * sizeof(unsigned) <= (size_t)kLdsHistBytes) {
+    } else if ((size_t)hist_words(2, n_slices) * sizeof(unsigned) <= (size_t)kLdsHistBytes) {
         // a thread takes ceil(n2 / (grid*block)) pairs: keep a workgroup's
         // particles (+1 for an odd tail) within the u16 counters
         const int64_t max_pairs = (kMaxParticlesPerGroup16 - 1) / (2 * (int64_t)block);
@@ -535,7 +523,7 @@ int Engine::hist_layout(int block, int *grid_out, size_t *lds_out) const {
             const int64_t g = (n2 + block * max_pairs - 1) / (block * max_pairs);
             if (g > grid) grid = (int)g;
             mode = 2;
-            lds = (size_t)((n_slices + 1) / 2) * sizeof(unsigned);
+            lds = (size_t)hist_words(2, n_slices) * sizeof(unsigned);
         }
     }
     *grid_out = grid;
@@ -552,53 +540,80 @@ static int lds_attribute(const void *fn, size_t lds) {
     return BLMI_OK;
 }
 
+// Partial-histogram rows for the u16-pair mode, sized for the current
+// launch shape (allocated outside any graph capture).
+int Engine::ensure_partials() {
+    if (n_slices <= 0 || !d_counts) return BLMI_OK;
+    int grid = 0;
+    size_t lds = 0;
+    if (hist_layout(block_size, &grid, &lds) != 2) return BLMI_OK;
+    const size_t need = sizeof(unsigned) * (size_t)grid * hist_words(2, n_slices);
+    if (need > partials_bytes) {
+        invalidate_graph();  // a captured graph holds the old rows
+        (void)hipFree(d_partials);
+        d_partials = nullptr;
+        partials_bytes = 0;
+        BLMI_HIP_TRY(hipMalloc(&d_partials, need));
+        partials_bytes = need;
+    }
+    return BLMI_OK;
+}
+
+int Engine::launch_reduce(hipStream_t s, int off) {
+    int grid = 0;
+    size_t lds = 0;
+    if (hist_layout(block_size, &grid, &lds) != 2) return BLMI_OK;
+    const int W = hist_words(2, n_slices);
+    hipLaunchKernelGGL(partials_reduce_kernel, dim3((W + kRedWords - 1) / kRedWords), dim3(256), 0,
+                       s, d_partials, grid, W, d_base, off, d_counts,
+                       d_counts + (size_t)groups * n_slices, n_slices);
+    return blmi_check_launch("partials_reduce");
+}
+
 int Engine::prime(hipStream_t s) {
     // counts of the current coordinates (the first pass of the map's
     // Profile.track()) into counts[t & 1]; the other buffer cleared
     if (!intensity || n_slices <= 0) return BLMI_OK;
-    int rc = launch_fill<unsigned>(d_counts, 2 * (int64_t)groups * n_slices, 0u, s);
+    int rc = ensure_partials();
+    if (rc) return rc;
+    rc = launch_fill<unsigned>(d_counts, 2 * (int64_t)groups * n_slices, 0u, s);
     if (rc) return rc;
     int grid = 0;
     size_t lds = 0;
-    const int mode = hist_layout(512, &grid, &lds);
-    // hist_layout counts pairs per thread; this kernel takes single particles
-    grid *= 2;
+    const int mode = hist_layout(block_size, &grid, &lds);
     const void *fn = mode == 2   ? (const void *)prime_hist_kernel<2>
                      : mode == 1 ? (const void *)prime_hist_kernel<1>
                                  : (const void *)prime_hist_kernel<0>;
     rc = lds_attribute(fn, lds);
     if (rc) return rc;
     unsigned *c1 = d_counts + (size_t)groups * n_slices;
+    const int W = hist_words(2, n_slices);
     if (mode == 2)
-        hipLaunchKernelGGL(prime_hist_kernel<2>, dim3(grid), dim3(256), lds, s, dt, n, d_base,
-                           d_counts, c1, n_slices, cut_left, inv_bw);
+        hipLaunchKernelGGL(prime_hist_kernel<2>, dim3(grid), dim3(block_size), lds, s, dt, n,
+                           d_base, d_counts, c1, n_slices, cut_left, inv_bw, d_partials, W);
     else if (mode == 1)
-        hipLaunchKernelGGL(prime_hist_kernel<1>, dim3(grid), dim3(256), lds, s, dt, n, d_base,
-                           d_counts, c1, n_slices, cut_left, inv_bw);
+        hipLaunchKernelGGL(prime_hist_kernel<1>, dim3(grid), dim3(block_size), lds, s, dt, n,
+                           d_base, d_counts, c1, n_slices, cut_left, inv_bw, d_partials, W);
     else
-        hipLaunchKernelGGL(prime_hist_kernel<0>, dim3(grid), dim3(256), 0, s, dt, n, d_base,
-                           d_counts, c1, n_slices, cut_left, inv_bw);
-    return blmi_check_launch("engine_prime");
+        hipLaunchKernelGGL(prime_hist_kernel<0>, dim3(grid), dim3(block_size), 0, s, dt, n,
+                           d_base, d_counts, c1, n_slices, cut_left, inv_bw, d_partials, W);
+    rc = blmi_check_launch("engine_prime");
+    if (rc) return rc;
+    return mode == 2 ? launch_reduce(s, 0) : BLMI_OK;
 }
 
 using TurnFn = void (*)(double *, double *, int64_t, TurnArgs);
 
 template <int S>
 static TurnFn pick_turn(int interp, bool hist, int ldsh) {
-    if (!hist)
-        return interp == 0 ? turn_kernel<S, 0, false, 0, false>
-               : interp == 1 ? turn_kernel<S, 1, false, 0, false>
-                             : turn_kernel<S, 2, false, 0, false>;
+    if (!hist) return interp ? turn_kernel<S, 1, false, 0, false> : turn_kernel<S, 0, false, 0, false>;
     switch (interp * 3 + ldsh) {
         case 0: return turn_kernel<S, 0, true, 0, true>;
         case 1: return turn_kernel<S, 0, true, 1, true>;
         case 2: return turn_kernel<S, 0, true, 2, true>;
         case 3: return turn_kernel<S, 1, true, 0, false>;
         case 4: return turn_kernel<S, 1, true, 1, false>;
-        case 5: return turn_kernel<S, 1, true, 2, false>;
-        case 6: return turn_kernel<S, 2, true, 0, false>;
-        case 7: return turn_kernel<S, 2, true, 1, false>;
-        default: return turn_kernel<S, 2, true, 2, false>;
+        default: return turn_kernel<S, 1, true, 2, false>;
     }
 }
 
@@ -619,8 +634,7 @@ InducedArgs Engine::induced_args(int off) const {
     ia.bc = bc;
     ia.charge = charge;
     // the four-step path leaves the slope/intercept to the particle kernel
-    ia.vk = method == 2 ? nullptr : d_table;
-    ia.fac = method == 2 ? nullptr : d_table + n_slices;
+    ia.tab = reinterpret_cast<double2 *>(d_table);
     ia.tab_params = d_table + 2 * n_slices;
     ia.gl = d_gl;
     return ia;
@@ -644,11 +658,8 @@ int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const Kernel
     a.base = d_base;
     a.off = off;
     a.last_turn = n_turns - 1;
-    a.vk = d_table;
-    a.fac = d_table ? d_table + n_slices : nullptr;
+    a.tab = reinterpret_cast<const double2 *>(d_table);
     a.tab_params = d_table ? d_table + 2 * n_slices : nullptr;
-    a.volt = volt;
-    a.bc = bc;
     a.n_tab = n_slices - 1;
     a.counts0 = d_counts;
     a.counts1 = d_counts ? d_counts + (size_t)groups * n_slices : nullptr;
@@ -659,6 +670,8 @@ int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const Kernel
     a.max_peel = max_peel;
     a.profile = profile;
     a.pending = d_base + 1;
+    a.partials = d_partials;
+    a.part_words = hist_words(2, n_slices);
     const int block = block_size;
     int grid = 0;
     size_t lds = 0;
@@ -667,7 +680,7 @@ int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const Kernel
         ldsh = 0;
         lds = 0;
     }
-    const int imode = interp ? (method == 2 ? 2 : 1) : 0;
+    const int imode = interp ? 1 : 0;
     TurnFn fn = solver == 0   ? pick_turn<0>(imode, hist, ldsh)
                 : solver == 1 ? pick_turn<1>(imode, hist, ldsh)
                               : pick_turn<2>(imode, hist, ldsh);
@@ -687,6 +700,7 @@ int Engine::launch_turns(hipStream_t s, int count, bool advance) {
             if (rc) return rc;
         }
         int rc = launch_turn_kernel(s, k, intensity, none);
+        if (!rc) rc = launch_reduce(s, k + 1);
         if (rc) return rc;
     }
     if (advance) return finish_chunk(s, count);
@@ -713,6 +727,8 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
     if (host_turn + turns > n_turns)
         return blmi_set_error(BLMI_ERR_ARG, "time_kernels: %d turns from turn %d passes n_turns=%d",
                               turns, host_turn, n_turns);
+    int prc = ensure_partials();
+    if (prc) return prc;
     BLMI_HIP_TRY(hipEventRecord(ev_in, user));
     BLMI_HIP_TRY(hipStreamWaitEvent(own_stream, ev_in, 0));
     std::vector<hipEvent_t> ev(4 * (size_t)turns, nullptr);
@@ -722,6 +738,7 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
         KernelEvents ei{ev[4 * k], ev[4 * k + 1]}, et{ev[4 * k + 2], ev[4 * k + 3]};
         if (intensity) rc = launch_induced(own_stream, k, ei);
         if (rc == BLMI_OK) rc = launch_turn_kernel(own_stream, k, intensity, et);
+        if (rc == BLMI_OK) rc = launch_reduce(own_stream, k + 1);
     }
     if (rc == BLMI_OK) rc = finish_chunk(own_stream, turns);
     if (rc == BLMI_OK) {
@@ -752,6 +769,8 @@ int Engine::track(int turns, hipStream_t user, bool use_graph) {
     if (host_turn + turns > n_turns)
         return blmi_set_error(BLMI_ERR_ARG, "tracking %d turns from turn %d passes n_turns=%d",
                               turns, host_turn, n_turns);
+    int prc = ensure_partials();
+    if (prc) return prc;
     // order the engine stream after the caller's stream
     BLMI_HIP_TRY(hipEventRecord(ev_in, user));
     BLMI_HIP_TRY(hipStreamWaitEvent(own_stream, ev_in, 0));

@@ -57,12 +57,21 @@ struct InducedArgs {
     int n_out;        // values of V written (n_slices <= n_out <= n_fft)
     const double *bc; // bin centres (table only)
     double charge;
-    double *vk, *fac, *tab_params;  // table outputs (optional)
+    // table outputs (optional): tab[i] = (slope, intercept) of bin i, and
+    // tab_params = (bin_centers[0], inv_bw)
+    double2 *tab;
+    double *tab_params;
     const double *gl; // direct method: g[(k - (ns-1)) mod n_fft], g = -scale * irfft(Z)
 };
 // Single workgroup: LDS Stockham rfft, impedance product, irfft.
 int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream_t s,
                                const KernelEvents &ev = KernelEvents());
+// One workgroup per bunch (d_args: device array of n_bunches argument sets,
+// each with one copy of counts in counts0, read and cleared in place; all
+// with this plan's n_fft and n_slices).
+int launch_induced_lds_batch(const InducedArgs *d_args, int n_bunches, int n_slices,
+                             const FftPlan &p, hipStream_t s,
+                             const KernelEvents &ev = KernelEvents());
 // All CUs: V[i] = sum_j p[j] g[(i - j) mod n_fft], the same linear map as the
 // FFT route (Z[0] = 0 and the c2r Nyquist convention included in g).  Only
 // n_out == n_slices.
@@ -90,8 +99,8 @@ struct Fft2Plan {
 };
 int fft2_plan_create(int64_t n_real, Fft2Plan *plan);
 void fft2_plan_destroy(Fft2Plan *plan);
-// Same contract as launch_induced_voltage_lds (InducedArgs; the table
-// outputs vk/fac are not produced here).  `ev`, when given, spans the three
+// Same contract as launch_induced_voltage_lds (InducedArgs, including the
+// interpolation table).  `ev`, when given, spans the three
 // launches (start at pass A, stop at pass C).
 int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t s,
                             const KernelEvents &ev = KernelEvents());

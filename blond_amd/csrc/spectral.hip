@@ -263,17 +263,14 @@ __global__ void irfft_lds_kernel(const double2 *__restrict__ in, int64_t m_in, d
 // ---------------------------------------------------------------------------
 // one-turn induced voltage on one workgroup
 // ---------------------------------------------------------------------------
-__global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
-                                   const double2 *__restrict__ tw, const double2 *__restrict__ tw2) {
+// The profile is read from a.src, or as the sum of a.groups copies of u32
+// counts at cin (then cleared at cclr, which may equal cin: each element is
+// read before it is cleared, by the same thread).
+__device__ void induced_lds_body(const InducedArgs &a, const unsigned *cin, unsigned *cclr, int M,
+                                 const StageList &st, const double2 *__restrict__ tw,
+                                 const double2 *__restrict__ tw2) {
     extern __shared__ double2 buf[];
     const int ns = a.n_slices;
-    const unsigned *cin = nullptr;
-    unsigned *cclr = nullptr;
-    if (a.counts0) {
-        const int t = *a.turn_base + a.turn_off;
-        cin = (t & 1) ? a.counts1 : a.counts0;
-        cclr = (t & 1) ? a.counts0 : a.counts1;
-    }
     // load the profile as z[m] = p[2m] + i p[2m+1], zero padded to n = 2M
     auto prof = [&](int j) -> double {
         if (!cin) return a.src[j];
@@ -345,7 +342,7 @@ __global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
         if (2 * m < a.n_out) a.volt[2 * m] = v0;
         if (2 * m + 1 < a.n_out) a.volt[2 * m + 1] = v1;
     }
-    if (a.vk) {
+    if (a.tab) {
         __syncthreads();
         // linear_interp_kick.cpp:33-48 on V (acc = 0 for the induced kick)
         const double inv_bw = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
@@ -355,10 +352,30 @@ __global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
         }
         for (int i = threadIdx.x; i < ns - 1; i += blockDim.x) {
             const double k = a.charge * (vl[i + 1] - vl[i]) * inv_bw;
-            a.vk[i] = k;
-            a.fac[i] = (a.charge * vl[i] - a.bc[i] * k) + 0.0;
+            a.tab[i] = make_double2(k, (a.charge * vl[i] - a.bc[i] * k) + 0.0);
         }
     }
+}
+
+__global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
+                                   const double2 *__restrict__ tw, const double2 *__restrict__ tw2) {
+    const unsigned *cin = nullptr;
+    unsigned *cclr = nullptr;
+    if (a.counts0) {
+        const int t = *a.turn_base + a.turn_off;
+        cin = (t & 1) ? a.counts1 : a.counts0;
+        cclr = (t & 1) ? a.counts0 : a.counts1;
+    }
+    induced_lds_body(a, cin, cclr, M, st, tw, tw2);
+}
+
+// One workgroup per bunch: bunch b's arguments are args[b]; its counts (one
+// copy) are read and cleared in place.
+__global__ void induced_lds_batch_kernel(const InducedArgs *__restrict__ args, int M, StageList st,
+                                         const double2 *__restrict__ tw,
+                                         const double2 *__restrict__ tw2) {
+    const InducedArgs a = args[blockIdx.x];
+    induced_lds_body(a, a.counts0, a.counts0, M, st, tw, tw2);
 }
 
 // ---------------------------------------------------------------------------
@@ -465,7 +482,7 @@ __global__ void __launch_bounds__(kConvThreads) induced_direct_kernel(InducedArg
                 for (int g = 0; g < a.groups; ++g) cclr[(size_t)g * ns + i] = 0u;
         }
     }
-    if (a.vk) {
+    if (a.tab) {
         // linear_interp_kick.cpp:33-48 on V (acc = 0 for the induced kick)
         const double inv_bw = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -477,8 +494,7 @@ __global__ void __launch_bounds__(kConvThreads) induced_direct_kernel(InducedArg
             if (i < ns - 1) {
                 const double vi = vout[threadIdx.x], vi1 = vout[threadIdx.x + 1];
                 const double k = a.charge * (vi1 - vi) * inv_bw;
-                a.vk[i] = k;
-                a.fac[i] = (a.charge * vi - a.bc[i] * k) + 0.0;
+                a.tab[i] = make_double2(k, (a.charge * vi - a.bc[i] * k) + 0.0);
             }
         }
     }
@@ -668,6 +684,22 @@ int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream
     return blmi_check_launch("induced_voltage");
 }
 
+int launch_induced_lds_batch(const InducedArgs *d_args, int n_bunches, int n_slices,
+                             const FftPlan &p, hipStream_t s, const KernelEvents &ev) {
+    if (p.M / 2 + 1 > 8 * p.threads || p.M > 16 * p.threads)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "FFT plan M=%d too large for %d threads", p.M,
+                              p.threads);
+    if (n_slices > p.n_real)
+        return blmi_set_error(BLMI_ERR_ARG, "n_slices %d exceeds n_fft %d", n_slices, p.n_real);
+    if (n_bunches <= 0) return BLMI_OK;
+    const size_t lds = sizeof(double2) * p.M;
+    int rc = set_lds_limit((const void *)induced_lds_batch_kernel, lds);
+    if (rc) return rc;
+    launch(induced_lds_batch_kernel, dim3(n_bunches), dim3(p.threads), lds, s, ev, d_args, p.M,
+           stages_of(p), p.tw, p.tw2);
+    return blmi_check_launch("induced_voltage_batch");
+}
+
 int launch_fast_resonator(double *re, double *im, const double *freq, const double *R,
                           const double *Q, const double *fr, int n_res, int n_f, hipStream_t s) {
     if (n_f <= 1) return BLMI_OK;
@@ -710,12 +742,13 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
                                                              double2 *__restrict__ T) {
     extern __shared__ double2 buf[];
     const int ld = M1 + 1;
-    const int M = M1 * M2;
+    double2 *twl = buf + kColBatch * ld;  // exp(-2 pi i j / M1), j < M1
     const int ns = a.n_slices;
     const int c0 = blockIdx.x * kColBatch;
     const unsigned *cin;
     unsigned *cclr;
     count_buffers(a, cin, cclr);
+    for (int j = threadIdx.x; j < M1; j += blockDim.x) twl[j] = twM[(size_t)j * M2];
     for (int j = threadIdx.x; j < kColBatch * M1; j += blockDim.x) {
         const int c = j % kColBatch, m1 = j / kColBatch;
         const int m = M2 * m1 + c0 + c;
@@ -738,7 +771,7 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
         buf[c * ld + m1] = make_double2(p[0], p[1]);
     }
     __syncthreads();
-    fft_lds_batch<false>(buf, M1, kColBatch, ld, st1, twM, M);
+    fft_lds_batch<false>(buf, M1, kColBatch, ld, st1, twl, M1);
     for (int j = threadIdx.x; j < kColBatch * M1; j += blockDim.x) {
         const int c = j % kColBatch, k1 = j / kColBatch;
         const int m2 = c0 + c;
@@ -750,20 +783,39 @@ __global__ void fft2_b_kernel(InducedArgs a, int M1, int M2, StageList st2,
                               const double2 *__restrict__ twM, const double2 *__restrict__ twN,
                               const double2 *__restrict__ T, double2 *__restrict__ S) {
     extern __shared__ double2 buf[];
+    double2 *twl = buf + 2 * M2;  // exp(-2 pi i j / M2), j < M2
     const int M = M1 * M2;
     const int q = blockIdx.x;  // rows q and M1 - q
     const int r0 = q, r1 = (M1 - q) % M1;
     const int nrow = r1 == r0 ? 1 : 2;
     const int ne = nrow * M2;
+    constexpr int EMAX = 8;  // host guarantees ne <= EMAX * blockDim.x
+    // everything the split/merge needs is requested up front, so its latency
+    // overlaps the row loads and the forward transforms
+    double2 zk[EMAX], zmk[EMAX], wk[EMAX], wmk[EMAX], wo[EMAX];
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+        const int j = threadIdx.x + e * blockDim.x;
+        if (j < ne) {
+            const int row = j >= M2, k2 = j - row * M2;
+            const int r = row ? r1 : r0;
+            const int k = r + M1 * k2;
+            zk[e] = a.Z[k];
+            zmk[e] = a.Z[M - k];
+            wk[e] = twN[k];
+            wmk[e] = twN[M - k];
+            wo[e] = twM[(size_t)r * k2];  // output twiddle of element (row, m2 = k2)
+        }
+    }
+    for (int j = threadIdx.x; j < M2; j += blockDim.x) twl[j] = twM[(size_t)j * M1];
     for (int j = threadIdx.x; j < ne; j += blockDim.x) {
         const int row = j >= M2, k2 = j - row * M2;
         buf[j] = T[(size_t)(row ? r1 : r0) * M2 + k2];
     }
     __syncthreads();
-    fft_lds_batch<false>(buf, M2, nrow, M2, st2, twM, M);
+    fft_lds_batch<false>(buf, M2, nrow, M2, st2, twl, M2);
     // buf[row][k2] = Zc[r + M1 k2]; the mirror bin M - k sits at
     // row 0, (M2 - k2) % M2 for r = 0 and at row of M1 - r, M2 - 1 - k2 otherwise
-    constexpr int EMAX = 8;  // host guarantees ne <= EMAX * blockDim.x
     double2 u[EMAX];
 #pragma unroll
     for (int e = 0; e < EMAX; ++e) {
@@ -780,14 +832,14 @@ __global__ void fft2_b_kernel(InducedArgs a, int M1, int M2, StageList st2,
                 pj = (pr == r0 ? 0 : M2) + (M2 - 1 - k2);
             }
             double2 Xk, Xmk;
-            real_split(buf[j], buf[pj], twN[k], twN[M - k], Xk, Xmk);
+            real_split(buf[j], buf[pj], wk[e], wmk[e], Xk, Xmk);
             // impedance.py:479-484: total_impedance * beam_spectrum
-            double2 Yk = cmul(a.Z[k], Xk), Ymk = cmul(a.Z[M - k], Xmk);
+            double2 Yk = cmul(zk[e], Xk), Ymk = cmul(zmk[e], Xmk);
             if (k == 0) {  // c2r ignores the imaginary parts of DC and Nyquist
                 Yk.y = 0.0;
                 Ymk.y = 0.0;
             }
-            u[e] = real_merge(Yk, Ymk, twN[k]);
+            u[e] = real_merge(Yk, Ymk, wk[e]);
         }
     }
     __syncthreads();
@@ -797,42 +849,70 @@ __global__ void fft2_b_kernel(InducedArgs a, int M1, int M2, StageList st2,
         if (j < ne) buf[j] = u[e];
     }
     __syncthreads();
-    fft_lds_batch<true>(buf, M2, nrow, M2, st2, twM, M);
-    for (int j = threadIdx.x; j < ne; j += blockDim.x) {
-        const int row = j >= M2, m2 = j - row * M2;
-        const int r = row ? r1 : r0;
-        S[(size_t)r * M2 + m2] = cmul(buf[j], cconj(twM[(size_t)r * m2]));
+    fft_lds_batch<true>(buf, M2, nrow, M2, st2, twl, M2);
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+        const int j = threadIdx.x + e * blockDim.x;
+        if (j < ne) {
+            const int row = j >= M2, m2 = j - row * M2;
+            const int r = row ? r1 : r0;
+            S[(size_t)r * M2 + m2] = cmul(buf[j], cconj(wo[e]));
+        }
     }
 }
 
+// Pass C also transforms the column after its batch (column 0, one row down,
+// for the last batch), so every V[2m+2] a table entry needs is at hand.
 __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int M1, int M2,
                                                              StageList st1,
                                                              const double2 *__restrict__ twM,
                                                              const double2 *__restrict__ S) {
     extern __shared__ double2 buf[];
+    constexpr int NC = kColBatch + 1;
     const int ld = M1 + 1;
+    double2 *twl = buf + NC * ld;
     const int M = M1 * M2;
     const int c0 = blockIdx.x * kColBatch;
-    for (int j = threadIdx.x; j < kColBatch * M1; j += blockDim.x) {
-        const int c = j % kColBatch, k1 = j / kColBatch;
-        buf[c * ld + k1] = S[(size_t)k1 * M2 + c0 + c];
+    const bool last = c0 + kColBatch == M2;
+    for (int j = threadIdx.x; j < M1; j += blockDim.x) twl[j] = twM[(size_t)j * M2];
+    for (int j = threadIdx.x; j < NC * M1; j += blockDim.x) {
+        const int c = j % NC, k1 = j / NC;
+        const int col = (c0 + c) % M2;
+        buf[c * ld + k1] = S[(size_t)k1 * M2 + col];
     }
     __syncthreads();
-    fft_lds_batch<true>(buf, M1, kColBatch, ld, st1, twM, M);
+    fft_lds_batch<true>(buf, M1, NC, ld, st1, twl, M1);
     // V = -(charge*e*ratio * irfft(...))[:n_out]   (impedance.py:475-489)
     const double fct = 1.0 / (double)M;
+    const int ns = a.n_slices;
+    double inv_bw = 0.0;
+    if (a.tab) {
+        // linear_interp_kick.cpp:33-48 on V (acc = 0 for the induced kick)
+        inv_bw = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.tab_params[0] = a.bc[0];
+            a.tab_params[1] = inv_bw;
+        }
+    }
     for (int j = threadIdx.x; j < kColBatch * M1; j += blockDim.x) {
         const int c = j % kColBatch, m1 = j / kColBatch;
         const int m = M2 * m1 + c0 + c;
-        const double2 v = buf[c * ld + m1];
-        if (2 * m < a.n_out) a.volt[2 * m] = -(a.scale * (v.x * fct));
-        if (2 * m + 1 < a.n_out) a.volt[2 * m + 1] = -(a.scale * (v.y * fct));
-    }
-    if (a.tab_params && blockIdx.x == 0 && threadIdx.x == 0) {
-        // linear_interp_kick.cpp:33-35: the kick reads V and the bin centres
-        const int ns = a.n_slices;
-        a.tab_params[0] = a.bc[0];
-        a.tab_params[1] = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
+        const double2 u = buf[c * ld + m1];
+        const double v0 = -(a.scale * (u.x * fct)), v1 = -(a.scale * (u.y * fct));
+        if (2 * m < a.n_out) a.volt[2 * m] = v0;
+        if (2 * m + 1 < a.n_out) a.volt[2 * m + 1] = v1;
+        if (a.tab && 2 * m < ns - 1) {
+            const double k0 = a.charge * (v1 - v0) * inv_bw;
+            a.tab[2 * m] = make_double2(k0, (a.charge * v0 - a.bc[2 * m] * k0) + 0.0);
+            if (2 * m + 1 < ns - 1) {
+                // V[2m+2]: next column, same row (column 0, next row after the last)
+                const double2 un = c + 1 < kColBatch ? buf[(c + 1) * ld + m1]
+                                   : buf[kColBatch * ld + (last ? m1 + 1 : m1)];
+                const double v2 = -(a.scale * (un.x * fct));
+                const double k1 = a.charge * (v2 - v1) * inv_bw;
+                a.tab[2 * m + 1] = make_double2(k1, (a.charge * v1 - a.bc[2 * m + 1] * k1) + 0.0);
+            }
+        }
     }
 }
 
@@ -865,7 +945,7 @@ int fft2_plan_create(int64_t n_real, Fft2Plan *plan) {
     for (int d = 2048; d >= kColBatch; d -= kColBatch) {
         if (M % d) continue;
         const int m1 = M / d;
-        if (m1 < 2 || kColBatch * m1 > 15 * kColThreads) continue;
+        if (m1 < 2 || (kColBatch + 1) * m1 > 15 * kColThreads) continue;
         int r[kMaxStages], n1, n2;
         if (!factor_list(m1, r, n1) || !factor_list(d, r, n2)) continue;
         M1 = m1;
@@ -883,9 +963,10 @@ int fft2_plan_create(int64_t n_real, Fft2Plan *plan) {
     p.n_real = 2 * M;
     factor_list(M1, p.r1, p.n1);
     factor_list(M2, p.r2, p.n2);
-    int tb = (2 * M2 + 7) / 8;
+    // pass B: at most 4 elements a thread (registers hold their operands)
+    int tb = (2 * M2 + 3) / 4;
     tb = ((tb + 63) / 64) * 64;
-    p.threads_b = tb < 256 ? 256 : tb;
+    p.threads_b = tb < 256 ? 256 : tb > 1024 ? 1024 : tb;
     auto cis = [](long double num, long double den) {
         const long double two_pi = 6.283185307179586476925286766559L;
         long double x = two_pi * num / den;
@@ -926,20 +1007,21 @@ int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t
                               a.n_out, p.n_real);
     if (a.counts0 && a.groups > kMaxCountGroups)
         return blmi_set_error(BLMI_ERR_ARG, "too many count groups");
-    const size_t lds_ac = sizeof(double2) * kColBatch * (size_t)(p.M1 + 1);
-    const size_t lds_b = sizeof(double2) * 2 * (size_t)p.M2;
-    int rc = set_lds_limit((const void *)fft2_a_kernel, lds_ac);
-    if (!rc) rc = set_lds_limit((const void *)fft2_c_kernel, lds_ac);
+    const size_t lds_a = sizeof(double2) * (kColBatch * (size_t)(p.M1 + 1) + p.M1);
+    const size_t lds_c = sizeof(double2) * ((kColBatch + 1) * (size_t)(p.M1 + 1) + p.M1);
+    const size_t lds_b = sizeof(double2) * 3 * (size_t)p.M2;
+    int rc = set_lds_limit((const void *)fft2_a_kernel, lds_a);
+    if (!rc) rc = set_lds_limit((const void *)fft2_c_kernel, lds_c);
     if (!rc) rc = set_lds_limit((const void *)fft2_b_kernel, lds_b);
     if (rc) return rc;
     const StageList st1 = stage_list(p.r1, p.n1), st2 = stage_list(p.r2, p.n2);
     const int cols = p.M2 / kColBatch;
     const KernelEvents first{ev.start, nullptr}, last{nullptr, ev.stop};
-    launch(fft2_a_kernel, dim3(cols), dim3(kColThreads), lds_ac, s, first, a, p.M1, p.M2, st1,
+    launch(fft2_a_kernel, dim3(cols), dim3(kColThreads), lds_a, s, first, a, p.M1, p.M2, st1,
            p.twM, p.T);
     hipLaunchKernelGGL(fft2_b_kernel, dim3(p.M1 / 2 + 1), dim3(p.threads_b), lds_b, s, a, p.M1,
                        p.M2, st2, p.twM, p.twN, p.T, p.S);
-    launch(fft2_c_kernel, dim3(cols), dim3(kColThreads), lds_ac, s, last, a, p.M1, p.M2, st1,
+    launch(fft2_c_kernel, dim3(cols), dim3(kColThreads), lds_c, s, last, a, p.M1, p.M2, st1,
            p.twM, p.S);
     return blmi_check_launch("induced_fourstep");
 }

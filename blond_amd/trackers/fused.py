@@ -37,6 +37,56 @@ def _dptr(a: np.ndarray):
     return a.ctypes.data_as(ct.c_void_p)
 
 
+def phase_program(rp):
+    """phi_rf for all turns with the accumulated offsets of tracker.py:556-568
+    applied, from the current counter on (the recurrence depends only on the
+    programs, so it can be precomputed).  Returns (phi [n_rf, n_turns+1],
+    dphi [n_rf, n_turns+1], whether any offset is non-zero)."""
+    phi = np.array(rp.host_array("phi_rf"), dtype=np.float64, copy=True)
+    omega = rp.host_array("omega_rf")
+    dphi = np.array(rp.dphi_rf, dtype=np.float64, copy=True)
+    dphi_prog = np.zeros((rp.n_rf, rp.n_turns + 1))
+    dphi_prog[:, rp.counter[0]] = dphi
+    for t in range(rp.counter[0], rp.n_turns):
+        dphi = dphi + (2.0 * np.pi * rp.harmonic[:, t + 1]
+                       * (omega[:, t + 1] - rp.omega_rf_d[:, t + 1]) / rp.omega_rf_d[:, t + 1])
+        phi[:, t + 1] += dphi
+        dphi_prog[:, t + 1] = dphi
+    return phi, dphi_prog, bool(np.any(dphi_prog != 0))
+
+
+def rf_program_array(rp, phi) -> np.ndarray:
+    """[n_turns+1][3][n_rf]: voltage, omega_rf and phi_rf per turn."""
+    rf = np.empty((rp.n_turns + 1, 3, rp.n_rf), dtype=np.float64)
+    rf[:, 0, :] = np.asarray(rp.host_array("voltage"), dtype=np.float64).T
+    rf[:, 1, :] = np.asarray(rp.host_array("omega_rf"), dtype=np.float64).T
+    rf[:, 2, :] = phi.T
+    return np.ascontiguousarray(rf)
+
+
+def ring_program_arrays(rp) -> list:
+    """t_rev, eta_0..2, alpha_0..2, beta, energy per turn (contiguous f64)."""
+    return [np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in
+            (rp.t_rev, rp.eta_0, rp.eta_1, rp.eta_2, rp.alpha_0, rp.alpha_1, rp.alpha_2,
+             rp.beta, rp.energy)]
+
+
+def total_impedance(tiv) -> tuple[np.ndarray, int]:
+    """Sum of the InducedVoltageFreq sources' impedance tables and their n_fft."""
+    srcs = tiv.induced_voltage_list
+    if not srcs or not all(isinstance(s, InducedVoltageFreq) for s in srcs):
+        raise NotImplementedError("the fused turn supports InducedVoltageFreq sources")
+    n_fft = srcs[0].n_fft
+    if any(s.n_fft != n_fft for s in srcs):
+        raise NotImplementedError("all sources must share n_fft")
+    z = 0
+    for s in srcs:
+        zi = s.total_impedance
+        zi = zi.cpu().numpy() if hasattr(zi, "cpu") else np.asarray(zi)
+        z = z + zi
+    return np.ascontiguousarray(z, dtype=np.complex128), int(n_fft)
+
+
 class FusedTracker:
     def __init__(self, tracker, profile=None, total_induced_voltage=None, use_graph=True,
                  block=1024, blocks_per_cu=1, graph_chunk=50, groups=0, max_peel=0,
@@ -119,37 +169,17 @@ class FusedTracker:
 
     # ------------------------------------------------------------------
     def _phase_program(self):
-        """phi_rf for all turns with the accumulated offsets of
-        tracker.py:556-568 applied, from the current counter on (the
-        recurrence depends only on the programs, so it can be precomputed)."""
-        rp = self.rf
-        phi = np.array(rp.host_array("phi_rf"), dtype=np.float64, copy=True)
-        omega = rp.host_array("omega_rf")
-        dphi = np.array(rp.dphi_rf, dtype=np.float64, copy=True)
-        self._dphi_prog = np.zeros((rp.n_rf, rp.n_turns + 1))
-        self._dphi_prog[:, rp.counter[0]] = dphi
-        for t in range(rp.counter[0], rp.n_turns):
-            dphi = dphi + (2.0 * np.pi * rp.harmonic[:, t + 1]
-                           * (omega[:, t + 1] - rp.omega_rf_d[:, t + 1]) / rp.omega_rf_d[:, t + 1])
-            phi[:, t + 1] += dphi
-            self._dphi_prog[:, t + 1] = dphi
-        self._phase_moves = bool(np.any(self._dphi_prog != 0))
+        phi, self._dphi_prog, self._phase_moves = phase_program(self.rf)
         return phi
 
     def _upload_programs(self):
         rp = self.rf
         nt = rp.n_turns + 1
         self._phi_prog = self._phase_program()
-        rf = np.empty((nt, 3, rp.n_rf), dtype=np.float64)
-        rf[:, 0, :] = np.asarray(rp.host_array("voltage"), dtype=np.float64).T
-        rf[:, 1, :] = np.asarray(rp.host_array("omega_rf"), dtype=np.float64).T
-        rf[:, 2, :] = self._phi_prog.T
-        rf = np.ascontiguousarray(rf)
+        rf = rf_program_array(rp, self._phi_prog)
         acc = np.zeros(nt, dtype=np.float64)
         acc[: len(self.tracker.acceleration_kick)] = self.tracker.acceleration_kick
-        arrs = [np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in
-                (rp.t_rev, rp.eta_0, rp.eta_1, rp.eta_2, rp.alpha_0, rp.alpha_1, rp.alpha_2,
-                 rp.beta, rp.energy)]
+        arrs = ring_program_arrays(rp)
         self._keep = (rf, acc, arrs)
         _lib.call("blmi_engine_set_programs", self._h, _dptr(rf), _dptr(acc), _dptr(arrs[0]),
                   float(rp.length_ratio), float(rp.alpha_order), _dptr(arrs[1]), _dptr(arrs[2]),
@@ -158,18 +188,7 @@ class FusedTracker:
 
     def _setup_induced(self):
         torch = self._torch
-        srcs = self.tiv.induced_voltage_list
-        if not srcs or not all(isinstance(s, InducedVoltageFreq) for s in srcs):
-            raise NotImplementedError("the fused turn supports InducedVoltageFreq sources")
-        n_fft = srcs[0].n_fft
-        if any(s.n_fft != n_fft for s in srcs):
-            raise NotImplementedError("all sources must share n_fft")
-        z = 0
-        for s in srcs:
-            zi = s.total_impedance
-            zi = zi.cpu().numpy() if hasattr(zi, "cpu") else np.asarray(zi)
-            z = z + zi
-        self._Z = np.ascontiguousarray(z, dtype=np.complex128)
+        self._Z, n_fft = total_impedance(self.tiv)
         scale = self.beam.particle.charge * e * self.beam.ratio
         p = self.profile
         iv = self.tiv.induced_voltage

@@ -4,13 +4,21 @@
       Gaussian bunch 4 sigma = 1 ns, 100 slices over [0, t_rf]; turn =
       tracker, profile.
   c2  PSB-like ring (157.08 m, gamma_t = 4.4, 310 MeV/c), h=1, 8 kV, phi=pi,
-      Gaussian bunch as c1, intensity 1e13, 4096 slices over [0, t_rf],
+      Gaussian bunch 4 sigma = 400 ns (a quarter of the 1.67 us bucket),
+      intensity 1e13, 4096 slices over [0, t_rf],
       n_fft = 16384; broadband resonator (5 kOhm, 1 GHz, Q=1) + 20 narrowband
       resonators drawn from default_rng(42); turn = profile, induced
       voltage, tracker.
   c3  c1 ring with a second RF system at 4h, V2/V1 = 0.25 in
       bunch-lengthening phase (phi2 = 0: the two slopes cancel at the bunch
-      centre, phi = pi), 65536 slices, n_fft = 131072, resonators as c2.
+      centre, phi = pi), Gaussian bunch as c1, 65536 slices, n_fft = 131072,
+      resonators as c2.
+  c4  ensemble of independent bunches in the c2 ring: particle counts
+      round(10**U(4,6)), V = U(4,8) kV, phi = pi + U(-0.1, 0.1), each with
+      its own 1024-slice profile over [0, t_rf], intensity 1e12, and 5
+      narrowband resonators drawn as in c2 (n_fft = 1024).  Seeds: counts,
+      V, phi from default_rng(seed); bunch b from bigaussian(seed*1e5 + 2b),
+      its resonators from default_rng(seed*1e5 + 5e4 + b).
 
 Bunch draws follow bigaussian(seed=1): dt from default_rng(1), dE from
 default_rng(2).
@@ -30,6 +38,9 @@ from .impedances.impedance_sources import Resonators
 from .input_parameters.rf_parameters import RFStation
 from .input_parameters.ring import Ring
 from .trackers.tracker import RingAndRFTracker
+
+
+C2_SIGMA_DT = 100e-9
 
 
 def resonator_set(n_narrow: int = 20, seed: int = 42, broadband=True):
@@ -170,7 +181,8 @@ def build_workload(name: str, n_particles: int | None = None, n_turns: int | Non
             rf = RFStation(ring, [35640, 142560], [6e6, 1.5e6], [0.0, 0.0], n_rf=2)
             intensity = 1e11
         beam = Beam(ring, n_particles, intensity)
-        bigaussian(ring, rf, beam, sigma_dt=1.0e-9 / 4, seed=seed)
+        bigaussian(ring, rf, beam, sigma_dt=C2_SIGMA_DT if name == "c2" else 1.0e-9 / 4,
+                   seed=seed)
         prof = _profile(beam, 0.0, float(rf.t_rf[0, 0]), n_slices)
         R, fr, Q = resonator_set(n_resonators)
         res = Resonators(R, fr, Q)
@@ -192,3 +204,107 @@ def build_workload(name: str, n_particles: int | None = None, n_turns: int | Non
 def induced_scale(beam) -> float:
     """charge * e * ratio (impedance.py:475-485)."""
     return beam.particle.charge * e * beam.ratio
+
+
+@dataclass
+class Ensemble:
+    """Per-bunch reference-style objects of the c4 workload."""
+    ring: Ring
+    trackers: list
+    profiles: list
+    tivs: list
+    counts: np.ndarray
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_particles(self) -> int:
+        return int(self.counts.sum())
+
+    def tracker(self, **kw):
+        from .trackers.ensemble import EnsembleTracker
+
+        return EnsembleTracker(self.trackers, self.profiles, self.tivs, **kw)
+
+    def bunch(self, b: int) -> Workload:
+        """Bunch b as a single-bunch workload (the sequential form)."""
+        tr = self.trackers[b]
+        return Workload(f"c4[{b}]", self.ring, tr.rf_params, tr.beam, tr, self.profiles[b],
+                        self.tivs[b], True)
+
+
+def build_ensemble(n_bunches: int = 256, n_turns: int = 1000, n_slices: int = 1024,
+                   n_min: float = 1e4, n_max: float = 1e6, seed: int = 7,
+                   n_resonators: int = 5, intensity: float = 1e12) -> Ensemble:
+    """The c4 ensemble on the host (coordinates as numpy arrays); the
+    ensemble tracker moves them to HBM."""
+    rng = np.random.default_rng(seed)
+    counts = np.rint(10 ** rng.uniform(np.log10(n_min), np.log10(n_max), n_bunches)).astype(np.int64)
+    volts = rng.uniform(4e3, 8e3, n_bunches)
+    phis = np.pi + rng.uniform(-0.1, 0.1, n_bunches)
+    ring = Ring(157.08, 1 / 4.4 ** 2, 310e6, Proton(), n_turns)
+    trackers, profiles, tivs = [], [], []
+    for b in range(n_bunches):
+        rf = RFStation(ring, [1], [volts[b]], [phis[b]])
+        beam = Beam(ring, int(counts[b]), intensity)
+        bigaussian(ring, rf, beam, sigma_dt=C2_SIGMA_DT, seed=seed * 100000 + 2 * b)
+        prof = _profile(beam, 0.0, float(rf.t_rf[0, 0]), n_slices)
+        R, fr, Q = resonator_set(n_resonators, seed=seed * 100000 + 50000 + b, broadband=False)
+        ivf = InducedVoltageFreq(beam, prof, [Resonators(R, fr, Q)])
+        tivs.append(TotalInducedVoltage(beam, prof, [ivf]))
+        trackers.append(RingAndRFTracker(rf, beam))
+        profiles.append(prof)
+    return Ensemble(ring, trackers, profiles, tivs, counts,
+                    meta=dict(n_bunches=n_bunches, n_turns=n_turns, n_slices=n_slices,
+                              n_fft=int(tivs[0].induced_voltage_list[0].n_fft), seed=seed))
+
+
+@dataclass
+class AdjointCase:
+    """c5: the c1 ring with one broadband resonator, a continuous profile
+    on the bucket and a target profile made under hidden parameters."""
+    ring: Ring
+    rf: RFStation
+    beam: Beam
+    tracker: RingAndRFTracker
+    profile: Profile
+    n_fft: int
+    theta0: dict
+    theta_star: dict
+    state: dict
+
+
+def build_adjoint_case(n_particles: int = 1_000_000, n_turns: int = 500, n_slices: int = 256,
+                       seed: int = 11, intensity: float = 1e11) -> AdjointCase:
+    """Nominal parameters theta0 = (V 6 MV, phi 0, broadband R 1 MOhm,
+    f_r 2 GHz, Q 1); hidden theta* drawn from default_rng(seed) within a few
+    per cent of them (phi within +-0.02 rad).  Host-only (no GPU needed)."""
+    ring = Ring(26658.883, 1 / 55.759505 ** 2, 450e9, Proton(), n_turns)
+    rf = RFStation(ring, [35640], [6e6], [0.0])
+    beam = Beam(ring, n_particles, intensity)
+    bigaussian(ring, rf, beam, sigma_dt=1.0e-9 / 4, seed=seed)
+    prof = Profile(beam, CutOptions(cut_left=0.0, cut_right=float(rf.t_rf[0, 0]),
+                                    n_slices=n_slices),
+                   other_slices_options=OtherSlicesOptions(direct_slicing=False))
+    tr = RingAndRFTracker(rf, beam)
+    from .toolbox.next_regular import next_regular
+
+    n_fft = next_regular(n_slices)
+    theta0 = dict(V=np.array([6e6]), phi=np.array([0.0]), R=np.array([1e6]),
+                  fr=np.array([2e9]), Q=np.array([1.0]))
+    rng = np.random.default_rng(seed + 1000)
+    theta_star = dict(V=theta0["V"] * (1 + rng.uniform(-0.03, 0.03)),
+                      phi=np.array([rng.uniform(-0.02, 0.02)]),
+                      R=theta0["R"] * (1 + rng.uniform(-0.1, 0.1)),
+                      fr=theta0["fr"] * (1 + rng.uniform(-0.05, 0.05)),
+                      Q=theta0["Q"] * (1 + rng.uniform(-0.1, 0.1)))
+    rp = rf
+    coeff = (np.asarray(rp.t_rev) * rp.length_ratio * np.asarray(rp.eta_0[0])
+             / (np.asarray(rp.beta) ** 2 * np.asarray(rp.energy)))
+    state = dict(omega=np.array(rp.host_array("omega_rf"), dtype=np.float64),
+                 coeff=coeff, acc=np.zeros(n_turns + 1),
+                 charge=float(beam.particle.charge), cut_left=float(prof.cut_left),
+                 cut_right=float(prof.cut_right), n_slices=n_slices,
+                 bin_centers=np.asarray(prof.bin_centers, dtype=np.float64).copy(),
+                 n_fft=n_fft, bin_size=float(prof.bin_size),
+                 scale=float(beam.particle.charge * e * beam.ratio))
+    return AdjointCase(ring, rf, beam, tr, prof, n_fft, theta0, theta_star, state)

@@ -175,6 +175,86 @@ int blmi_engine_track(void *engine, int turns, int use_graph, void *stream);
 int blmi_engine_time_kernels(void *engine, int turns, float *ms_induced, float *ms_turn,
                              void *stream);
 
+/* ---- batched ensemble ------------------------------------------------------
+ * Many independent bunches (own particle count, RF program, cuts, profile,
+ * impedance and induced voltage; one ring) tracked together: per turn one
+ * launch computes every bunch's induced voltage (one workgroup each, LDS
+ * FFT) and one launch streams all particles, split evenly over the grid;
+ * chunks of turns replay as a captured graph.  This replaces running the
+ * reference's per-bunch map (EX_18_robinson_instability.py:226) once per
+ * bunch; each bunch's arithmetic is that of blmi_engine_* with method 0.
+ * Order: create, set_particles, set_programs, set_slices, set_induced
+ * (optional: without it the map is tracker then profile), then
+ * set_turn/prime/track.
+ *   offsets[b], counts[b]: bunch b occupies dt/dE[offsets[b] .. +counts[b]);
+ *     offsets even, bunches not overlapping (an odd bunch owns one pad slot).
+ *   rf_host: [bunch][n_turns+1][3][n_rf] (V, omega_rf, phi_rf); acc and the
+ *     ring programs are per turn as for the engine.
+ *   cut_left/cut_right: [bunch] (host); profiles, volt, bin_centers:
+ *     device [bunch][n_slices]; Z_host [bunch][n_fft/2+1] complex; scale
+ *     [bunch] = charge * e * ratio_b. */
+void *blmi_ensemble_create(int n_bunches, int n_rf, int n_turns, int solver);
+int blmi_ensemble_destroy(void *ensemble);
+int blmi_ensemble_set_particles(void *ensemble, double *dt, double *dE, const int64_t *offsets,
+                                const int64_t *counts);
+int blmi_ensemble_set_programs(void *ensemble, const double *rf_host, const double *acc_host,
+                               const double *t_rev_host, double length_ratio, double alpha_order,
+                               const double *eta0_host, const double *eta1_host,
+                               const double *eta2_host, const double *alpha0_host,
+                               const double *alpha1_host, const double *alpha2_host,
+                               const double *beta_host, const double *energy_host, double charge);
+int blmi_ensemble_set_slices(void *ensemble, int n_slices, const double *cut_left,
+                             const double *cut_right, double *profiles);
+int blmi_ensemble_set_induced(void *ensemble, int64_t n_fft, const double *Z_host,
+                              const double *scale, double *volt, const double *bin_centers);
+/* Launch shape: workgroup size (multiple of 64 up to 1024), workgroups per
+ * CU, turns per captured graph, wavefront-aggregation steps of the histogram. */
+int blmi_ensemble_set_launch(void *ensemble, int block, int wg_per_cu, int graph_chunk,
+                             int max_peel);
+int blmi_ensemble_get_launch(void *ensemble, int *block, int *wg_per_cu);
+int blmi_ensemble_set_turn(void *ensemble, int turn);
+int blmi_ensemble_prime(void *ensemble, void *stream);
+int blmi_ensemble_track(void *ensemble, int turns, int use_graph, void *stream);
+int blmi_ensemble_time_turns(void *ensemble, int turns, float *ms_induced, float *ms_turn,
+                             void *stream);
+/* Times `turns` eager turns for each launch shape (block 256/512/1024 x
+ * workgroups per CU up to 2048 threads), keeps the fastest and restores the
+ * coordinates, counts and turn it started from. */
+int blmi_ensemble_autotune(void *ensemble, int turns, int *block, int *wg_per_cu,
+                           float *ms_per_turn, void *stream);
+
+/* ---- differentiable tracking (adjoint) -------------------------------------
+ * The intensity map of the fused engine -- profile, induced voltage, linear
+ * interpolated kick (linear_interp_kick.cpp:21-71), RF kick (kick.cpp:16-35),
+ * "simple" drift (drift.cpp:34-37) -- with the profile deposited
+ * continuously (cloud-in-cell on bin centres; the reference's
+ * smooth_histogram, histogram.cpp:75-132, jumps at bin centres and has no
+ * derivative there) and V = -scale * (g circular-convolved with the
+ * profile), g = irfft(Z, n_fft) (impedance.py:457-489).  forward() tracks
+ * `turns` turns from the particles given to set_particles (copied), keeping
+ * (dt, dE) every `checkpoint_every` turns; backward() takes dL/d(final
+ * profile) and returns dL/dV_j, dL/dphi_j summed over turns
+ * (grad_rf[2j], grad_rf[2j+1], host) and corr[m] = sum_t sum_i
+ * dL/dV_t[i] p_t[(i - m) mod n_fft] (host), from which dL/dtheta of any
+ * impedance parameter is -scale * sum_m irfft(dZ/dtheta)[m] corr[m].
+ * n_rf 1..4, 2..1024 slices, n_fft <= 8192.  No reference counterpart: the
+ * reference has no adjoint. */
+void *blmi_adjoint_create(int64_t n, int n_rf, int n_turns, int n_slices, int checkpoint_every);
+int blmi_adjoint_destroy(void *adjoint);
+int blmi_adjoint_set_particles(void *adjoint, const double *dt, const double *dE);
+int blmi_adjoint_set_programs(void *adjoint, const double *rf_host, const double *acc_host,
+                              const double *t_rev_host, double length_ratio, double alpha_order,
+                              const double *eta0_host, const double *eta1_host,
+                              const double *eta2_host, const double *alpha0_host,
+                              const double *alpha1_host, const double *alpha2_host,
+                              const double *beta_host, const double *energy_host, double charge);
+int blmi_adjoint_set_slices(void *adjoint, double cut_left, double cut_right,
+                            const double *bin_centers_host);
+int blmi_adjoint_set_wake(void *adjoint, int64_t n_fft, const double *g_host, double scale);
+int blmi_adjoint_forward(void *adjoint, int turns, double *profile_out, void *stream);
+int blmi_adjoint_backward(void *adjoint, const double *dprofile, double *grad_rf_host,
+                          double *corr_host, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
