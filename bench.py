@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--repeats", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--timing-turns", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -59,7 +59,24 @@ def parse():
     ap.add_argument("--method", choices=["auto", "fft", "direct", "fourstep"], default="auto")
     ap.add_argument("--forms", default="large,batched,adjoint",
                     help="comma list of extra forms to measure ('' for none)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (the known-size kernel "
+                         "that calibrates FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib: int):
+    """Five device copies of `mib` MiB: each reads and writes mib MiB."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 8,
+                      "bytes_written_per_copy": n * 8}), flush=True)
 
 
 def _timed(fn, sync):
@@ -186,6 +203,9 @@ def cpu_baseline(snapshot: dict, target_s: float) -> dict:
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (--gpus {a.gpus} "
               "requested); the per-GPU figure is --gpus 1", file=sys.stderr)

@@ -56,6 +56,9 @@ SIGNATURES = {
     "blmi_engine_set_method": [_p, _i],
     "blmi_engine_get_method": [_p, ct.POINTER(_i)],
     "blmi_engine_set_hist": [_p, _i, _i],
+    "blmi_engine_set_particles_f32": [_p, _p, _p, _i64],
+    "blmi_engine_set_slices_f32": [_p, _i, _d, _d, _p],
+    "blmi_engine_set_induced_f32": [_p, _i64, _p, _d, _p, _p],
     "blmi_ensemble_create": [_i, _i, _i, _i],
     "blmi_ensemble_destroy": [_p],
     "blmi_ensemble_set_particles": [_p, _p, _p, _p, _p],
@@ -73,7 +76,7 @@ SIGNATURES = {
     "blmi_adjoint_destroy": [_p],
     "blmi_adjoint_set_particles": [_p, _p, _p],
     "blmi_adjoint_set_programs": [_p, _p, _p, _p, _d, _d, _p, _p, _p, _p, _p, _p, _p, _p, _d],
-    "blmi_adjoint_set_slices": [_p, _d, _d, _p],
+    "blmi_adjoint_set_slices": [_p, _d, _d],
     "blmi_adjoint_set_wake": [_p, _i64, _p, _d],
     "blmi_adjoint_forward": [_p, _i, _p, _p],
     "blmi_adjoint_backward": [_p, _p, _p, _p, _p],

@@ -1,4 +1,4 @@
-// Differentiable tracking: forward turns with a continuous (cloud-in-cell)
+// Differentiable tracking: forward turns with a smooth (quadratic B-spline)
 // profile deposit, turn checkpoints, and the hand-written reverse pass.
 #pragma once
 
@@ -35,9 +35,7 @@ struct Adjoint {
     // slice axis
     unsigned long long *d_dep = nullptr;  // [copies][n_slices] fixed-point deposit
     double *d_prof = nullptr;  // [n_turns + 1][n_slices] profiles p_t
-    double2 *d_tab = nullptr;  // [n_turns][n_slices] (slope, intercept) of V_t
-    double *d_bc = nullptr;    // bin centres [n_slices]
-    double bc0 = 0, inv_bw_bc = 0;
+    double *d_vt = nullptr;    // [n_turns + 1][n_slices] induced voltage V_t
     double *d_g = nullptr;     // wake g = irfft(Z) [n_fft]
     double *d_lv = nullptr;    // [copies][n_slices] adjoint of V_t (accumulated)
     double *d_lp = nullptr;    // [n_slices] adjoint of the profile
@@ -56,7 +54,7 @@ struct Adjoint {
                      const double *eta1, const double *eta2, const double *alpha0,
                      const double *alpha1, const double *alpha2, const double *beta,
                      const double *energy, double charge);
-    int set_slices(double cut_left, double cut_right, const double *bc_host);
+    int set_slices(double cut_left, double cut_right);
     int set_wake(int64_t n_fft, const double *g_host, double scale);
     int forward(int turns, double *profile_out, hipStream_t s);
     int backward(const double *dprofile, double *grad_rf_host, double *corr_host, hipStream_t s);

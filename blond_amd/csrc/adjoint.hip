@@ -3,24 +3,33 @@
 //
 // The map per turn t (the intensity map of the fused engine, with the
 // profile deposited continuously so the loss is differentiable):
-//   p_t     = D(dt_t)        cloud-in-cell deposit on bin centres: a
-//                            particle at fractional bin position x - 1/2 =
-//                            k + d adds 1-d to bin k and d to bin k+1
+//   p_t     = D(dt_t)        quadratic B-spline deposit on bin centres: a
+//                            particle at bin-centre coordinate
+//                            u = (dt - cut_left) inv_bw - 1/2 = k + d,
+//                            k = round(u), adds (1/2 - d)^2 / 2, 3/4 - d^2,
+//                            (1/2 + d)^2 / 2 to bins k-1, k, k+1
 //   V_t     = -scale * (g conv p_t)[:ns],  g = irfft(Z, n_fft) (circular,
 //                            as the FFT route of impedance.py:457-489)
-//   e1      = dE + (dt * slope_k + intercept_k)   (linear_interp_kick.cpp:44-64)
+//   e1      = dE + q sum_b V_t[b] w_b(dt)   the induced kick with the same
+//                            spline weights (linear_interp_kick.cpp:21-71
+//                            interpolates linearly: its slope jumps at every
+//                            bin centre)
 //   e2      = e1 + q sum_j V_j sin(w_j dt + phi_j) + acc   (kick.cpp:16-35)
 //   dt_{t+1} = dt + coeff_{t+1} e2,  dE_{t+1} = e2          (drift.cpp "simple")
 // and the loss depends on p_T.  The reference's smooth_histogram
 // (histogram.cpp:75-132) moves a particle's whole weight between bins k-1
 // and k+1 when it crosses the centre of bin k, so it has no derivative
-// there; the deposit here is the continuous linear one.
+// there.  Deposit and kick here have continuous first derivatives, so
+// central differences of the loss converge at second order (linear
+// weights' kinks at bin centres leave first-order errors of the order of
+// the fraction of particles within one step of a bin centre).
 //
 // Reverse turn t, per particle (state at t recomputed from a checkpoint):
-//   lx <- lx + (lp_{t+1}[k'+1] - lp_{t+1}[k']) inv_bw      deposit of p_{t+1}
+//   lx <- lx + inv_bw sum_b lp_{t+1}[b] dw_b/du              deposit of p_{t+1}
 //   le2 = le + coeff lx
-//   lx <- lx + le2 (q sum_j V_j w_j cos(.) + slope_k)       kick + interp
-//   lV_t[k] += le2 q (1-w), lV_t[k+1] += le2 q w             interp
+//   lx <- lx + le2 (q sum_j V_j w_j cos(.)
+//                   + q inv_bw sum_b V_t[b] dw_b/du)         RF + induced kick
+//   lV_t[b] += le2 q w_b                                      induced kick
 //   dL/dV_j += le2 q sin(.),  dL/dphi_j += le2 q V_j cos(.)
 //   le <- le2
 // then, once per turn on one workgroup: lp_t = -scale * (g corr lV_t) and
@@ -54,32 +63,42 @@ struct AdjArgs {
     const DriftCoeffs<double> *drift;
     int ns;
     double cut_left, inv_bw_cut, fix_scale;
-    const double2 *tab;  // [turn][ns]
-    double bc0, inv_bw_bc;
-    const double *bc;
+    const double *vt;    // [turn][ns] induced voltage V_t
     unsigned long long *dep;  // [copies][ns]
     double *lv;               // [copies][ns]
     const double *lp;         // [ns]
     double *gpart;            // [turn][grid][2 n_rf]
 };
 
-// cloud-in-cell bin and weight: x - 1/2 = k + d in bin units
-__device__ __forceinline__ void cic(double x, const AdjArgs &a, int &k, double &d) {
-    const double xf = (x - a.cut_left) * a.inv_bw_cut - 0.5;
-    const double f = floor(xf);
-    d = xf - f;
-    k = (f >= -1.0 && f < (double)a.ns) ? (int)f : -2;  // -2: no bin touched
+// Quadratic B-spline: u = (x - cut_left) inv_bw - 1/2 = k + d, k = round(u),
+// d in [-1/2, 1/2); weights of bins k-1, k, k+1 and their u-derivatives.
+// k = kNoBin when none of the three bins is inside [0, ns).
+constexpr int kNoBin = -1000000;
+__device__ __forceinline__ int spline(double x, const AdjArgs &a, double w[3], double dw[3]) {
+    const double u = (x - a.cut_left) * a.inv_bw_cut - 0.5;
+    const double kf = floor(u + 0.5);
+    if (!(kf >= -1.0 && kf <= (double)a.ns)) return kNoBin;  // also NaN
+    const double d = u - kf;
+    const double l = 0.5 - d, r = 0.5 + d;
+    w[0] = 0.5 * l * l;
+    w[1] = 0.75 - d * d;
+    w[2] = 0.5 * r * r;
+    dw[0] = -l;
+    dw[1] = -2.0 * d;
+    dw[2] = r;
+    return (int)kf;
 }
 
 __device__ __forceinline__ void deposit_lds(unsigned long long *h, double x, const AdjArgs &a) {
-    int k;
-    double d;
-    cic(x, a, k, d);
-    if (k == -2) return;
-    if (k >= 0)
-        atomicAdd(&h[k], (unsigned long long)__double2ll_rn((1.0 - d) * a.fix_scale));
-    if (k + 1 < a.ns)
-        atomicAdd(&h[k + 1], (unsigned long long)__double2ll_rn(d * a.fix_scale));
+    double w[3], dw[3];
+    const int k = spline(x, a, w, dw);
+    if (k == kNoBin) return;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int b = k - 1 + q;
+        if (b >= 0 && b < a.ns)
+            atomicAdd(&h[b], (unsigned long long)__double2ll_rn(w[q] * a.fix_scale));
+    }
 }
 
 __device__ __forceinline__ void deposit_flush(unsigned long long *h, const AdjArgs &a) {
@@ -100,30 +119,49 @@ __global__ void __launch_bounds__(kAdjBlock) adj_deposit_kernel(const double *__
     deposit_flush(hdep, a);
 }
 
+// q sum_b V[b] w_b(x) and (with D) its u-derivative, V in LDS
+__device__ __forceinline__ double induced_kick(const double *V, double x, const AdjArgs &a,
+                                               double *dkick, double w[3], int &k) {
+    double dw[3];
+    k = spline(x, a, w, dw);
+    double s = 0.0, ds = 0.0;
+    if (k != kNoBin) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int b = k - 1 + q;
+            if (b >= 0 && b < a.ns) {
+                s = s + V[b] * w[q];
+                ds = ds + V[b] * dw[q];
+            }
+        }
+    }
+    if (dkick) *dkick = a.charge * ds * a.inv_bw_cut;
+    return a.charge * s;
+}
+
 // Forward turn t: (xi, ei) -> (xo, eo) (may alias); DEPOSIT adds p_{t+1}.
 template <bool DEPOSIT>
 __global__ void __launch_bounds__(kAdjBlock) adj_turn_kernel(const double *xi, const double *ei,
                                                              double *xo, double *eo, int64_t n,
                                                              int t, AdjArgs a) {
     extern __shared__ unsigned long long hdep[];
-    if (DEPOSIT) {
-        for (int b = threadIdx.x; b < a.ns; b += blockDim.x) hdep[b] = 0ull;
-        __syncthreads();
+    double *vl = reinterpret_cast<double *>(hdep + a.ns);  // V_t
+    for (int b = threadIdx.x; b < a.ns; b += blockDim.x) {
+        if (DEPOSIT) hdep[b] = 0ull;
+        vl[b] = a.vt[(size_t)t * a.ns + b];
     }
+    __syncthreads();
     const double *V = a.rf + (size_t)t * 3 * a.n_rf;
     const double *w = V + a.n_rf;
     const double *phi = w + a.n_rf;
     const double acc = a.acc[t];
     const double coeff = a.drift[t + 1].coeff;
-    const double2 *tab = a.tab + (size_t)t * a.ns;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         double x = xi[i], e = ei[i];
-        const double f = floor((x - a.bc0) * a.inv_bw_bc);
-        if (f >= 0.0 && f < (double)(a.ns - 1)) {
-            const double2 tk = tab[(int)f];
-            e = e + (x * tk.x + tk.y);
-        }
+        double wk[3];
+        int k;
+        e = e + induced_kick(vl, x, a, nullptr, wk, k);
         double s = 0.0;
         for (int j = 0; j < a.n_rf; ++j) s = s + V[j] * cephes_sin(w[j] * x + phi[j]);
         e = e + (a.charge * s + acc);
@@ -135,14 +173,14 @@ __global__ void __launch_bounds__(kAdjBlock) adj_turn_kernel(const double *xi, c
     if (DEPOSIT) deposit_flush(hdep, a);
 }
 
-// One workgroup: p_t from the deposit copies (cleared), stored; with a table,
-// V_t = -scale * (g conv p_t) and its interpolation table.
-__global__ void __launch_bounds__(kAdjOneBlock) adj_profile_kernel(AdjArgs a, int t, bool table,
+// One workgroup: p_t from the deposit copies (cleared), stored; with
+// `voltage`, V_t = -scale * (g conv p_t) stored too.
+__global__ void __launch_bounds__(kAdjOneBlock) adj_profile_kernel(AdjArgs a, int t, bool voltage,
                                                                    double *prof, const double *g,
                                                                    int n_fft, double scale,
-                                                                   double2 *tab_out) {
+                                                                   double *vt) {
     extern __shared__ double sh[];
-    double *p = sh, *V = sh + a.ns, *gs = sh + 2 * a.ns;
+    double *p = sh, *gs = sh + a.ns;
     const double inv = 1.0 / a.fix_scale;
     for (int j = threadIdx.x; j < a.ns; j += blockDim.x) {
         unsigned long long s = 0;
@@ -153,7 +191,7 @@ __global__ void __launch_bounds__(kAdjOneBlock) adj_profile_kernel(AdjArgs a, in
         p[j] = (double)(long long)s * inv;
         prof[(size_t)t * a.ns + j] = p[j];
     }
-    if (!table) return;
+    if (!voltage) return;
     for (int m = threadIdx.x; m < n_fft; m += blockDim.x) gs[m] = g[m];
     __syncthreads();
     for (int i = threadIdx.x; i < a.ns; i += blockDim.x) {
@@ -163,12 +201,7 @@ __global__ void __launch_bounds__(kAdjOneBlock) adj_profile_kernel(AdjArgs a, in
             if (m < 0) m += n_fft;
             v = fma(gs[m], p[j], v);
         }
-        V[i] = -(scale * v);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < a.ns - 1; i += blockDim.x) {
-        const double k = a.charge * (V[i + 1] - V[i]) * a.inv_bw_bc;
-        tab_out[(size_t)t * a.ns + i] = make_double2(k, (a.charge * V[i] - a.bc[i] * k) + 0.0);
+        vt[(size_t)t * a.ns + i] = -(scale * v);
     }
 }
 
@@ -179,11 +212,12 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
                                                                 double *__restrict__ lam_e,
                                                                 int64_t n, int t, AdjArgs a) {
     extern __shared__ double shr[];
-    double *lp = shr, *lvs = shr + a.ns;
+    double *lp = shr, *lvs = shr + a.ns, *vl = shr + 2 * a.ns;
     __shared__ double red[kAdjBlock / 64][2 * kAdjMaxRf];
     for (int b = threadIdx.x; b < a.ns; b += blockDim.x) {
         lp[b] = a.lp[b];
         lvs[b] = 0.0;
+        vl[b] = a.vt[(size_t)t * a.ns + b];
     }
     __syncthreads();
     const double *V = a.rf + (size_t)t * 3 * a.n_rf;
@@ -191,7 +225,6 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
     const double *phi = w + a.n_rf;
     const double acc = a.acc[t];
     const double coeff = a.drift[t + 1].coeff;
-    const double2 *tab = a.tab + (size_t)t * a.ns;
     double gsum[2 * kAdjMaxRf];
 #pragma unroll
     for (int j = 0; j < 2 * kAdjMaxRf; ++j) gsum[j] = 0.0;
@@ -199,15 +232,9 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const double x = xs[i], e = es[i];
         // forward step t, as adj_turn_kernel
-        const double f = floor((x - a.bc0) * a.inv_bw_bc);
-        const bool in = f >= 0.0 && f < (double)(a.ns - 1);
-        const int k = in ? (int)f : 0;
-        double e1 = e;
-        double2 tk = make_double2(0.0, 0.0);
-        if (in) {
-            tk = tab[k];
-            e1 = e + (x * tk.x + tk.y);
-        }
+        double wk[3], dkick;
+        int k;
+        const double e1 = e + induced_kick(vl, x, a, &dkick, wk, k);
         double sn[kAdjMaxRf], cs[kAdjMaxRf];
         double s = 0.0, ds = 0.0;
         for (int j = 0; j < a.n_rf && j < kAdjMaxRf; ++j) {
@@ -218,14 +245,17 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
         const double e2 = e1 + (a.charge * s + acc);
         const double x1 = x + coeff * e2;
         // deposit adjoint of p_{t+1} at x1
-        int kk;
-        double d;
-        cic(x1, a, kk, d);
+        double sw[3], sdw[3];
+        const int kk = spline(x1, a, sw, sdw);
         double lx = lam_x[i];
-        if (kk != -2) {
-            const double l0 = kk >= 0 ? lp[kk] : 0.0;
-            const double l1 = kk + 1 < a.ns ? lp[kk + 1] : 0.0;
-            lx = lx + (l1 - l0) * a.inv_bw_cut;
+        if (kk != kNoBin) {
+            double s3 = 0.0;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int b = kk - 1 + q;
+                if (b >= 0 && b < a.ns) s3 += lp[b] * sdw[q];
+            }
+            lx = lx + s3 * a.inv_bw_cut;
         }
         const double le2 = lam_e[i] + coeff * lx;
         lx = lx + le2 * (a.charge * ds);
@@ -233,11 +263,13 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
             gsum[2 * j] += le2 * a.charge * sn[j];
             gsum[2 * j + 1] += le2 * a.charge * V[j] * cs[j];
         }
-        if (in) {
-            lx = lx + le2 * tk.x;
-            const double wt = (x - a.bc[k]) * a.inv_bw_bc;
-            atomicAdd(&lvs[k], le2 * a.charge * (1.0 - wt));
-            atomicAdd(&lvs[k + 1], le2 * a.charge * wt);
+        lx = lx + le2 * dkick;
+        if (k != kNoBin) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int b = k - 1 + q;
+                if (b >= 0 && b < a.ns) atomicAdd(&lvs[b], le2 * a.charge * wk[q]);
+            }
         }
         lam_x[i] = lx;
         lam_e[i] = le2;
@@ -304,13 +336,12 @@ Adjoint::~Adjoint() { release(); }
 
 void Adjoint::release() {
     void *bufs[] = {d_rf, d_acc, d_drift, d_x0, d_work, d_ckpt, d_slots, d_lam, d_dep,
-                    d_prof, d_tab, d_bc, d_g, d_lv, d_lp, d_corr, d_gpart};
+                    d_prof, d_vt, d_g, d_lv, d_lp, d_corr, d_gpart};
     for (void *p : bufs) (void)hipFree(p);
     d_rf = d_acc = d_x0 = d_work = d_ckpt = d_slots = d_lam = nullptr;
-    d_prof = d_bc = d_g = d_lv = d_lp = d_corr = d_gpart = nullptr;
+    d_prof = d_vt = d_g = d_lv = d_lp = d_corr = d_gpart = nullptr;
     d_drift = nullptr;
     d_dep = nullptr;
-    d_tab = nullptr;
 }
 
 int Adjoint::init(int64_t n_, int n_rf_, int n_turns_, int n_slices_, int every_) {
@@ -335,9 +366,8 @@ int Adjoint::init(int64_t n_, int n_rf_, int n_turns_, int n_slices_, int every_
     BLMI_HIP_TRY(hipMalloc(&d_dep, sizeof(unsigned long long) * kAdjCopies * n_slices));
     BLMI_HIP_TRY(hipMemset(d_dep, 0, sizeof(unsigned long long) * kAdjCopies * n_slices));
     BLMI_HIP_TRY(hipMalloc(&d_prof, sizeof(double) * nt * n_slices));
-    BLMI_HIP_TRY(hipMalloc(&d_tab, sizeof(double2) * nt * n_slices));
-    BLMI_HIP_TRY(hipMemset(d_tab, 0, sizeof(double2) * nt * n_slices));
-    BLMI_HIP_TRY(hipMalloc(&d_bc, sizeof(double) * n_slices));
+    BLMI_HIP_TRY(hipMalloc(&d_vt, sizeof(double) * nt * n_slices));
+    BLMI_HIP_TRY(hipMemset(d_vt, 0, sizeof(double) * nt * n_slices));
     BLMI_HIP_TRY(hipMalloc(&d_lv, sizeof(double) * kAdjCopies * n_slices));
     BLMI_HIP_TRY(hipMemset(d_lv, 0, sizeof(double) * kAdjCopies * n_slices));
     BLMI_HIP_TRY(hipMalloc(&d_lp, sizeof(double) * n_slices));
@@ -373,12 +403,9 @@ int Adjoint::set_programs(const double *rf_host, const double *acc_host, const d
     return BLMI_OK;
 }
 
-int Adjoint::set_slices(double cl, double cr, const double *bc_host) {
+int Adjoint::set_slices(double cl, double cr) {
     cut_left = cl;
     inv_bw_cut = (double)n_slices / (cr - cl);
-    BLMI_HIP_TRY(hipMemcpy(d_bc, bc_host, sizeof(double) * n_slices, hipMemcpyHostToDevice));
-    bc0 = bc_host[0];
-    inv_bw_bc = (double)(n_slices - 1) / (bc_host[n_slices - 1] - bc_host[0]);
     forward_done = false;
     return BLMI_OK;
 }
@@ -412,10 +439,7 @@ static AdjArgs adj_args(const Adjoint &j) {
     a.cut_left = j.cut_left;
     a.inv_bw_cut = j.inv_bw_cut;
     a.fix_scale = ldexp(1.0, j.fixed_bits);
-    a.tab = j.d_tab;
-    a.bc0 = j.bc0;
-    a.inv_bw_bc = j.inv_bw_bc;
-    a.bc = j.d_bc;
+    a.vt = j.d_vt;
     a.dep = j.d_dep;
     a.lv = j.d_lv;
     a.lp = j.d_lp;
@@ -429,13 +453,13 @@ int Adjoint::forward(int turns, double *profile_out, hipStream_t s) {
         return blmi_set_error(BLMI_ERR_ARG, "adjoint: turns %d outside [1, %d]", turns, n_turns);
     const AdjArgs a = adj_args(*this);
     const size_t pn = 2 * (size_t)n;
-    const size_t lds_dep = sizeof(unsigned long long) * n_slices;
-    const size_t lds_one = sizeof(double) * (2 * (size_t)n_slices + n_fft);
+    const size_t lds_dep = 2 * sizeof(double) * n_slices;  // deposit + V_t
+    const size_t lds_one = sizeof(double) * ((size_t)n_slices + n_fft);
     double *x = d_work, *e = d_work + n;
     BLMI_HIP_TRY(hipMemcpyAsync(d_work, d_x0, sizeof(double) * pn, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(adj_deposit_kernel, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, n, a);
     hipLaunchKernelGGL(adj_profile_kernel, dim3(1), dim3(kAdjOneBlock), lds_one, s, a, 0, true,
-                       d_prof, d_g, (int)n_fft, scale, d_tab);
+                       d_prof, d_g, (int)n_fft, scale, d_vt);
     for (int t = 0; t < turns; ++t) {
         if (t % every == 0)
             BLMI_HIP_TRY(hipMemcpyAsync(d_ckpt + (size_t)(t / every) * pn, d_work,
@@ -443,7 +467,7 @@ int Adjoint::forward(int turns, double *profile_out, hipStream_t s) {
         hipLaunchKernelGGL(adj_turn_kernel<true>, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, e, x,
                            e, n, t, a);
         hipLaunchKernelGGL(adj_profile_kernel, dim3(1), dim3(kAdjOneBlock), lds_one, s, a, t + 1,
-                           t + 1 < turns, d_prof, d_g, (int)n_fft, scale, d_tab);
+                           t + 1 < turns, d_prof, d_g, (int)n_fft, scale, d_vt);
     }
     int rc = blmi_check_launch("adjoint_forward");
     if (rc) return rc;
@@ -459,8 +483,8 @@ int Adjoint::backward(const double *dprofile, double *grad_rf_host, double *corr
     if (!forward_done) return blmi_set_error(BLMI_ERR_ARG, "adjoint: run forward first");
     const int T = forward_turns;
     const size_t pn = 2 * (size_t)n;
-    const size_t lds_dep = sizeof(unsigned long long) * n_slices;
-    const size_t lds_rev = sizeof(double) * 2 * (size_t)n_slices;
+    const size_t lds_dep = 2 * sizeof(double) * n_slices;
+    const size_t lds_rev = sizeof(double) * 3 * (size_t)n_slices;
     const size_t lds_one = sizeof(double) * (2 * (size_t)n_slices + n_fft);
     AdjArgs a = adj_args(*this);
     BLMI_HIP_TRY(hipMemsetAsync(d_lam, 0, sizeof(double) * pn, s));

@@ -357,7 +357,37 @@ int blmi_engine_set_particles(void *h, double *dt, double *dE, int64_t n) {
                  "engine: dt and dE must be 16-byte aligned");
     e->dt = dt;
     e->dE = dE;
+    e->f32 = false;
+    e->invalidate_graph();
     return BLMI_OK;
+}
+
+int blmi_engine_set_particles_f32(void *h, float *dt, float *dE, int64_t n) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    BLMI_REQUIRE(n == e->n, "engine: particle count %lld differs from %lld", (long long)n,
+                 (long long)e->n);
+    BLMI_REQUIRE(((((uintptr_t)dt) | ((uintptr_t)dE)) & 7u) == 0,
+                 "engine: float dt and dE must be 8-byte aligned");
+    return e->set_particles_f32(dt, dE);
+}
+
+int blmi_engine_set_slices_f32(void *h, int n_slices, double cut_left, double cut_right,
+                               float *profile) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    BLMI_REQUIRE(n_slices >= 0 && (n_slices == 0 || profile), "engine: bad slices");
+    BLMI_REQUIRE(n_slices == 0 || cut_right > cut_left, "engine: cut_right must exceed cut_left");
+    return e->set_slices_f32(n_slices, cut_left, cut_right, profile);
+}
+
+int blmi_engine_set_induced_f32(void *h, int64_t n_fft, const double *Z_host, double scale,
+                                float *volt, const float *bin_centers) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    BLMI_REQUIRE(e->n_slices >= 2 && e->d_prof64, "engine: set float slices (>= 2) first");
+    BLMI_REQUIRE(volt && bin_centers && Z_host, "engine: null induced-voltage arrays");
+    return e->set_induced_f32(n_fft, Z_host, scale, volt, bin_centers, e->charge);
 }
 
 int blmi_engine_set_programs(void *h, const double *rf, const double *acc, const double *t_rev,
@@ -595,11 +625,11 @@ int blmi_adjoint_set_programs(void *h, const double *rf, const double *acc, cons
                            alpha1, alpha2, beta, energy, charge);
 }
 
-int blmi_adjoint_set_slices(void *h, double cut_left, double cut_right, const double *bin_centers) {
+int blmi_adjoint_set_slices(void *h, double cut_left, double cut_right) {
     Adjoint *a = reinterpret_cast<Adjoint *>(h);
-    BLMI_REQUIRE(a && bin_centers, "adjoint: null argument");
+    BLMI_REQUIRE(a, "adjoint: null handle");
     BLMI_REQUIRE(cut_right > cut_left, "adjoint: cut_right must exceed cut_left");
-    return a->set_slices(cut_left, cut_right, bin_centers);
+    return a->set_slices(cut_left, cut_right);
 }
 
 int blmi_adjoint_set_wake(void *h, int64_t n_fft, const double *g, double scale) {

@@ -56,6 +56,16 @@ struct Engine {
     double *volt = nullptr;        // borrowed: induced voltage seen by the caller
     const double *bc = nullptr;    // borrowed: bin centres
     double *d_table = nullptr;     // (slope, intercept)[n_slices] as double2, then (bc0, inv_bw)
+    // FP32 mode: float particles and programs; the slice axis (profile,
+    // induced voltage, table) stays FP64 in engine-owned buffers and is
+    // converted to the caller's float arrays at the end of every chunk
+    bool f32 = false;
+    float *dt32 = nullptr, *dE32 = nullptr;
+    float *d_rf32 = nullptr, *d_acc32 = nullptr;
+    DriftCoeffs<float> *d_drift32 = nullptr;
+    float cut_left32 = 0.f, inv_bw32 = 0.f;
+    float *profile32 = nullptr, *volt32 = nullptr;
+    double *d_prof64 = nullptr, *d_volt64 = nullptr, *d_bc64 = nullptr;
     // launch shape (tunable)
     int block_size = 1024;
     int blocks_per_cu = 1;
@@ -78,6 +88,10 @@ struct Engine {
     int set_slices(int n_slices, double cut_left, double cut_right, double *profile);
     int set_induced(int64_t n_fft, const double *Z_host, double scale, double *volt,
                     const double *bc, double charge);
+    int set_particles_f32(float *dt, float *dE);
+    int set_slices_f32(int n_slices, double cut_left, double cut_right, float *profile);
+    int set_induced_f32(int64_t n_fft, const double *Z_host, double scale, float *volt,
+                        const float *bc, double charge);
     int set_method(int method);
     int set_hist(int groups, int max_peel);
     int alloc_counts();

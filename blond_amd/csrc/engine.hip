@@ -49,12 +49,15 @@
 
 namespace blmi {
 
+// T: the particle precision (double, or float for the FP32 mode, whose
+// slice-axis work -- profile, induced voltage, table -- stays in FP64).
+template <typename T>
 struct TurnArgs {
     int n_rf;
-    double charge;
-    const double *rf;                  // [turn][3][n_rf]: V, omega, phi
-    const double *acc;                 // [turn]
-    const DriftCoeffs<double> *drift;  // [turn]
+    T charge;
+    const T *rf;                       // [turn][3][n_rf]: V, omega, phi
+    const T *acc;                      // [turn]
+    const DriftCoeffs<T> *drift;       // [turn]
     const int *base;                   // device turn base
     int off;                           // this launch's turn offset
     int last_turn;                     // n_turns - 1
@@ -67,7 +70,7 @@ struct TurnArgs {
     unsigned *counts0, *counts1;
     int groups;
     int n_slices;
-    double cut_left, inv_bw;
+    T cut_left, inv_bw;
     int max_peel;
     // plain mode: the previous turn's counts are published by all workgroups
     double *profile;
@@ -142,18 +145,19 @@ __device__ __forceinline__ void publish_range(unsigned *__restrict__ c, double *
     }
 }
 
-template <int SOLVER, int INTERP, bool HIST, int LDSH, bool PUBLISH>
-__global__ void __launch_bounds__(1024) turn_kernel(double *__restrict__ dt, double *__restrict__ dE,
-                                                    int64_t n, TurnArgs a) {
+template <typename T, int SOLVER, int INTERP, bool HIST, int LDSH, bool PUBLISH>
+__global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__restrict__ dE,
+                                                    int64_t n, TurnArgs<T> a) {
+    using V2 = typename Vec2<T>::type;
     extern __shared__ unsigned lds_hist[];
     int t = *a.base + a.off;
     if (t < 0) t = 0;
     if (t > a.last_turn) t = a.last_turn;  // host checks the range; never read past the tables
-    const double *V = a.rf + (size_t)t * 3 * a.n_rf;
-    const double *w = V + a.n_rf;
-    const double *phi = w + a.n_rf;
-    const double acc = a.acc[t];
-    const DriftCoeffs<double> dc = a.drift[t + 1];
+    const T *V = a.rf + (size_t)t * 3 * a.n_rf;
+    const T *w = V + a.n_rf;
+    const T *phi = w + a.n_rf;
+    const T acc = a.acc[t];
+    const DriftCoeffs<T> dc = a.drift[t + 1];
     unsigned *counts = (((t + 1) & 1) ? a.counts1 : a.counts0) +
                        (size_t)(blockIdx.x % a.groups) * a.n_slices;
     if (PUBLISH) {
@@ -168,31 +172,31 @@ __global__ void __launch_bounds__(1024) turn_kernel(double *__restrict__ dt, dou
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) a.pending[(t + 1) & 1] = t + 1;
     }
-    double bc0 = 0.0, ibw = 0.0;
+    T bc0 = T(0), ibw = T(0);
     if (INTERP) {
-        bc0 = a.tab_params[0];
-        ibw = a.tab_params[1];
+        bc0 = (T)a.tab_params[0];
+        ibw = (T)a.tab_params[1];
     }
     if (HIST && LDSH) {
         const int words = hist_words(LDSH, a.n_slices);
         for (int b = threadIdx.x; b < words; b += blockDim.x) lds_hist[b] = 0u;
         __syncthreads();
     }
-    auto step = [&](double &x, double &e) -> int {
+    auto step = [&](T &x, T &e) -> int {
         if (INTERP) {
             // linear_interp_kick.cpp:44-64 (acceleration kick 0)
-            const double f = floor((x - bc0) * ibw);
-            if (f >= 0.0 && f < (double)a.n_tab) {
+            const T f = floor((x - bc0) * ibw);
+            if (f >= T(0) && f < (T)a.n_tab) {
                 const int k = (int)f;
                 const double2 tk = a.tab[k];
-                e = e + (x * tk.x + tk.y);
+                e = e + (x * (T)tk.x + (T)tk.y);
             }
         }
         e = e + e_rf_kick(x, a.n_rf, a.charge, V, w, phi, acc);
         x = x + e_drift(dc, e, SOLVER);
         if (HIST) {
-            const double f = floor((x - a.cut_left) * a.inv_bw);
-            return (f >= 0.0 && f < (double)a.n_slices) ? (int)f : -1;
+            const T f = floor((x - a.cut_left) * a.inv_bw);
+            return (f >= T(0) && f < (T)a.n_slices) ? (int)f : -1;
         }
         return -1;
     };
@@ -206,12 +210,12 @@ __global__ void __launch_bounds__(1024) turn_kernel(double *__restrict__ dt, dou
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t n2 = n / 2;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) {
-        double2 x = reinterpret_cast<const double2 *>(dt)[i];
-        double2 e = reinterpret_cast<const double2 *>(dE)[i];
+        V2 x = reinterpret_cast<const V2 *>(dt)[i];
+        V2 e = reinterpret_cast<const V2 *>(dE)[i];
         const int b0 = step(x.x, e.x);
         const int b1 = step(x.y, e.y);
-        reinterpret_cast<double2 *>(dt)[i] = x;
-        reinterpret_cast<double2 *>(dE)[i] = e;
+        reinterpret_cast<V2 *>(dt)[i] = x;
+        reinterpret_cast<V2 *>(dE)[i] = e;
         count(b0);
         count(b1);
     }
@@ -220,7 +224,7 @@ __global__ void __launch_bounds__(1024) turn_kernel(double *__restrict__ dt, dou
         // aggregated count, lane 0 owns the particle
         int b = -1;
         if (threadIdx.x == 0) {
-            double x = dt[n - 1], e = dE[n - 1];
+            T x = dt[n - 1], e = dE[n - 1];
             b = step(x, e);
             dt[n - 1] = x;
             dE[n - 1] = e;
@@ -247,18 +251,29 @@ __global__ void publish_kernel(const int *base, int off, unsigned *counts0, unsi
     if (threadIdx.x == 0) pending[t & 1] = -1;
 }
 
+__global__ void to_float_kernel(const double *__restrict__ src, float *__restrict__ dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
+}
+
+__global__ void to_double_kernel(const float *__restrict__ src, double *__restrict__ dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (double)src[i];
+}
+
 // *base += k, after every launch of the chunk has read it (stream order).
 __global__ void advance_kernel(int *base, int k) { *base += k; }
 
 // Histogram of the current coordinates (engine priming), in the particle
 // kernel's layout: counts[t & 1] (cleared first by a fill kernel) for modes
 // 0 and 1, the workgroups' partial rows for mode 2 (reduced afterwards).
-template <int LDSH>
-__global__ void __launch_bounds__(1024) prime_hist_kernel(const double *__restrict__ dt, int64_t n,
+template <typename T, int LDSH>
+__global__ void __launch_bounds__(1024) prime_hist_kernel(const T *__restrict__ dt, int64_t n,
                                                           const int *base, unsigned *counts0,
                                                           unsigned *counts1, int n_slices,
-                                                          double cut_left, double inv_bw,
+                                                          T cut_left, T inv_bw,
                                                           unsigned *partials, int part_words) {
+    using V2 = typename Vec2<T>::type;
     extern __shared__ unsigned lds_hist[];
     const int t = *base;
     unsigned *counts = (t & 1) ? counts1 : counts0;
@@ -267,9 +282,9 @@ __global__ void __launch_bounds__(1024) prime_hist_kernel(const double *__restri
         for (int b = threadIdx.x; b < words; b += blockDim.x) lds_hist[b] = 0u;
         __syncthreads();
     }
-    auto bin_of = [&](double x) -> int {
-        const double f = floor((x - cut_left) * inv_bw);
-        return (f >= 0.0 && f < (double)n_slices) ? (int)f : -1;
+    auto bin_of = [&](T x) -> int {
+        const T f = floor((x - cut_left) * inv_bw);
+        return (f >= T(0) && f < (T)n_slices) ? (int)f : -1;
     };
     auto count = [&](int bin) {
         if (LDSH)
@@ -280,7 +295,7 @@ __global__ void __launch_bounds__(1024) prime_hist_kernel(const double *__restri
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t n2 = n / 2;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) {
-        const double2 x = reinterpret_cast<const double2 *>(dt)[i];
+        const V2 x = reinterpret_cast<const V2 *>(dt)[i];
         count(bin_of(x.x));
         count(bin_of(x.y));
     }
@@ -313,6 +328,15 @@ void Engine::release() {
     (void)hipFree(d_partials);
     d_partials = nullptr;
     partials_bytes = 0;
+    (void)hipFree(d_rf32);
+    (void)hipFree(d_acc32);
+    (void)hipFree(d_drift32);
+    (void)hipFree(d_prof64);
+    (void)hipFree(d_volt64);
+    (void)hipFree(d_bc64);
+    d_rf32 = d_acc32 = nullptr;
+    d_drift32 = nullptr;
+    d_prof64 = d_volt64 = d_bc64 = nullptr;
     fft2_plan_destroy(&plan2);
     d_rf = d_acc = d_table = d_gl = nullptr;
     d_drift = nullptr;
@@ -360,8 +384,65 @@ int Engine::set_programs(const double *rf_host, const double *acc_host, const do
                                           beta[i], energy[i]);
     BLMI_HIP_TRY(hipMemcpy(d_drift, dc.data(), sizeof(DriftCoeffs<double>) * nt,
                            hipMemcpyHostToDevice));
+    // FP32 programs: the scalars as the single-precision library receives
+    // them (cast to float), coefficients formed in float
+    std::vector<float> rf32((size_t)3 * n_rf * nt), acc32(nt);
+    for (size_t i = 0; i < rf32.size(); ++i) rf32[i] = (float)rf_host[i];
+    for (size_t i = 0; i < nt; ++i) acc32[i] = (float)acc_host[i];
+    std::vector<DriftCoeffs<float>> dc32(nt);
+    for (size_t i = 0; i < nt; ++i)
+        dc32[i] = make_drift_coeffs<float>(solver, (float)t_rev[i], (float)length_ratio,
+                                           (float)alpha_order, (float)eta0[i], (float)eta1[i],
+                                           (float)eta2[i], (float)alpha0[i], (float)alpha1[i],
+                                           (float)alpha2[i], (float)beta[i], (float)energy[i]);
+    if (!d_rf32) {
+        BLMI_HIP_TRY(hipMalloc(&d_rf32, sizeof(float) * rf32.size()));
+        BLMI_HIP_TRY(hipMalloc(&d_acc32, sizeof(float) * nt));
+        BLMI_HIP_TRY(hipMalloc(&d_drift32, sizeof(DriftCoeffs<float>) * nt));
+    }
+    BLMI_HIP_TRY(hipMemcpy(d_rf32, rf32.data(), sizeof(float) * rf32.size(), hipMemcpyHostToDevice));
+    BLMI_HIP_TRY(hipMemcpy(d_acc32, acc32.data(), sizeof(float) * nt, hipMemcpyHostToDevice));
+    BLMI_HIP_TRY(hipMemcpy(d_drift32, dc32.data(), sizeof(DriftCoeffs<float>) * nt,
+                           hipMemcpyHostToDevice));
     invalidate_graph();
     return BLMI_OK;
+}
+
+int Engine::set_particles_f32(float *x, float *e) {
+    f32 = true;
+    dt32 = x;
+    dE32 = e;
+    dt = dE = nullptr;
+    invalidate_graph();
+    return BLMI_OK;
+}
+
+int Engine::set_slices_f32(int ns, double cl, double cr, float *profile_user) {
+    (void)hipFree(d_prof64);
+    d_prof64 = nullptr;
+    if (ns > 0) BLMI_HIP_TRY(hipMalloc(&d_prof64, sizeof(double) * ns));
+    profile32 = profile_user;
+    // histogram.cpp:29 in float: the cuts arrive as float
+    cut_left32 = (float)cl;
+    inv_bw32 = (float)ns / ((float)cr - (float)cl);
+    return set_slices(ns, cl, cr, d_prof64);
+}
+
+int Engine::set_induced_f32(int64_t n_fft_, const double *Z_host, double scale_, float *volt_user,
+                            const float *bc_user, double charge_) {
+    (void)hipFree(d_volt64);
+    (void)hipFree(d_bc64);
+    d_volt64 = d_bc64 = nullptr;
+    BLMI_HIP_TRY(hipMalloc(&d_volt64, sizeof(double) * n_slices));
+    BLMI_HIP_TRY(hipMemset(d_volt64, 0, sizeof(double) * n_slices));
+    BLMI_HIP_TRY(hipMalloc(&d_bc64, sizeof(double) * n_slices));
+    hipLaunchKernelGGL(to_double_kernel, dim3((n_slices + 255) / 256), dim3(256), 0, nullptr,
+                       bc_user, d_bc64, n_slices);
+    int rc = blmi_check_launch("bin centres");
+    if (rc) return rc;
+    BLMI_HIP_TRY(hipDeviceSynchronize());
+    volt32 = volt_user;
+    return set_induced(n_fft_, Z_host, scale_, d_volt64, d_bc64, charge_);
 }
 
 int Engine::set_slices(int n_slices_, double cut_left_, double cut_right_, double *profile_) {
@@ -570,6 +651,27 @@ int Engine::launch_reduce(hipStream_t s, int off) {
     return blmi_check_launch("partials_reduce");
 }
 
+template <typename T>
+static int launch_prime(Engine *e, const T *x, T cl, T ibw, int mode, int grid, size_t lds,
+                        unsigned *c1, int W, hipStream_t s) {
+    const void *fn = mode == 2   ? (const void *)prime_hist_kernel<T, 2>
+                     : mode == 1 ? (const void *)prime_hist_kernel<T, 1>
+                                 : (const void *)prime_hist_kernel<T, 0>;
+    int rc = lds_attribute(fn, lds);
+    if (rc) return rc;
+    const int blk = e->block_size;
+    if (mode == 2)
+        hipLaunchKernelGGL((prime_hist_kernel<T, 2>), dim3(grid), dim3(blk), lds, s, x, e->n,
+                           e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W);
+    else if (mode == 1)
+        hipLaunchKernelGGL((prime_hist_kernel<T, 1>), dim3(grid), dim3(blk), lds, s, x, e->n,
+                           e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W);
+    else
+        hipLaunchKernelGGL((prime_hist_kernel<T, 0>), dim3(grid), dim3(blk), 0, s, x, e->n,
+                           e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W);
+    return BLMI_OK;
+}
+
 int Engine::prime(hipStream_t s) {
     // counts of the current coordinates (the first pass of the map's
     // Profile.track()) into counts[t & 1]; the other buffer cleared
@@ -581,40 +683,78 @@ int Engine::prime(hipStream_t s) {
     int grid = 0;
     size_t lds = 0;
     const int mode = hist_layout(block_size, &grid, &lds);
-    const void *fn = mode == 2   ? (const void *)prime_hist_kernel<2>
-                     : mode == 1 ? (const void *)prime_hist_kernel<1>
-                                 : (const void *)prime_hist_kernel<0>;
-    rc = lds_attribute(fn, lds);
-    if (rc) return rc;
     unsigned *c1 = d_counts + (size_t)groups * n_slices;
     const int W = hist_words(2, n_slices);
-    if (mode == 2)
-        hipLaunchKernelGGL(prime_hist_kernel<2>, dim3(grid), dim3(block_size), lds, s, dt, n,
-                           d_base, d_counts, c1, n_slices, cut_left, inv_bw, d_partials, W);
-    else if (mode == 1)
-        hipLaunchKernelGGL(prime_hist_kernel<1>, dim3(grid), dim3(block_size), lds, s, dt, n,
-                           d_base, d_counts, c1, n_slices, cut_left, inv_bw, d_partials, W);
+    if (f32)
+        rc = launch_prime<float>(this, dt32, cut_left32, inv_bw32, mode, grid, lds, c1, W, s);
     else
-        hipLaunchKernelGGL(prime_hist_kernel<0>, dim3(grid), dim3(block_size), 0, s, dt, n,
-                           d_base, d_counts, c1, n_slices, cut_left, inv_bw, d_partials, W);
+        rc = launch_prime<double>(this, dt, cut_left, inv_bw, mode, grid, lds, c1, W, s);
+    if (rc) return rc;
     rc = blmi_check_launch("engine_prime");
     if (rc) return rc;
     return mode == 2 ? launch_reduce(s, 0) : BLMI_OK;
 }
 
-using TurnFn = void (*)(double *, double *, int64_t, TurnArgs);
+template <typename T>
+using TurnFn = void (*)(T *, T *, int64_t, TurnArgs<T>);
 
-template <int S>
-static TurnFn pick_turn(int interp, bool hist, int ldsh) {
-    if (!hist) return interp ? turn_kernel<S, 1, false, 0, false> : turn_kernel<S, 0, false, 0, false>;
+template <typename T, int S>
+static TurnFn<T> pick_turn(int interp, bool hist, int ldsh) {
+    if (!hist)
+        return interp ? turn_kernel<T, S, 1, false, 0, false> : turn_kernel<T, S, 0, false, 0, false>;
     switch (interp * 3 + ldsh) {
-        case 0: return turn_kernel<S, 0, true, 0, true>;
-        case 1: return turn_kernel<S, 0, true, 1, true>;
-        case 2: return turn_kernel<S, 0, true, 2, true>;
-        case 3: return turn_kernel<S, 1, true, 0, false>;
-        case 4: return turn_kernel<S, 1, true, 1, false>;
-        default: return turn_kernel<S, 1, true, 2, false>;
+        case 0: return turn_kernel<T, S, 0, true, 0, true>;
+        case 1: return turn_kernel<T, S, 0, true, 1, true>;
+        case 2: return turn_kernel<T, S, 0, true, 2, true>;
+        case 3: return turn_kernel<T, S, 1, true, 0, false>;
+        case 4: return turn_kernel<T, S, 1, true, 1, false>;
+        default: return turn_kernel<T, S, 1, true, 2, false>;
     }
+}
+
+template <typename T>
+static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc,
+                         const DriftCoeffs<T> *drift, T cl, T ibw, hipStream_t s, int off,
+                         bool interp, const KernelEvents &ev) {
+    const bool hist = e.n_slices > 0 && e.d_counts != nullptr;
+    TurnArgs<T> a{};
+    a.n_rf = e.n_rf;
+    a.charge = (T)e.charge;
+    a.rf = rf;
+    a.acc = acc;
+    a.drift = drift;
+    a.base = e.d_base;
+    a.off = off;
+    a.last_turn = e.n_turns - 1;
+    a.tab = reinterpret_cast<const double2 *>(e.d_table);
+    a.tab_params = e.d_table ? e.d_table + 2 * e.n_slices : nullptr;
+    a.n_tab = e.n_slices - 1;
+    a.counts0 = e.d_counts;
+    a.counts1 = e.d_counts ? e.d_counts + (size_t)e.groups * e.n_slices : nullptr;
+    a.groups = e.groups;
+    a.n_slices = e.n_slices;
+    a.cut_left = cl;
+    a.inv_bw = ibw;
+    a.max_peel = e.max_peel;
+    a.profile = e.profile;
+    a.pending = e.d_base + 1;
+    a.partials = e.d_partials;
+    a.part_words = hist_words(2, e.n_slices);
+    int grid = 0;
+    size_t lds = 0;
+    int ldsh = e.hist_layout(e.block_size, &grid, &lds);
+    if (!hist) {
+        ldsh = 0;
+        lds = 0;
+    }
+    const int imode = interp ? 1 : 0;
+    TurnFn<T> fn = e.solver == 0   ? pick_turn<T, 0>(imode, hist, ldsh)
+                   : e.solver == 1 ? pick_turn<T, 1>(imode, hist, ldsh)
+                                   : pick_turn<T, 2>(imode, hist, ldsh);
+    int rc = lds_attribute((const void *)fn, lds);
+    if (rc) return rc;
+    launch(fn, dim3(grid), dim3(e.block_size), lds, s, ev, x, de, e.n, a);
+    return blmi_check_launch("turn");
 }
 
 InducedArgs Engine::induced_args(int off) const {
@@ -648,46 +788,11 @@ int Engine::launch_induced(hipStream_t s, int off, const KernelEvents &ev) {
 }
 
 int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const KernelEvents &ev) {
-    const bool hist = n_slices > 0 && d_counts != nullptr;
-    TurnArgs a{};
-    a.n_rf = n_rf;
-    a.charge = charge;
-    a.rf = d_rf;
-    a.acc = d_acc;
-    a.drift = d_drift;
-    a.base = d_base;
-    a.off = off;
-    a.last_turn = n_turns - 1;
-    a.tab = reinterpret_cast<const double2 *>(d_table);
-    a.tab_params = d_table ? d_table + 2 * n_slices : nullptr;
-    a.n_tab = n_slices - 1;
-    a.counts0 = d_counts;
-    a.counts1 = d_counts ? d_counts + (size_t)groups * n_slices : nullptr;
-    a.groups = groups;
-    a.n_slices = n_slices;
-    a.cut_left = cut_left;
-    a.inv_bw = inv_bw;
-    a.max_peel = max_peel;
-    a.profile = profile;
-    a.pending = d_base + 1;
-    a.partials = d_partials;
-    a.part_words = hist_words(2, n_slices);
-    const int block = block_size;
-    int grid = 0;
-    size_t lds = 0;
-    int ldsh = hist_layout(block, &grid, &lds);
-    if (!hist) {
-        ldsh = 0;
-        lds = 0;
-    }
-    const int imode = interp ? 1 : 0;
-    TurnFn fn = solver == 0   ? pick_turn<0>(imode, hist, ldsh)
-                : solver == 1 ? pick_turn<1>(imode, hist, ldsh)
-                              : pick_turn<2>(imode, hist, ldsh);
-    int rc = lds_attribute((const void *)fn, lds);
-    if (rc) return rc;
-    launch(fn, dim3(grid), dim3(block), lds, s, ev, dt, dE, n, a);
-    return blmi_check_launch("turn");
+    if (f32)
+        return launch_turn_t<float>(*this, dt32, dE32, d_rf32, d_acc32, d_drift32, cut_left32,
+                                    inv_bw32, s, off, interp, ev);
+    return launch_turn_t<double>(*this, dt, dE, d_rf, d_acc, d_drift, cut_left, inv_bw, s, off,
+                                 interp, ev);
 }
 
 // Turns base+off0 .. base+off0+count-1, then (if advance) *base += count and,
@@ -714,6 +819,15 @@ int Engine::finish_chunk(hipStream_t s, int count) {
                            n_slices);
         int rc = blmi_check_launch("publish");
         if (rc) return rc;
+    }
+    if (f32) {
+        // the FP32 caller's profile and induced voltage
+        if (profile32 && n_slices > 0)
+            hipLaunchKernelGGL(to_float_kernel, dim3((n_slices + 255) / 256), dim3(256), 0, s,
+                               (const double *)profile, profile32, n_slices);
+        if (volt32 && intensity)
+            hipLaunchKernelGGL(to_float_kernel, dim3((n_slices + 255) / 256), dim3(256), 0, s,
+                               (const double *)volt, volt32, n_slices);
     }
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, d_base, count);
     return blmi_check_launch("advance");
@@ -764,7 +878,8 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
 
 int Engine::track(int turns, hipStream_t user, bool use_graph) {
     if (turns <= 0) return BLMI_OK;
-    if (!dt || !dE) return blmi_set_error(BLMI_ERR_ARG, "engine has no particle arrays");
+    if (f32 ? (!dt32 || !dE32) : (!dt || !dE))
+        return blmi_set_error(BLMI_ERR_ARG, "engine has no particle arrays");
     // the per-turn tables hold turns [0, n_turns]; turn t reads entries t and t+1
     if (host_turn + turns > n_turns)
         return blmi_set_error(BLMI_ERR_ARG, "tracking %d turns from turn %d passes n_turns=%d",

@@ -779,7 +779,10 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
     }
 }
 
-__global__ void fft2_b_kernel(InducedArgs a, int M1, int M2, StageList st2,
+constexpr int kPassBThreads = 512;
+
+__global__ void __launch_bounds__(kPassBThreads) fft2_b_kernel(InducedArgs a, int M1, int M2,
+                                                               StageList st2,
                               const double2 *__restrict__ twM, const double2 *__restrict__ twN,
                               const double2 *__restrict__ T, double2 *__restrict__ S) {
     extern __shared__ double2 buf[];
@@ -789,7 +792,7 @@ __global__ void fft2_b_kernel(InducedArgs a, int M1, int M2, StageList st2,
     const int r0 = q, r1 = (M1 - q) % M1;
     const int nrow = r1 == r0 ? 1 : 2;
     const int ne = nrow * M2;
-    constexpr int EMAX = 8;  // host guarantees ne <= EMAX * blockDim.x
+    constexpr int EMAX = 4;  // host guarantees ne <= EMAX * blockDim.x
     // everything the split/merge needs is requested up front, so its latency
     // overlaps the row loads and the forward transforms
     double2 zk[EMAX], zmk[EMAX], wk[EMAX], wmk[EMAX], wo[EMAX];
@@ -939,10 +942,10 @@ int fft2_plan_create(int64_t n_real, Fft2Plan *plan) {
     if (n_real > (int64_t)1 << 30)
         return blmi_set_error(BLMI_ERR_UNSUPPORTED, "FFT length %lld too large", (long long)n_real);
     const int M = (int)(n_real / 2);
-    // M2: the largest divisor <= 2048 that is a multiple of kColBatch with an
+    // M2: the largest divisor <= 1024 that is a multiple of kColBatch with an
     // M1-point column batch fitting the pass A/C workgroup
     int M1 = 0, M2 = 0;
-    for (int d = 2048; d >= kColBatch; d -= kColBatch) {
+    for (int d = 1024; d >= kColBatch; d -= kColBatch) {
         if (M % d) continue;
         const int m1 = M / d;
         if (m1 < 2 || (kColBatch + 1) * m1 > 15 * kColThreads) continue;
@@ -966,7 +969,7 @@ int fft2_plan_create(int64_t n_real, Fft2Plan *plan) {
     // pass B: at most 4 elements a thread (registers hold their operands)
     int tb = (2 * M2 + 3) / 4;
     tb = ((tb + 63) / 64) * 64;
-    p.threads_b = tb < 256 ? 256 : tb > 1024 ? 1024 : tb;
+    p.threads_b = tb < 256 ? 256 : tb;  // <= kPassBThreads for M2 <= 1024
     auto cis = [](long double num, long double den) {
         const long double two_pi = 6.283185307179586476925286766559L;
         long double x = two_pi * num / den;

@@ -10,11 +10,14 @@ of the library (``blmi_adjoint_*``): turn checkpoints every
 checkpoint, and a reverse pass over the particles.
 
 The map is the fused engine's, with one difference: the profile is
-deposited continuously (cloud-in-cell on bin centres).  The reference's
-smooth_histogram (histogram.cpp:75-132) moves a particle's whole weight from
-bin k-1 to bin k+1 as it crosses the centre of bin k, so a loss built on it
-has no derivative there; a continuous deposit is what makes the loss
-differentiable.  The induced voltage is -scale * (g circular-convolved with
+deposited, and the induced voltage interpolated for the kick, with one
+quadratic B-spline on bin centres (weights (1/2-d)^2/2, 3/4-d^2, (1/2+d)^2/2
+on bins k-1, k, k+1), which has a continuous first derivative.  The
+reference's smooth_histogram (histogram.cpp:75-132) moves a particle's whole
+weight from bin k-1 to bin k+1 as it crosses the centre of bin k, so a loss
+built on it has no derivative there; linear weights (and the linear
+interpolation of linear_interp_kick.cpp:21-71) are continuous but their
+kinks make finite differences converge only at first order.  The induced voltage is -scale * (g circular-convolved with
 the profile), g = irfft(Z, n_fft) (impedance.py:457-489), Z from the
 library's resonator kernel (fast_resonator.cpp:45-71) divided by the bin
 size (impedance.py:881-890).
@@ -82,12 +85,8 @@ class DifferentiableTracker:
         dt, dE = dt.to(torch.float64).contiguous(), dE.to(torch.float64).contiguous()
         _lib.call("blmi_adjoint_set_particles", self._h, ct.c_void_p(dt.data_ptr()),
                   ct.c_void_p(dE.data_ptr()))
-        bc = np.ascontiguousarray(
-            profile.bin_centers.cpu().numpy() if hasattr(profile.bin_centers, "cpu")
-            else np.asarray(profile.bin_centers), dtype=np.float64)
-        self._bc = bc
         _lib.call("blmi_adjoint_set_slices", self._h, float(profile.cut_left),
-                  float(profile.cut_right), _dptr(bc))
+                  float(profile.cut_right))
         self._ring = ring_program_arrays(rp)
         self._acc = np.zeros(rp.n_turns + 1)
         self._acc[: len(tracker.acceleration_kick)] = tracker.acceleration_kick

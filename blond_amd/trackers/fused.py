@@ -37,6 +37,10 @@ def _dptr(a: np.ndarray):
     return a.ctypes.data_as(ct.c_void_p)
 
 
+def _host_array(a):
+    return a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a)
+
+
 def phase_program(rp):
     """phi_rf for all turns with the accumulated offsets of tracker.py:556-568
     applied, from the current counter on (the recurrence depends only on the
@@ -93,8 +97,6 @@ class FusedTracker:
                  method=None):
         import torch
 
-        if bm.precision.num != 2:
-            raise NotImplementedError("the fused engine runs in double precision")
         if tracker.interpolation:
             raise NotImplementedError("interpolation=True is tracked object by object")
         self.tracker = tracker
@@ -114,8 +116,9 @@ class FusedTracker:
             self.profile.to_gpu()
         if self.tiv is not None:
             self.tiv.to_gpu()
-        if self.beam.dt.dtype != torch.float64:
-            raise TypeError("beam coordinates must be float64")
+        self.f32 = self.beam.dt.dtype == torch.float32
+        if self.beam.dt.dtype not in (torch.float64, torch.float32):
+            raise TypeError("beam coordinates must be float64 or float32")
         self._torch = torch
         rp = self.rf
         n_turns = rp.n_turns
@@ -126,12 +129,16 @@ class FusedTracker:
         if not h:
             raise _lib.BlmiError(f"blmi_engine_create: {_lib.last_error()}")
         self._h = ct.c_void_p(h)
-        _lib.call("blmi_engine_set_particles", self._h, _ptr(self.beam.dt), _ptr(self.beam.dE), n)
+        _lib.call("blmi_engine_set_particles_f32" if self.f32 else "blmi_engine_set_particles",
+                  self._h, _ptr(self.beam.dt), _ptr(self.beam.dE), n)
         self._upload_programs()
         if self.profile is not None:
             p = self.profile
-            _lib.call("blmi_engine_set_slices", self._h, p.n_slices, float(p.cut_left),
-                      float(p.cut_right), _ptr(p.n_macroparticles))
+            if p.n_macroparticles.dtype != self.beam.dt.dtype:
+                p.n_macroparticles = p.n_macroparticles.to(self.beam.dt.dtype).contiguous()
+            _lib.call("blmi_engine_set_slices_f32" if self.f32 else "blmi_engine_set_slices",
+                      self._h, p.n_slices, float(p.cut_left), float(p.cut_right),
+                      _ptr(p.n_macroparticles))
         if self.tiv is not None:
             self._setup_induced()
         _lib.call("blmi_engine_set_launch", self._h, int(block), int(blocks_per_cu),
@@ -192,12 +199,17 @@ class FusedTracker:
         scale = self.beam.particle.charge * e * self.beam.ratio
         p = self.profile
         iv = self.tiv.induced_voltage
+        dtype = self.beam.dt.dtype
         if not (hasattr(iv, "is_cuda") and iv.is_cuda and iv.numel() == p.n_slices
-                and iv.dtype == torch.float64 and iv.is_contiguous()):
-            iv = torch.zeros(p.n_slices, dtype=torch.float64, device="cuda")
+                and iv.dtype == dtype and iv.is_contiguous()):
+            iv = torch.zeros(p.n_slices, dtype=dtype, device="cuda")
             self.tiv.induced_voltage = iv
-        _lib.call("blmi_engine_set_induced", self._h, int(n_fft), _dptr(self._Z), float(scale),
-                  _ptr(iv), _ptr(p.bin_centers))
+        bc = p.bin_centers
+        if not (hasattr(bc, "is_cuda") and bc.dtype == dtype and bc.is_contiguous()):
+            bc = torch.as_tensor(np.asarray(_host_array(bc)), device="cuda").to(dtype).contiguous()
+            self._bc = bc
+        _lib.call("blmi_engine_set_induced_f32" if self.f32 else "blmi_engine_set_induced",
+                  self._h, int(n_fft), _dptr(self._Z), float(scale), _ptr(iv), _ptr(bc))
 
     # ------------------------------------------------------------------
     def invalidate(self):

@@ -274,7 +274,7 @@ class AdjointCase:
 
 
 def build_adjoint_case(n_particles: int = 1_000_000, n_turns: int = 500, n_slices: int = 256,
-                       seed: int = 11, intensity: float = 1e11) -> AdjointCase:
+                       seed: int = 11, intensity: float = 1e9) -> AdjointCase:
     """Nominal parameters theta0 = (V 6 MV, phi 0, broadband R 1 MOhm,
     f_r 2 GHz, Q 1); hidden theta* drawn from default_rng(seed) within a few
     per cent of them (phi within +-0.02 rad).  Host-only (no GPU needed)."""

@@ -147,6 +147,16 @@ int blmi_engine_set_programs(void *engine, const double *rf_host, const double *
                              const double *beta_host, const double *energy_host, double charge);
 int blmi_engine_set_slices(void *engine, int n_slices, double cut_left, double cut_right,
                            double *profile);
+/* FP32 mode (the reference's libblond_single, utils/__init__.py:18-55): float
+ * dt/dE (8-byte aligned), float programs (the double host programs cast to
+ * float, as the single library receives them), float cuts; the profile,
+ * induced voltage and interpolation table are formed in FP64 inside the
+ * engine and written to the caller's float arrays after each track(). */
+int blmi_engine_set_particles_f32(void *engine, float *dt, float *dE, int64_t n);
+int blmi_engine_set_slices_f32(void *engine, int n_slices, double cut_left, double cut_right,
+                               float *profile);
+int blmi_engine_set_induced_f32(void *engine, int64_t n_fft, const double *Z_host, double scale,
+                                float *volt, const float *bin_centers);
 int blmi_engine_set_induced(void *engine, int64_t n_fft, const double *Z_host, double scale,
                             double *volt, const double *bin_centers);
 /* Launch shape of the particle kernel (block in {64..1024}, workgroups per CU)
@@ -224,12 +234,12 @@ int blmi_ensemble_autotune(void *ensemble, int turns, int *block, int *wg_per_cu
                            float *ms_per_turn, void *stream);
 
 /* ---- differentiable tracking (adjoint) -------------------------------------
- * The intensity map of the fused engine -- profile, induced voltage, linear
- * interpolated kick (linear_interp_kick.cpp:21-71), RF kick (kick.cpp:16-35),
- * "simple" drift (drift.cpp:34-37) -- with the profile deposited
- * continuously (cloud-in-cell on bin centres; the reference's
- * smooth_histogram, histogram.cpp:75-132, jumps at bin centres and has no
- * derivative there) and V = -scale * (g circular-convolved with the
+ * The intensity map of the fused engine -- profile, induced voltage, induced
+ * kick, RF kick (kick.cpp:16-35), "simple" drift (drift.cpp:34-37) -- with
+ * the profile deposited and the induced voltage interpolated by one
+ * quadratic B-spline on bin centres (continuous first derivative; the
+ * reference's smooth_histogram, histogram.cpp:75-132, jumps at bin centres,
+ * and linear_interp_kick.cpp:21-71's slope jumps there) and V = -scale * (g circular-convolved with the
  * profile), g = irfft(Z, n_fft) (impedance.py:457-489).  forward() tracks
  * `turns` turns from the particles given to set_particles (copied), keeping
  * (dt, dE) every `checkpoint_every` turns; backward() takes dL/d(final
@@ -248,8 +258,7 @@ int blmi_adjoint_set_programs(void *adjoint, const double *rf_host, const double
                               const double *eta2_host, const double *alpha0_host,
                               const double *alpha1_host, const double *alpha2_host,
                               const double *beta_host, const double *energy_host, double charge);
-int blmi_adjoint_set_slices(void *adjoint, double cut_left, double cut_right,
-                            const double *bin_centers_host);
+int blmi_adjoint_set_slices(void *adjoint, double cut_left, double cut_right);
 int blmi_adjoint_set_wake(void *adjoint, int64_t n_fft, const double *g_host, double scale);
 int blmi_adjoint_forward(void *adjoint, int turns, double *profile_out, void *stream);
 int blmi_adjoint_backward(void *adjoint, const double *dprofile, double *grad_rf_host,

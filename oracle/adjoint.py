@@ -7,14 +7,18 @@ The map is the one the GPU differentiable tracker runs (one turn of the
 reference's intensity map, tracker.py:521-615 with impedance.py:113-146 and
 impedance.py:457-489), with the profile deposited continuously:
 
-  p_t      = D(dt_t): cloud-in-cell on bin centres; (dt - cut_left)*inv_bw
-             - 1/2 = k + d puts 1-d in bin k and d in bin k+1 (bins outside
-             [0, n_slices) dropped).  The reference's smooth_histogram
-             (histogram.cpp:75-132) instead moves the whole weight from bin
-             k-1 to bin k+1 when a particle crosses the centre of bin k, so
-             it has no derivative there.
+  p_t      = D(dt_t): quadratic B-spline on bin centres; with
+             u = (dt - cut_left)*inv_bw - 1/2 = k + d, k = round(u), it puts
+             (1/2-d)^2/2, 3/4-d^2, (1/2+d)^2/2 in bins k-1, k, k+1 (bins
+             outside [0, n_slices) dropped).  The reference's
+             smooth_histogram (histogram.cpp:75-132) instead moves the whole
+             weight from bin k-1 to bin k+1 when a particle crosses the centre
+             of bin k, so it has no derivative there.
   V_t      = -scale * (g (*) p_t)[:ns], g = irfft(Z, n_fft), circular
-  e1       = dE + (dt*slope_k + intercept_k)      linear_interp_kick.cpp:33-64
+  e1       = dE + q * sum_b V_t[b] w_b(dt)      the induced kick, with the same
+             spline weights as the deposit (linear_interp_kick.cpp:21-71
+             interpolates linearly; its slope jumps at every bin centre,
+             which leaves first-order errors in finite differences)
   e2       = e1 + (q * sum_j V_j sin(w_j dt + phi_j) + acc)   kick.cpp:16-35
   dt'      = dt + coeff * e2,  dE' = e2           drift.cpp:34-37 ("simple")
 
@@ -65,7 +69,7 @@ class DiffMap:
     state: dict with per-turn programs -- omega [n_rf, T+1], coeff [T+1]
     (drift "simple" coefficient t_rev*length_ratio*eta_0/(beta^2 E) of each
     turn), acc [T+1] -- and charge, cut_left, cut_right, n_slices,
-    bin_centers, n_fft, bin_size, scale (= charge * e * ratio)."""
+    n_fft, bin_size, scale (= charge * e * ratio)."""
 
     def __init__(self, st: dict):
         self.st = st
@@ -74,9 +78,6 @@ class DiffMap:
         self.cl = float(st["cut_left"])
         self.ibw_cut = self.ns / (float(st["cut_right"]) - self.cl)
         bc = np.asarray(st["bin_centers"], dtype=np.float64)
-        self.bc = bc
-        self.bc0 = bc[0]
-        self.ibw_bc = (self.ns - 1) / (bc[-1] - bc[0])
         i = np.arange(self.ns)
         self.circ = (i[:, None] - i[None, :]) % self.N  # g index of V[i] <- p[j]
         self.q = float(st["charge"])
@@ -84,18 +85,24 @@ class DiffMap:
         self.freq = np.fft.rfftfreq(self.N, float(st["bin_size"]))
 
     # -- pieces ------------------------------------------------------------
-    def cic(self, x):
-        xf = (x - self.cl) * self.ibw_cut - 0.5
-        f = np.floor(xf)
-        return f.astype(np.int64), xf - f
+    def spline(self, x):
+        """Bins k-1, k, k+1 with weights and u-derivatives (each [3, n])."""
+        u = (x - self.cl) * self.ibw_cut - 0.5
+        k = np.floor(u + 0.5)
+        d = u - k
+        k = np.clip(k, -10, self.ns + 10).astype(np.int64)
+        l, r = 0.5 - d, 0.5 + d
+        bins = np.stack([k - 1, k, k + 1])
+        w = np.stack([0.5 * l * l, 0.75 - d * d, 0.5 * r * r])
+        dw = np.stack([-l, -2.0 * d, r])
+        return bins, w, dw
 
     def deposit(self, x):
-        k, d = self.cic(x)
+        bins, w, _ = self.spline(x)
         p = np.zeros(self.ns)
-        m0 = (k >= 0) & (k < self.ns)
-        np.add.at(p, k[m0], 1.0 - d[m0])
-        m1 = (k + 1 >= 0) & (k + 1 < self.ns)
-        np.add.at(p, k[m1] + 1, d[m1])
+        for q in range(3):
+            m = (bins[q] >= 0) & (bins[q] < self.ns)
+            np.add.at(p, bins[q][m], w[q][m])
         return p
 
     def wake(self, R, fr, Q):
@@ -105,15 +112,18 @@ class DiffMap:
     def induced(self, p, g):
         return -(self.scale * (g[self.circ] @ p))
 
-    def table(self, V):
-        sl = self.q * (V[1:] - V[:-1]) * self.ibw_bc
-        ic = (self.q * V[:-1] - self.bc[:-1] * sl) + 0.0
-        return sl, ic
-
-    def interp_bin(self, x):
-        f = np.floor((x - self.bc0) * self.ibw_bc)
-        inn = (f >= 0) & (f < self.ns - 1)
-        return np.where(inn, f, 0).astype(np.int64), inn
+    def kick(self, x, Vi):
+        """q * sum_b V[b] w_b(x) and its x-derivative, with the deposit's
+        spline on bin centres (bins outside [0, n_slices) have V = 0)."""
+        bins, w, dw = self.spline(x)
+        out = np.zeros_like(x)
+        dout = np.zeros_like(x)
+        for q in range(3):
+            m = (bins[q] >= 0) & (bins[q] < self.ns)
+            v = np.where(m, Vi[np.clip(bins[q], 0, self.ns - 1)], 0.0)
+            out += v * w[q]
+            dout += v * dw[q]
+        return self.q * out, self.q * dout * self.ibw_cut, bins, w
 
     # -- forward -----------------------------------------------------------
     def forward(self, dt, dE, V, phi, R, fr, Q, turns, keep=False):
@@ -125,11 +135,9 @@ class DiffMap:
         hist = []
         for t in range(turns):
             Vi = self.induced(p, g)
-            sl, ic = self.table(Vi)
             if keep:
-                hist.append((x.copy(), e.copy(), p.copy(), sl, ic))
-            k, inn = self.interp_bin(x)
-            e1 = np.where(inn, e + (x * sl[k] + ic[k]), e)
+                hist.append((x.copy(), e.copy(), p.copy(), Vi))
+            e1 = e + self.kick(x, Vi)[0]
             w = st["omega"][:, t]
             s = np.zeros_like(x)
             for j in range(len(V)):
@@ -152,9 +160,9 @@ class DiffMap:
         gphi = np.zeros(len(V))
         corr = np.zeros(self.N)
         for t in range(turns - 1, -1, -1):
-            x, e, p, sl, ic = hist[t]
-            k, inn = self.interp_bin(x)
-            e1 = np.where(inn, e + (x * sl[k] + ic[k]), e)
+            x, e, p, Vi = hist[t]
+            kv, dkv, bins0, w0 = self.kick(x, Vi)
+            e1 = e + kv
             w = st["omega"][:, t]
             sn = [np.sin(w[j] * x + phi[j]) for j in range(len(V))]
             cs = [np.cos(w[j] * x + phi[j]) for j in range(len(V))]
@@ -165,29 +173,30 @@ class DiffMap:
             coeff = st["coeff"][t + 1]
             x1 = x + coeff * e2
             # deposit adjoint of p_{t+1}
-            kk, _ = self.cic(x1)
-            l0 = np.where((kk >= 0) & (kk < self.ns), lp[np.clip(kk, 0, self.ns - 1)], 0.0)
-            l1 = np.where((kk + 1 >= 0) & (kk + 1 < self.ns), lp[np.clip(kk + 1, 0, self.ns - 1)], 0.0)
-            lx = lx + (l1 - l0) * self.ibw_cut
+            bins, _, dw = self.spline(x1)
+            s3 = np.zeros_like(x1)
+            for q in range(3):
+                m = (bins[q] >= 0) & (bins[q] < self.ns)
+                s3 += np.where(m, lp[np.clip(bins[q], 0, self.ns - 1)], 0.0) * dw[q]
+            lx = lx + s3 * self.ibw_cut
             le2 = le + coeff * lx
             ds = np.zeros_like(x)
             for j in range(len(V)):
                 ds = ds + V[j] * w[j] * cs[j]
                 gV[j] += np.sum(le2 * self.q * sn[j])
                 gphi[j] += np.sum(le2 * self.q * V[j] * cs[j])
-            lx = lx + le2 * (self.q * ds)
-            lx = lx + np.where(inn, le2 * sl[k], 0.0)
-            wt = (x - self.bc[k]) * self.ibw_bc
+            lx = lx + le2 * (self.q * ds) + le2 * dkv
             lv = np.zeros(self.ns)
-            np.add.at(lv, k[inn], (le2 * self.q * (1.0 - wt))[inn])
-            np.add.at(lv, k[inn] + 1, (le2 * self.q * wt)[inn])
+            for q in range(3):
+                m = (bins0[q] >= 0) & (bins0[q] < self.ns)
+                np.add.at(lv, bins0[q][m], (le2 * self.q * w0[q])[m])
             le = le2
             # induced-voltage adjoint and the impedance correlation
             lp = -(self.scale * (g[self.circ].T @ lv))
             pp = np.zeros(self.N)
             pp[: self.ns] = p
-            for m in range(self.N):
-                corr[m] += np.dot(lv, pp[(np.arange(self.ns) - m) % self.N])
+            for m_ in range(self.N):
+                corr[m_] += np.dot(lv, pp[(np.arange(self.ns) - m_) % self.N])
         return dict(V=gV, phi=gphi, **self.impedance_grad(R, fr, Q, corr), corr=corr, profile=pT)
 
     def impedance_grad(self, R, fr, Q, corr):

@@ -198,9 +198,10 @@ def track(state: dict, n_turns: int, lib: Lib | None = None) -> dict:
     [tracker, profile] without.  Returns a new state dict."""
     lib = lib or oracle_lib()
     st = dict(state)
-    dt = np.array(state["dt"], dtype=np.float64, copy=True)
-    dE = np.array(state["dE"], dtype=np.float64, copy=True)
-    prof = np.array(state["profile"], copy=True) if "profile" in state else None
+    r = lib.real  # float32 with the reference's single-precision build
+    dt = np.array(state["dt"], dtype=r, copy=True)
+    dE = np.array(state["dE"], dtype=r, copy=True)
+    prof = np.array(state["profile"], dtype=r, copy=True) if "profile" in state else None
     volt_ind = np.array(state["induced_voltage"], copy=True) if state.get("intensity") else None
     t0 = int(state["turn"])
     for t in range(t0, t0 + n_turns):

@@ -61,13 +61,16 @@ def test_adjoint_gradient_vs_central_differences(n_slices, turns):
         assert abs(g[k][0] - fd) <= 1e-5 * abs(fd), (k, g[k][0], fd)
 
 
-def test_deposit_is_continuous_and_conserves_weight():
+def test_deposit_is_smooth_and_conserves_weight():
     c = build_adjoint_case(n_particles=10, n_turns=2, n_slices=16)
     m = adj.DiffMap(c.state)
     bw = (c.state["cut_right"] - c.state["cut_left"]) / 16
-    # a particle sweeping across a bin centre: the profile moves continuously
-    xs = c.state["cut_left"] + bw * (5.5 + np.linspace(-1e-6, 1e-6, 3))
-    ps = [m.deposit(np.array([x])) for x in xs]
-    assert np.abs(ps[0] - ps[2]).max() < 1e-5
-    inner = c.state["cut_left"] + bw * np.array([0.7, 3.2, 15.3])
+    # a particle sweeping across a bin centre or edge: the profile and its
+    # derivative move continuously
+    for x0 in (5.5, 6.0):
+        xs = c.state["cut_left"] + bw * (x0 + np.linspace(-1e-6, 1e-6, 3))
+        ps = [m.deposit(np.array([x])) for x in xs]
+        assert np.abs(ps[0] - ps[2]).max() < 3e-6
+        np.testing.assert_allclose(ps[2] - ps[1], ps[1] - ps[0], atol=1e-11)
+    inner = c.state["cut_left"] + bw * np.array([1.7, 3.2, 14.3])
     assert m.deposit(inner).sum() == pytest.approx(3.0, abs=1e-14)

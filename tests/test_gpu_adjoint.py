@@ -1,7 +1,16 @@
 """GPU parity of the differentiable tracker: the forward profile against
 the float64 CPU reference map, the hand-written adjoint against the CPU
 reference gradient and against central finite differences of the GPU
-forward (step 1e-6 relative, 1e-6 relative agreement)."""
+forward.
+
+Finite differences: over 100 turns, central differences with step 1e-6
+relative agree with the adjoint to 1e-6 relative.  Over 500 turns the
+filamented bunch makes the loss vary on a scale of ~1e-7 relative in V: on
+the CPU reference (float64, 1e5 particles) central differences of dL/dV
+converge to the adjoint as the step shrinks -- error 7e-3 at step 1e-6,
+4e-5 at 1e-7, 3e-6 at 3e-8, 4e-7 at 1e-8 -- and the GPU reproduces those
+values digit for digit.  The 500-turn check therefore uses step 1e-8, with
+2e-6 for the deposit's fixed-point rounding (2^-43 per weight)."""
 import numpy as np
 import pytest
 
@@ -60,32 +69,58 @@ def test_adjoint_matches_cpu_reference_gradient():
     dtr.close()
 
 
-def test_adjoint_vs_central_differences_with_checkpoints():
-    """100 turns with checkpoints every 30 (segments of 30, 30, 30, 10)."""
-    c, dtr = setup(100000, 100, 256, 30)
+def _fd_check(n, turns, every, step, tol):
+    c, dtr = setup(n, turns, 256, every)
     th, ts = c.theta0, c.theta_star
     with torch.no_grad():
-        target = dtr.profile_after(*params(ts), turns=100).clone()
+        target = dtr.profile_after(*params(ts), turns=turns).clone()
 
     def loss_at(t):
         with torch.no_grad():
-            p = dtr.profile_after(*params(t), turns=100)
+            p = dtr.profile_after(*params(t), turns=turns)
             return float(torch.sum((p - target) ** 2))
 
     ps = params(th)
-    p = dtr.profile_after(*ps, turns=100)
+    p = dtr.profile_after(*ps, turns=turns)
     loss = torch.sum((p - target) ** 2)
     loss.backward()
     for k, t in zip(KEYS, ps):
         x0 = th[k][0]
-        h = 1e-6 * abs(x0) if x0 != 0 else 1e-6
+        h = step * abs(x0) if x0 != 0 else step
         tp, tm = dict(th), dict(th)
         tp[k] = np.array([x0 + h])
         tm[k] = np.array([x0 - h])
         fd = (loss_at(tp) - loss_at(tm)) / (2 * h)
         got = t.grad.cpu().numpy()[0]
         print(f"\n{k}: adjoint {got:.12e} fd {fd:.12e} rel {abs(got - fd) / abs(fd):.2e}")
-        assert abs(got - fd) <= 1e-6 * abs(fd)
+        assert abs(got - fd) <= tol * abs(fd)
+    dtr.close()
+
+
+def test_adjoint_vs_central_differences_with_checkpoints():
+    """100 turns, checkpoints every 30 (segments of 30, 30, 30, 10), step 1e-6."""
+    _fd_check(100000, 100, 30, 1e-6, 1e-6)
+
+
+def test_adjoint_vs_central_differences_500_turns():
+    """The c5 length: 500 turns, checkpoints every 50, step 1e-8."""
+    _fd_check(100000, 500, 50, 1e-8, 2e-6)
+
+
+def test_adjoint_matches_cpu_reference_500_turns():
+    c, dtr = setup(20000, 500, 256, 50)
+    m = adj.DiffMap(c.state)
+    th, ts = c.theta0, c.theta_star
+    target = m.forward(c.beam.dt, c.beam.dE, ts["V"], ts["phi"], ts["R"], ts["fr"], ts["Q"], 500)
+    ps = params(th)
+    p = dtr.profile_after(*ps, turns=500)
+    torch.sum((p - torch.from_numpy(target).cuda()) ** 2).backward()
+    ref = m.gradient(c.beam.dt, c.beam.dE, th["V"], th["phi"], th["R"], th["fr"], th["Q"], 500,
+                     2 * (p.detach().cpu().numpy() - target))
+    for k, t in zip(KEYS, ps):
+        got, want = t.grad.cpu().numpy()[0], ref[k][0]
+        print(f"\n{k}: gpu {got:.12e} cpu {want:.12e} rel {abs(got - want) / abs(want):.2e}")
+        assert abs(got - want) <= 1e-6 * abs(want)
     dtr.close()
 
 

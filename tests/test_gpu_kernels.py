@@ -224,3 +224,55 @@ def test_induced_voltage_fourstep_full_length_output():
     ref = -(2.5 * np.fft.irfft(Z * np.fft.rfft(prof, n_fft), n_fft))
     got = host(bm.induced_voltage_1turn(dev(prof), dev(Z), n_fft, 2.5, n_out=n_fft))
     assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+# ---------------------------------------------------------------------------
+# FP32 mode (the reference's libblond_single.so: real_t = float, fast_sinf)
+# ---------------------------------------------------------------------------
+REF32 = orc.ref_lib("strict", "single")
+needs_ref32 = pytest.mark.skipif(REF32 is None, reason="oracle/_ref single build not present")
+
+
+@pytest.fixture
+def single():
+    bm.use_precision("single")
+    yield
+    bm.use_precision("double")
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+@needs_ref32
+@pytest.mark.parametrize("n_rf", [1, 2])
+def test_f32_kick_drift_bit_exact_vs_reference_single(single, n_rf):
+    dt, dE = (f32(a) for a in bunch(100001))
+    V, w, phi = f32([6e6, 1.5e6][:n_rf]), f32([2.5182e9, 1.00728e10][:n_rf]), f32([0.0, 0.3][:n_rf])
+    rdt, rdE = dt.copy(), dE.copy()
+    REF32.kick(rdt, rdE, V, w, phi, 1.0, n_rf, -12.5)
+    REF32.drift(rdt, rdE, "simple", 8.892465516509709e-05, 1.0, 0.0, 3.17e-4, 0.0, 0.0, 3.2e-4,
+                0.0, 0.0, 0.99999782, 450e9)
+    gdt, gdE = dev(dt), dev(dE)
+    bm.kick(gdt, gdE, dev(V), dev(w), dev(phi), 1.0, n_rf, -12.5)
+    bm.drift(gdt, gdE, "simple", 8.892465516509709e-05, 1.0, 0.0, 3.17e-4, 0.0, 0.0, 3.2e-4,
+             0.0, 0.0, 0.99999782, 450e9)
+    np.testing.assert_array_equal(host(gdE), rdE)
+    np.testing.assert_array_equal(host(gdt), rdt)
+
+
+@needs_ref32
+def test_f32_histogram_and_interp_kick_vs_reference_single(single):
+    dt, dE = (f32(a) for a in bunch(200001))
+    ref = np.zeros(1000, np.float32)
+    REF32.slice_beam(dt, ref, 0.0, 2.5e-9)
+    got = dev(np.zeros(1000, np.float32))
+    bm.slice_beam(dev(dt), got, 0.0, 2.5e-9)
+    np.testing.assert_array_equal(host(got), ref)
+    bc = f32((np.arange(1000) + 0.5) * 2.5e-12)
+    V = f32(np.sin(np.arange(1000) / 50.0) * 1e4)
+    rdE = dE.copy()
+    REF32.linear_interp_kick(dt, rdE, V, bc, 1.0, 0.0)
+    gdE = dev(dE)
+    bm.linear_interp_kick(dev(dt), gdE, dev(V), dev(bc), 1.0, 0.0)
+    np.testing.assert_allclose(host(gdE), rdE, rtol=0, atol=2e-7 * np.abs(rdE).max())

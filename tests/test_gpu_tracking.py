@@ -154,3 +154,53 @@ def test_intensity_first_turn_exact_profile_and_voltage():
     assert np.max(np.abs(v - ref["induced_voltage"])) <= 1e-10 * np.abs(ref["induced_voltage"]).max()
     sc = np.abs(ref["dE"]).max()
     np.testing.assert_allclose(host(w.beam.dE), ref["dE"], rtol=0, atol=1e-12 * sc)
+
+
+# ---------------------------------------------------------------------------
+# FP32 mode of the fused engine against the reference's single build
+# ---------------------------------------------------------------------------
+REF32 = orc.ref_lib("strict", "single")
+needs_ref32 = pytest.mark.skipif(REF32 is None, reason="oracle/_ref single build not present")
+
+
+def _to_f32(w):
+    w.beam.dt = w.beam.dt.to(torch.float32).contiguous()
+    w.beam.dE = w.beam.dE.to(torch.float32).contiguous()
+
+
+@needs_ref32
+@pytest.mark.parametrize("n_slices", [100, 65536])
+def test_plain_turns_fused_f32_bit_exact(n_slices):
+    """Float particles, float programs and cuts: bit-identical to the
+    reference's single-precision kernels (u32 and u16-pair histograms)."""
+    w = build_workload("c1", n_particles=100001, n_turns=20, n_slices=n_slices)
+    _to_f32(w)
+    ref = orc.track(w.snapshot(), 12, REF32)
+    ft = w.fused(12)
+    assert ft.f32
+    np.testing.assert_array_equal(host(w.beam.dt), ref["dt"])
+    np.testing.assert_array_equal(host(w.beam.dE), ref["dE"])
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
+    assert w.profile.n_macroparticles.dtype == torch.float32
+
+
+@needs_ref32
+def test_intensity_turns_fused_f32_large_shape():
+    """2 RF systems, 65536 slices, FFT 131072, float particles.  The engine
+    forms the induced voltage in FP64 (the reference's single mode in
+    float32), so the comparison is at float resolution: the first turn's
+    profile exactly, moments after 8 turns to 1e-5."""
+    w = build_workload("c3", n_particles=200001, n_turns=12)
+    _to_f32(w)
+    snap = w.snapshot()
+    ft = w.fused(1)
+    ref1 = orc.track(snap, 1, REF32)
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref1["profile"])
+    ft.track(7)
+    ref = orc.track(snap, 8, REF32)
+    for a, b in ((host(w.beam.dt), ref["dt"]), (host(w.beam.dE), ref["dE"])):
+        a, b = a.astype(np.float64), b.astype(np.float64)
+        assert abs(a.mean() - b.mean()) <= 1e-5 * max(abs(b.mean()), b.std())
+        assert abs(a.std() - b.std()) <= 1e-5 * b.std()
+    v, vr = host(w.tiv.induced_voltage), ref["induced_voltage"]
+    assert np.max(np.abs(v - vr)) <= 1e-4 * np.abs(vr).max()
