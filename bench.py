@@ -176,6 +176,20 @@ def form_adjoint(torch, turns=500) -> dict:
             "whole_frac": bytes_ / el / 1e9 / HBM_PEAK_GBS}
 
 
+def measured_traffic(workload: str, n: int):
+    """HBM bytes per launch of the particle kernel from the committed
+    FETCH_SIZE / WRITE_SIZE profile of this workload (profiles/traffic.json,
+    each counter scaled by a 1 GiB copy's reading), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+    if not t or t.get("macroparticles") != n:
+        return None
+    return t["traffic_bytes"]
+
+
 def cpu_baseline(snapshot: dict, target_s: float) -> dict:
     """Time the reference's C++ kernels (or, without oracle/_ref, the oracle's
     C restatement) on a bounded sample of the same workload."""
@@ -262,7 +276,9 @@ def main():
                    "block": a.block, "blocks_per_cu": a.blocks_per_cu, "groups": a.groups,
                    "max_peel": a.max_peel},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(a.workload, n),
+                     "algorithmic_bytes": BYTES_PER_PARTICLE_TURN * n,
                      "kernel": "turn_kernel (interp kick + RF kick + drift + histogram)",
                      "kernel_ms": ms_turn, "induced_ms": ms_ind,
                      "induced_method": ft.method,

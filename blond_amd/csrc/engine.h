@@ -15,6 +15,9 @@ namespace blmi {
 // 65535 particles, so no counter can carry into its neighbour).
 constexpr int kLdsHistBytes = 131072;
 constexpr int kMaxParticlesPerGroup16 = 65535;
+// LDS the particle kernel may use for its histogram plus a staged copy of the
+// interpolation table (random 16-byte gathers served by LDS instead of L2)
+constexpr int kLdsTableBudget = 144 * 1024;
 // Largest slice count for which the induced voltage uses the direct
 // (Toeplitz) convolution: ns^2 FP64 FMAs spread over all CUs.
 constexpr int kDirectMaxSlices = 8192;

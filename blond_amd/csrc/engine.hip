@@ -177,9 +177,13 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
         bc0 = (T)a.tab_params[0];
         ibw = (T)a.tab_params[1];
     }
-    if (HIST && LDSH) {
-        const int words = hist_words(LDSH, a.n_slices);
-        for (int b = threadIdx.x; b < words; b += blockDim.x) lds_hist[b] = 0u;
+    // INTERP == 2: the table is staged in LDS after the histogram words
+    const int hwords = (HIST && LDSH) ? hist_words(LDSH, a.n_slices) : 0;
+    double2 *ltab = reinterpret_cast<double2 *>(lds_hist + ((hwords + 3) & ~3));
+    if ((HIST && LDSH) || INTERP == 2) {
+        for (int b = threadIdx.x; b < hwords; b += blockDim.x) lds_hist[b] = 0u;
+        if (INTERP == 2)
+            for (int k = threadIdx.x; k < a.n_tab; k += blockDim.x) ltab[k] = a.tab[k];
         __syncthreads();
     }
     auto step = [&](T &x, T &e) -> int {
@@ -188,7 +192,7 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
             const T f = floor((x - bc0) * ibw);
             if (f >= T(0) && f < (T)a.n_tab) {
                 const int k = (int)f;
-                const double2 tk = a.tab[k];
+                const double2 tk = INTERP == 2 ? ltab[k] : a.tab[k];
                 e = e + (x * (T)tk.x + (T)tk.y);
             }
         }
@@ -701,7 +705,11 @@ using TurnFn = void (*)(T *, T *, int64_t, TurnArgs<T>);
 template <typename T, int S>
 static TurnFn<T> pick_turn(int interp, bool hist, int ldsh) {
     if (!hist)
-        return interp ? turn_kernel<T, S, 1, false, 0, false> : turn_kernel<T, S, 0, false, 0, false>;
+        return interp == 2   ? turn_kernel<T, S, 2, false, 0, false>
+               : interp == 1 ? turn_kernel<T, S, 1, false, 0, false>
+                             : turn_kernel<T, S, 0, false, 0, false>;
+    if (interp == 2)
+        return ldsh == 1 ? turn_kernel<T, S, 2, true, 1, false> : turn_kernel<T, S, 2, true, 0, false>;
     switch (interp * 3 + ldsh) {
         case 0: return turn_kernel<T, S, 0, true, 0, true>;
         case 1: return turn_kernel<T, S, 0, true, 1, true>;
@@ -747,7 +755,14 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
         ldsh = 0;
         lds = 0;
     }
-    const int imode = interp ? 1 : 0;
+    int imode = interp ? 1 : 0;
+    // stage the interpolation table in LDS when it fits beside a u32 histogram
+    const size_t tab_bytes = sizeof(double2) * (size_t)(e.n_slices > 1 ? e.n_slices - 1 : 0);
+    const size_t hist_bytes = ((lds + 15) / 16) * 16;
+    if (interp && ldsh != 2 && hist_bytes + tab_bytes <= (size_t)kLdsTableBudget) {
+        imode = 2;
+        lds = hist_bytes + tab_bytes;
+    }
     TurnFn<T> fn = e.solver == 0   ? pick_turn<T, 0>(imode, hist, ldsh)
                    : e.solver == 1 ? pick_turn<T, 1>(imode, hist, ldsh)
                                    : pick_turn<T, 2>(imode, hist, ldsh);

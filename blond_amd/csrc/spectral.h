@@ -90,6 +90,7 @@ int launch_fast_resonator(double *re, double *im, const double *freq, const doub
 // M-complex scratch arrays (one plan per user: launches on one stream).
 struct Fft2Plan {
     int M = 0, M1 = 0, M2 = 0, n_real = 0;
+    int cols = 1;                 // columns per workgroup in passes A and C
     int threads_b = 0;            // pass B workgroup size (passes A, C: 256)
     int n1 = 0, n2 = 0;           // stage counts of the M1- and M2-point transforms
     int r1[kMaxStages], r2[kMaxStages];

@@ -389,7 +389,7 @@ __global__ void induced_lds_batch_kernel(const InducedArgs *__restrict__ args, i
 // LDS: (2 ns + kConvC) doubles.
 // ---------------------------------------------------------------------------
 constexpr int kConvC = 16;
-constexpr int kConvThreads = 1024;
+constexpr int kConvThreads = 512;
 
 __global__ void __launch_bounds__(kConvThreads) induced_direct_kernel(InducedArgs a) {
     extern __shared__ double conv_lds[];
@@ -719,10 +719,10 @@ int launch_fast_resonator(double *re, double *im, const double *freq, const doub
 //      then the inverse M2-point transform and the conjugate twiddle -> S
 //   C: per column, inverse M1-point transform of S -> u[M2 m1 + m2], and
 //      V[2m], V[2m+1] = -(scale * (u / M))   (the merge carries the 1/2)
-// Columns are processed kColBatch at a time so that each row access is a
+// Columns are processed `cols` at a time so that each row access is a
 // contiguous run (coalesced); LDS rows are padded by one element.
 // ---------------------------------------------------------------------------
-constexpr int kColBatch = 8;
+constexpr int kMaxCols = 8;
 constexpr int kColThreads = 256;
 
 __device__ __forceinline__ void count_buffers(const InducedArgs &a, const unsigned *&cin,
@@ -737,20 +737,20 @@ __device__ __forceinline__ void count_buffers(const InducedArgs &a, const unsign
 }
 
 __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int M1, int M2,
-                                                             StageList st1,
+                                                             int cols, StageList st1,
                                                              const double2 *__restrict__ twM,
                                                              double2 *__restrict__ T) {
     extern __shared__ double2 buf[];
     const int ld = M1 + 1;
-    double2 *twl = buf + kColBatch * ld;  // exp(-2 pi i j / M1), j < M1
+    double2 *twl = buf + cols * ld;  // exp(-2 pi i j / M1), j < M1
     const int ns = a.n_slices;
-    const int c0 = blockIdx.x * kColBatch;
+    const int c0 = blockIdx.x * cols;
     const unsigned *cin;
     unsigned *cclr;
     count_buffers(a, cin, cclr);
     for (int j = threadIdx.x; j < M1; j += blockDim.x) twl[j] = twM[(size_t)j * M2];
-    for (int j = threadIdx.x; j < kColBatch * M1; j += blockDim.x) {
-        const int c = j % kColBatch, m1 = j / kColBatch;
+    for (int j = threadIdx.x; j < cols * M1; j += blockDim.x) {
+        const int c = j % cols, m1 = j / cols;
         const int m = M2 * m1 + c0 + c;
         const int j0 = 2 * m, j1 = j0 + 1;
         double p[2] = {0.0, 0.0};
@@ -771,9 +771,9 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
         buf[c * ld + m1] = make_double2(p[0], p[1]);
     }
     __syncthreads();
-    fft_lds_batch<false>(buf, M1, kColBatch, ld, st1, twl, M1);
-    for (int j = threadIdx.x; j < kColBatch * M1; j += blockDim.x) {
-        const int c = j % kColBatch, k1 = j / kColBatch;
+    fft_lds_batch<false>(buf, M1, cols, ld, st1, twl, M1);
+    for (int j = threadIdx.x; j < cols * M1; j += blockDim.x) {
+        const int c = j % cols, k1 = j / cols;
         const int m2 = c0 + c;
         T[(size_t)k1 * M2 + m2] = cmul(buf[c * ld + k1], twM[(size_t)m2 * k1]);
     }
@@ -867,16 +867,16 @@ __global__ void __launch_bounds__(kPassBThreads) fft2_b_kernel(InducedArgs a, in
 // Pass C also transforms the column after its batch (column 0, one row down,
 // for the last batch), so every V[2m+2] a table entry needs is at hand.
 __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int M1, int M2,
-                                                             StageList st1,
+                                                             int cols, StageList st1,
                                                              const double2 *__restrict__ twM,
                                                              const double2 *__restrict__ S) {
     extern __shared__ double2 buf[];
-    constexpr int NC = kColBatch + 1;
+    const int NC = cols + 1;
     const int ld = M1 + 1;
     double2 *twl = buf + NC * ld;
     const int M = M1 * M2;
-    const int c0 = blockIdx.x * kColBatch;
-    const bool last = c0 + kColBatch == M2;
+    const int c0 = blockIdx.x * cols;
+    const bool last = c0 + cols == M2;
     for (int j = threadIdx.x; j < M1; j += blockDim.x) twl[j] = twM[(size_t)j * M2];
     for (int j = threadIdx.x; j < NC * M1; j += blockDim.x) {
         const int c = j % NC, k1 = j / NC;
@@ -897,8 +897,8 @@ __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int 
             a.tab_params[1] = inv_bw;
         }
     }
-    for (int j = threadIdx.x; j < kColBatch * M1; j += blockDim.x) {
-        const int c = j % kColBatch, m1 = j / kColBatch;
+    for (int j = threadIdx.x; j < cols * M1; j += blockDim.x) {
+        const int c = j % cols, m1 = j / cols;
         const int m = M2 * m1 + c0 + c;
         const double2 u = buf[c * ld + m1];
         const double v0 = -(a.scale * (u.x * fct)), v1 = -(a.scale * (u.y * fct));
@@ -909,8 +909,8 @@ __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int 
             a.tab[2 * m] = make_double2(k0, (a.charge * v0 - a.bc[2 * m] * k0) + 0.0);
             if (2 * m + 1 < ns - 1) {
                 // V[2m+2]: next column, same row (column 0, next row after the last)
-                const double2 un = c + 1 < kColBatch ? buf[(c + 1) * ld + m1]
-                                   : buf[kColBatch * ld + (last ? m1 + 1 : m1)];
+                const double2 un = c + 1 < cols ? buf[(c + 1) * ld + m1]
+                                   : buf[cols * ld + (last ? m1 + 1 : m1)];
                 const double v2 = -(a.scale * (un.x * fct));
                 const double k1 = a.charge * (v2 - v1) * inv_bw;
                 a.tab[2 * m + 1] = make_double2(k1, (a.charge * v1 - a.bc[2 * m + 1] * k1) + 0.0);
@@ -942,27 +942,37 @@ int fft2_plan_create(int64_t n_real, Fft2Plan *plan) {
     if (n_real > (int64_t)1 << 30)
         return blmi_set_error(BLMI_ERR_UNSUPPORTED, "FFT length %lld too large", (long long)n_real);
     const int M = (int)(n_real / 2);
-    // M2: the largest divisor <= 1024 that is a multiple of kColBatch with an
-    // M1-point column batch fitting the pass A/C workgroup
-    int M1 = 0, M2 = 0;
-    for (int d = 1024; d >= kColBatch; d -= kColBatch) {
+    // Split M = M1 * M2 so that every pass has work for many workgroups
+    // and little per workgroup (the passes are latency-bound): M2 a divisor
+    // in [16, 1024] as close to sqrt(M) as possible, pass A/C batches of
+    // `cols` columns with about 512 elements per workgroup.
+    int M1 = 0, M2 = 0, cols = 1;
+    double best = 1e300;
+    for (int d = 16; d <= 1024 && d <= M / 2; ++d) {
         if (M % d) continue;
         const int m1 = M / d;
-        if (m1 < 2 || (kColBatch + 1) * m1 > 15 * kColThreads) continue;
+        int c = 1;
+        while (c < kMaxCols && 2 * c * m1 <= 512 && d % (2 * c) == 0) c *= 2;
+        if ((c + 1) * m1 > 15 * kColThreads) continue;  // pass C's batch
         int r[kMaxStages], n1, n2;
         if (!factor_list(m1, r, n1) || !factor_list(d, r, n2)) continue;
-        M1 = m1;
-        M2 = d;
-        break;
+        const double score = fabs(log((double)d) - 0.5 * log((double)M));
+        if (score < best) {
+            best = score;
+            M1 = m1;
+            M2 = d;
+            cols = c;
+        }
     }
     if (!M1)
         return blmi_set_error(BLMI_ERR_UNSUPPORTED,
-                              "FFT length %lld has no four-step split (needs 2*2^a*3^b*5^c with a "
-                              "factor 8)", (long long)n_real);
+                              "FFT length %lld has no four-step split (needs 2*2^a*3^b*5^c)",
+                              (long long)n_real);
     Fft2Plan p;
     p.M = M;
     p.M1 = M1;
     p.M2 = M2;
+    p.cols = cols;
     p.n_real = 2 * M;
     factor_list(M1, p.r1, p.n1);
     factor_list(M2, p.r2, p.n2);
@@ -1010,22 +1020,22 @@ int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t
                               a.n_out, p.n_real);
     if (a.counts0 && a.groups > kMaxCountGroups)
         return blmi_set_error(BLMI_ERR_ARG, "too many count groups");
-    const size_t lds_a = sizeof(double2) * (kColBatch * (size_t)(p.M1 + 1) + p.M1);
-    const size_t lds_c = sizeof(double2) * ((kColBatch + 1) * (size_t)(p.M1 + 1) + p.M1);
+    const size_t lds_a = sizeof(double2) * (p.cols * (size_t)(p.M1 + 1) + p.M1);
+    const size_t lds_c = sizeof(double2) * ((p.cols + 1) * (size_t)(p.M1 + 1) + p.M1);
     const size_t lds_b = sizeof(double2) * 3 * (size_t)p.M2;
     int rc = set_lds_limit((const void *)fft2_a_kernel, lds_a);
     if (!rc) rc = set_lds_limit((const void *)fft2_c_kernel, lds_c);
     if (!rc) rc = set_lds_limit((const void *)fft2_b_kernel, lds_b);
     if (rc) return rc;
     const StageList st1 = stage_list(p.r1, p.n1), st2 = stage_list(p.r2, p.n2);
-    const int cols = p.M2 / kColBatch;
+    const int cols = p.M2 / p.cols;
     const KernelEvents first{ev.start, nullptr}, last{nullptr, ev.stop};
-    launch(fft2_a_kernel, dim3(cols), dim3(kColThreads), lds_a, s, first, a, p.M1, p.M2, st1,
-           p.twM, p.T);
+    launch(fft2_a_kernel, dim3(cols), dim3(kColThreads), lds_a, s, first, a, p.M1, p.M2, p.cols,
+           st1, p.twM, p.T);
     hipLaunchKernelGGL(fft2_b_kernel, dim3(p.M1 / 2 + 1), dim3(p.threads_b), lds_b, s, a, p.M1,
                        p.M2, st2, p.twM, p.twN, p.T, p.S);
-    launch(fft2_c_kernel, dim3(cols), dim3(kColThreads), lds_c, s, last, a, p.M1, p.M2, st1,
-           p.twM, p.S);
+    launch(fft2_c_kernel, dim3(cols), dim3(kColThreads), lds_c, s, last, a, p.M1, p.M2, p.cols,
+           st1, p.twM, p.S);
     return blmi_check_launch("induced_fourstep");
 }
 
