@@ -56,6 +56,7 @@ SIGNATURES = {
     "blmi_engine_set_method": [_p, _i],
     "blmi_engine_get_method": [_p, ct.POINTER(_i)],
     "blmi_engine_set_hist": [_p, _i, _i],
+    "blmi_engine_set_order": [_p, _i],
     "blmi_engine_set_particles_f32": [_p, _p, _p, _i64],
     "blmi_engine_set_slices_f32": [_p, _i, _d, _d, _p],
     "blmi_engine_set_induced_f32": [_p, _i64, _p, _d, _p, _p],

@@ -478,6 +478,16 @@ int blmi_engine_set_hist(void *h, int groups, int max_peel) {
     return e->set_hist(groups, max_peel);
 }
 
+int blmi_engine_set_order(void *h, int sort_every) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    BLMI_REQUIRE(sort_every >= 0 && sort_every <= 1000, "engine: sort_every in [0, 1000]");
+    e->sort_every = sort_every;
+    e->ord_since = 0;
+    e->invalidate_graph();
+    return BLMI_OK;
+}
+
 int blmi_engine_time_kernels(void *h, int turns, float *ms_induced, float *ms_turn,
                              void *stream) {
     Engine *e = reinterpret_cast<Engine *>(h);

@@ -18,6 +18,11 @@ constexpr int kMaxParticlesPerGroup16 = 65535;
 // LDS the particle kernel may use for its histogram plus a staged copy of the
 // interpolation table (random 16-byte gathers served by LDS instead of L2)
 constexpr int kLdsTableBudget = 144 * 1024;
+// Bin-ordered working set (slice counts above kSortWindow): bins of a
+// workgroup's LDS window (u32 histogram + interpolation table) and the
+// coarse-digit width of the counting sort (2^shift bins).
+constexpr int kSortWindow = 4096;
+constexpr int kSortDigitShift = 6;
 // Largest slice count for which the induced voltage uses the direct
 // (Toeplitz) convolution: ns^2 FP64 FMAs spread over all CUs.
 constexpr int kDirectMaxSlices = 8192;
@@ -69,6 +74,15 @@ struct Engine {
     float cut_left32 = 0.f, inv_bw32 = 0.f;
     float *profile32 = nullptr, *volt32 = nullptr;
     double *d_prof64 = nullptr, *d_volt64 = nullptr, *d_bc64 = nullptr;
+    // bin-ordered working set: engine-owned double-buffered copies of the
+    // particles grouped by coarse bin, re-grouped every sort_every turns
+    int sort_every = 8;            // 0: off
+    double *d_ord = nullptr;       // [2][2][n]: dt, dE per buffer
+    unsigned *d_perm = nullptr;    // [2][n]: caller's index of each slot
+    int *d_win = nullptr;          // [grid]: window base per workgroup
+    unsigned *d_digcnt = nullptr;  // [ndig][grid] column offsets, then [ndig] digit starts
+    int64_t ord_n = 0;
+    int ord_grid = 0, ord_ndig = 0, ord_cur = 0, ord_since = 0;
     // launch shape (tunable)
     int block_size = 1024;
     int blocks_per_cu = 1;
@@ -108,6 +122,16 @@ struct Engine {
     // histogram mode of the particle kernel (0 global atomics, 1 LDS u32,
     // 2 LDS u16 pairs) and its grid and LDS bytes for `block` threads
     int hist_layout(int block, int *grid, size_t *lds) const;
+    bool ordered() const;
+    int order_grid() const;
+    int ensure_order_buffers();
+    double *ord_x(int which) const;
+    double *ord_e(int which) const;
+    unsigned *ord_p(int which) const;
+    int order_in(hipStream_t s);
+    int order_sort(hipStream_t s);
+    int order_out(hipStream_t s);
+    int launch_window_turn(hipStream_t s, int off, bool interp, const KernelEvents &ev);
     int ensure_partials();
     int launch_reduce(hipStream_t s, int off);
     int launch_induced(hipStream_t s, int off, const KernelEvents &ev);

@@ -132,19 +132,6 @@ __global__ void __launch_bounds__(256) partials_reduce_kernel(const unsigned *__
     }
 }
 
-// profile[b] = sum over groups of c[g][b], and the counts cleared, for b in [b0, b1)
-__device__ __forceinline__ void publish_range(unsigned *__restrict__ c, double *__restrict__ profile,
-                                              int n_slices, int groups, int b0, int b1) {
-    for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) {
-        unsigned s = 0;
-        for (int g = 0; g < groups; ++g) {
-            s += c[(size_t)g * n_slices + b];
-            c[(size_t)g * n_slices + b] = 0u;
-        }
-        profile[b] = (double)s;
-    }
-}
-
 template <typename T, int SOLVER, int INTERP, bool HIST, int LDSH, bool PUBLISH>
 __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__restrict__ dE,
                                                     int64_t n, TurnArgs<T> a) {
@@ -332,6 +319,15 @@ void Engine::release() {
     (void)hipFree(d_partials);
     d_partials = nullptr;
     partials_bytes = 0;
+    (void)hipFree(d_ord);
+    (void)hipFree(d_perm);
+    (void)hipFree(d_win);
+    (void)hipFree(d_digcnt);
+    d_ord = nullptr;
+    d_perm = nullptr;
+    d_win = nullptr;
+    d_digcnt = nullptr;
+    ord_n = 0;
     (void)hipFree(d_rf32);
     (void)hipFree(d_acc32);
     (void)hipFree(d_drift32);
@@ -628,7 +624,7 @@ static int lds_attribute(const void *fn, size_t lds) {
 // Partial-histogram rows for the u16-pair mode, sized for the current
 // launch shape (allocated outside any graph capture).
 int Engine::ensure_partials() {
-    if (n_slices <= 0 || !d_counts) return BLMI_OK;
+    if (n_slices <= 0 || !d_counts || ordered()) return BLMI_OK;
     int grid = 0;
     size_t lds = 0;
     if (hist_layout(block_size, &grid, &lds) != 2) return BLMI_OK;
@@ -645,6 +641,7 @@ int Engine::ensure_partials() {
 }
 
 int Engine::launch_reduce(hipStream_t s, int off) {
+    if (ordered()) return BLMI_OK;
     int grid = 0;
     size_t lds = 0;
     if (hist_layout(block_size, &grid, &lds) != 2) return BLMI_OK;
@@ -686,7 +683,12 @@ int Engine::prime(hipStream_t s) {
     if (rc) return rc;
     int grid = 0;
     size_t lds = 0;
-    const int mode = hist_layout(block_size, &grid, &lds);
+    int mode = hist_layout(block_size, &grid, &lds);
+    if (mode == 2 && ordered()) {
+        // no partial rows in the ordered mode: count with device atomics
+        mode = 0;
+        lds = 0;
+    }
     unsigned *c1 = d_counts + (size_t)groups * n_slices;
     const int W = hist_words(2, n_slices);
     if (f32)
@@ -696,7 +698,7 @@ int Engine::prime(hipStream_t s) {
     if (rc) return rc;
     rc = blmi_check_launch("engine_prime");
     if (rc) return rc;
-    return mode == 2 ? launch_reduce(s, 0) : BLMI_OK;
+    return mode == 2 ? launch_reduce(s, 0) : BLMI_OK;  // mode 2 only when not ordered
 }
 
 template <typename T>
@@ -803,6 +805,7 @@ int Engine::launch_induced(hipStream_t s, int off, const KernelEvents &ev) {
 }
 
 int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const KernelEvents &ev) {
+    if (ordered()) return launch_window_turn(s, off, interp, ev);
     if (f32)
         return launch_turn_t<float>(*this, dt32, dE32, d_rf32, d_acc32, d_drift32, cut_left32,
                                     inv_bw32, s, off, interp, ev);
@@ -821,6 +824,10 @@ int Engine::launch_turns(hipStream_t s, int count, bool advance) {
         }
         int rc = launch_turn_kernel(s, k, intensity, none);
         if (!rc) rc = launch_reduce(s, k + 1);
+        if (!rc && ordered() && ++ord_since == sort_every) {
+            ord_since = 0;
+            rc = order_sort(s);
+        }
         if (rc) return rc;
     }
     if (advance) return finish_chunk(s, count);
@@ -858,8 +865,18 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
                               turns, host_turn, n_turns);
     int prc = ensure_partials();
     if (prc) return prc;
+    const bool ord = ordered();
+    if (ord) {
+        prc = ensure_order_buffers();
+        if (prc) return prc;
+    }
     BLMI_HIP_TRY(hipEventRecord(ev_in, user));
     BLMI_HIP_TRY(hipStreamWaitEvent(own_stream, ev_in, 0));
+    if (ord) {
+        prc = order_in(own_stream);
+        if (prc) return prc;
+        ord_since = 0;
+    }
     std::vector<hipEvent_t> ev(4 * (size_t)turns, nullptr);
     for (auto &x : ev) BLMI_HIP_TRY(hipEventCreate(&x));
     int rc = BLMI_OK;
@@ -868,8 +885,13 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
         if (intensity) rc = launch_induced(own_stream, k, ei);
         if (rc == BLMI_OK) rc = launch_turn_kernel(own_stream, k, intensity, et);
         if (rc == BLMI_OK) rc = launch_reduce(own_stream, k + 1);
+        if (rc == BLMI_OK && ordered() && ++ord_since == sort_every) {
+            ord_since = 0;
+            rc = order_sort(own_stream);
+        }
     }
     if (rc == BLMI_OK) rc = finish_chunk(own_stream, turns);
+    if (rc == BLMI_OK && ord) rc = order_out(own_stream);
     if (rc == BLMI_OK) {
         host_turn += turns;
         BLMI_HIP_TRY(hipStreamSynchronize(own_stream));
@@ -901,16 +923,30 @@ int Engine::track(int turns, hipStream_t user, bool use_graph) {
                               turns, host_turn, n_turns);
     int prc = ensure_partials();
     if (prc) return prc;
+    const bool ord = ordered();
+    if (ord) {
+        prc = ensure_order_buffers();
+        if (prc) return prc;
+    }
+    // a captured chunk holds an even number of sorts, so it starts and ends
+    // on the same ordered buffer
+    int chunk = graph_chunk;
+    if (ord) chunk = chunk < 2 * sort_every ? 2 * sort_every : chunk - chunk % (2 * sort_every);
     // order the engine stream after the caller's stream
     BLMI_HIP_TRY(hipEventRecord(ev_in, user));
     BLMI_HIP_TRY(hipStreamWaitEvent(own_stream, ev_in, 0));
+    if (ord) {
+        prc = order_in(own_stream);
+        if (prc) return prc;
+        ord_since = 0;
+    }
     int done = 0;
     int rc = BLMI_OK;
-    if (use_graph && turns >= graph_chunk) {
-        if (!graph_exec || graph_turns != graph_chunk) {
+    if (use_graph && turns >= chunk) {
+        if (!graph_exec || graph_turns != chunk) {
             invalidate_graph();
             BLMI_HIP_TRY(hipStreamBeginCapture(own_stream, hipStreamCaptureModeThreadLocal));
-            rc = launch_turns(own_stream, graph_chunk, true);
+            rc = launch_turns(own_stream, chunk, true);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(own_stream, &g);
             if (rc) {
@@ -921,7 +957,7 @@ int Engine::track(int turns, hipStream_t user, bool use_graph) {
                 return blmi_set_error(BLMI_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
             graph = g;
             BLMI_HIP_TRY(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-            graph_turns = graph_chunk;
+            graph_turns = chunk;
         }
         while (turns - done >= graph_turns) {
             BLMI_HIP_TRY(hipGraphLaunch(graph_exec, own_stream));
@@ -929,6 +965,7 @@ int Engine::track(int turns, hipStream_t user, bool use_graph) {
         }
     }
     if (done < turns) rc = launch_turns(own_stream, turns - done, true);
+    if (!rc && ord) rc = order_out(own_stream);
     if (rc) return rc;
     host_turn += turns;
     BLMI_HIP_TRY(hipEventRecord(ev_out, own_stream));

@@ -94,4 +94,17 @@ __device__ __forceinline__ void hist_flush(const unsigned *__restrict__ h,
     }
 }
 
+// profile[b] = sum over groups of c[g][b], and the counts cleared, for b in [b0, b1)
+__device__ __forceinline__ void publish_range(unsigned *__restrict__ c, double *__restrict__ profile,
+                                              int n_slices, int groups, int b0, int b1) {
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) {
+        unsigned s = 0;
+        for (int g = 0; g < groups; ++g) {
+            s += c[(size_t)g * n_slices + b];
+            c[(size_t)g * n_slices + b] = 0u;
+        }
+        profile[b] = (double)s;
+    }
+}
+
 }  // namespace blmi

@@ -94,7 +94,7 @@ def total_impedance(tiv) -> tuple[np.ndarray, int]:
 class FusedTracker:
     def __init__(self, tracker, profile=None, total_induced_voltage=None, use_graph=True,
                  block=1024, blocks_per_cu=1, graph_chunk=50, groups=0, max_peel=0,
-                 method=None):
+                 method=None, sort_every=8):
         import torch
 
         if tracker.interpolation:
@@ -144,6 +144,7 @@ class FusedTracker:
         _lib.call("blmi_engine_set_launch", self._h, int(block), int(blocks_per_cu),
                   int(graph_chunk))
         _lib.call("blmi_engine_set_hist", self._h, int(groups), int(max_peel))
+        _lib.call("blmi_engine_set_order", self._h, int(sort_every))
         if method is not None and self.tiv is not None:
             _lib.call("blmi_engine_set_method", self._h, _METHODS[method])
         _lib.call("blmi_engine_set_turn", self._h, int(rp.counter[0]))

@@ -175,6 +175,13 @@ int blmi_engine_get_method(void *engine, int *method);
  * steps (0..64) before plain LDS atomics.  Changing `groups` drops the staged
  * counts of the current turn (prime again). */
 int blmi_engine_set_hist(void *engine, int groups, int max_peel);
+/* Bin-ordered working set for slice counts above 4096 (FP64): the engine
+ * tracks copies of the particles grouped by coarse bin (a counting sort
+ * every `sort_every` turns, 0 = off, default 8) so that each workgroup's
+ * histogram and interpolation table fit a 4096-bin LDS window; the caller's
+ * arrays are copied in and written back in their own order by every
+ * track().  Results are bit-identical to the unordered engine. */
+int blmi_engine_set_order(void *engine, int sort_every);
 int blmi_engine_set_turn(void *engine, int turn);   /* synchronous */
 int blmi_engine_get_turn(void *engine, int *turn);  /* synchronous */
 int blmi_engine_prime(void *engine, void *stream);
