@@ -53,11 +53,14 @@ struct EnsTurnArgs {
     int max_peel;
 };
 
+// LDS: the segment's histogram (ns u32) and, with INTERP, its bunch's
+// interpolation table ((ns - 1) double2, 16-byte aligned after the counts).
 template <int SOLVER, bool INTERP, bool HIST>
 __global__ void __launch_bounds__(1024) ensemble_turn_kernel(double *__restrict__ dt,
                                                              double *__restrict__ dE,
                                                              EnsTurnArgs a) {
     extern __shared__ unsigned lds_hist[];
+    double2 *ltab = reinterpret_cast<double2 *>(lds_hist + ((a.n_slices + 3) & ~3));
     int t = *a.base + a.off;
     if (t < 0) t = 0;
     if (t > a.last_turn) t = a.last_turn;
@@ -80,8 +83,11 @@ __global__ void __launch_bounds__(1024) ensemble_turn_kernel(double *__restrict_
             ibw = tb[2 * ns + 1];
         }
         const double cl = a.cut_left[b], hbw = a.inv_bw[b];
-        if (HIST) {
-            for (int k = threadIdx.x; k < ns; k += blockDim.x) lds_hist[k] = 0u;
+        if (HIST || INTERP) {
+            for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+                if (HIST) lds_hist[k] = 0u;
+                if (INTERP && k < ns - 1) ltab[k] = tab[k];
+            }
             __syncthreads();
         }
         auto step = [&](double &x, double &e) -> int {
@@ -90,7 +96,7 @@ __global__ void __launch_bounds__(1024) ensemble_turn_kernel(double *__restrict_
                 const double f = floor((x - bc0) * ibw);
                 if (f >= 0.0 && f < (double)(ns - 1)) {
                     const int k = (int)f;
-                    const double2 tk = tab[k];
+                    const double2 tk = ltab[k];
                     e = e + (x * tk.x + tk.y);
                 }
             }
@@ -116,8 +122,8 @@ __global__ void __launch_bounds__(1024) ensemble_turn_kernel(double *__restrict_
                 lds_count<false>(lds_hist, b1, a.max_peel);
             }
         }
+        if (HIST || INTERP) __syncthreads();  // every lane done with this segment's LDS
         if (HIST) {
-            __syncthreads();
             hist_flush<1>(lds_hist, a.counts + (size_t)b * ns, ns);
             __syncthreads();
         }
@@ -335,6 +341,10 @@ int Ensemble::set_slices(int n_slices_, const double *cut_left, const double *cu
 int Ensemble::set_induced(int64_t n_fft_, const double *Z_host, const double *scale, double *volt_,
                           const double *bc_) {
     if (n_slices <= 0) return blmi_set_error(BLMI_ERR_ARG, "ensemble: set_slices first");
+    // the particle kernel holds a segment's counts and table in LDS
+    if (sizeof(unsigned) * ((n_slices + 3) & ~3) + sizeof(double2) * (n_slices - 1) > 150 * 1024)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED,
+                              "ensemble: %d slices with intensity effects exceed LDS", n_slices);
     int rc = fft_plan_get(n_fft_, &plan);
     if (rc) return rc;
     if (n_slices > n_fft_)
@@ -458,7 +468,8 @@ int Ensemble::launch_turn(hipStream_t s, int k, const KernelEvents &ev) {
     EnsFn fn = solver == 0 ? pick_ens<0>(interp, hist)
                : solver == 1 ? pick_ens<1>(interp, hist)
                              : pick_ens<2>(interp, hist);
-    const size_t lds = hist ? sizeof(unsigned) * n_slices : 0;
+    size_t lds = hist ? sizeof(unsigned) * ((n_slices + 3) & ~3) : 0;
+    if (interp) lds = sizeof(unsigned) * ((n_slices + 3) & ~3) + sizeof(double2) * (n_slices - 1);
     int rc = ens_lds_attr((const void *)fn, lds);
     if (rc) return rc;
     launch(fn, dim3(grid), dim3(block), lds, s, ev, dt, dE, turn_args(*this, k));
