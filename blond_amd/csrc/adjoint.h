@@ -13,7 +13,7 @@ namespace blmi {
 // Largest slice count of the differentiable tracker (its induced-voltage
 // kernels hold the profile, the wake and the adjoints in one workgroup).
 constexpr int kAdjMaxSlices = 1024;
-constexpr int kAdjCopies = 8;  // copies of the global deposit / adjoint accumulators
+constexpr int kAdjCopies = 16;  // copies of the global deposit / adjoint accumulators
 
 struct Adjoint {
     int device = 0;

@@ -52,7 +52,7 @@ namespace blmi {
 int smooth_scale_bits(int64_t n);
 
 constexpr int kAdjMaxRf = 4;
-constexpr int kAdjBlock = 256;
+constexpr int kAdjBlock = 1024;
 constexpr int kAdjOneBlock = 1024;
 
 struct AdjArgs {
@@ -351,7 +351,7 @@ int Adjoint::init(int64_t n_, int n_rf_, int n_turns_, int n_slices_, int every_
     n_slices = n_slices_;
     every = every_;
     BLMI_HIP_TRY(hipGetDevice(&device));
-    grid = device_cu_count() * 4;
+    grid = device_cu_count();
     const size_t nt = (size_t)n_turns + 1;
     const size_t pn = 2 * (size_t)n;
     const int n_ckpt = (n_turns + every - 1) / every;
