@@ -597,16 +597,15 @@ int Ensemble::autotune(int turns, hipStream_t user, int *best_block, int *best_w
     const int64_t span = off.back() + cnt.back() + (cnt.back() & 1);
     const size_t pbytes = sizeof(double) * (size_t)span;
     const size_t cbytes = d_counts ? sizeof(unsigned) * (size_t)n_bunches * n_slices : 0;
-    double *save = nullptr;
-    unsigned *csave = nullptr;
-    BLMI_HIP_TRY(hipMalloc(&save, 2 * pbytes));
-    if (cbytes) {
-        hipError_t e = hipMalloc(&csave, cbytes);
-        if (e != hipSuccess) {
-            (void)hipFree(save);
-            return blmi_set_error(BLMI_ERR_ALLOC, "autotune: %s", hipGetErrorString(e));
-        }
-    }
+    // scratch copies of the state, freed on every return path
+    struct Scratch {
+        void *p = nullptr;
+        ~Scratch() { (void)hipFree(p); }
+    } save_buf, csave_buf;
+    BLMI_HIP_TRY(hipMalloc(&save_buf.p, 2 * pbytes));
+    if (cbytes) BLMI_HIP_TRY(hipMalloc(&csave_buf.p, cbytes));
+    double *save = static_cast<double *>(save_buf.p);
+    unsigned *csave = static_cast<unsigned *>(csave_buf.p);
     auto snapshot = [&](bool restore) -> int {
         hipMemcpyKind k = hipMemcpyDeviceToDevice;
         if (restore) {
@@ -648,8 +647,6 @@ int Ensemble::autotune(int turns, hipStream_t user, int *best_block, int *best_w
             }
         }
     }
-    (void)hipFree(save);
-    (void)hipFree(csave);
     block = rc == BLMI_OK ? bb : blk0;
     wg_per_cu = rc == BLMI_OK ? bw : wpc0;
     int rc2 = build_segments();
