@@ -41,6 +41,8 @@ SIGNATURES = {
     "blmi_rfft": [_p, _i64, _p, _i64, _p],
     "blmi_irfft": [_p, _i64, _p, _i64, _p],
     "blmi_induced_voltage": [_p, _i, _p, _i64, _d, _p, _i, _p],
+    "blmi_beam_statistics": [_p, _p, _p, _i64, _p, _p],
+    "blmi_beam_statistics_f32": [_p, _p, _p, _i64, _p, _p],
     "blmi_engine_create": [_i64, _i, _i, _i],
     "blmi_engine_destroy": [_p],
     "blmi_engine_set_particles": [_p, _p, _p, _i64],

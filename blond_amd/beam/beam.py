@@ -98,13 +98,8 @@ class Beam:
     def statistics(self):
         """beam.py:430-458: mean and r.m.s. (ddof=0) of the alive particles."""
         if self._device == "GPU":
-            alive = self.id != 0
-            dt = self.dt[alive]
-            dE = self.dE[alive]
-            self.mean_dt = float(dt.mean().item())
-            self.sigma_dt = float(dt.std(unbiased=False).item())
-            self.mean_dE = float(dE.mean().item())
-            self.sigma_dE = float(dE.std(unbiased=False).item())
+            _, self.mean_dt, self.sigma_dt, self.mean_dE, self.sigma_dE = bm.beam_statistics(
+                self.dt, self.dE, self.id)
         else:
             idx = np.nonzero(self.id)[0]
             self.mean_dt = float(np.mean(self.dt[idx]))

@@ -329,6 +329,20 @@ int blmi_induced_voltage(const double *profile, int n_slices, const double *Z, i
     return launch_induced_fourstep(a, p2, S(stream));
 }
 
+// Bunch moments over the alive particles (id != 0; id may be null): out =
+// device double[5] {n_alive, mean_dt, sigma_dt, mean_dE, sigma_dE}.
+int blmi_beam_statistics(const double *dt, const double *dE, const int64_t *id, int64_t n,
+                         double *out, void *stream) {
+    BLMI_REQUIRE(n >= 0 && out && (n == 0 || (dt && dE)), "beam_statistics: bad arguments");
+    return launch_moments<double>(dt, dE, id, n, out, S(stream));
+}
+
+int blmi_beam_statistics_f32(const float *dt, const float *dE, const int64_t *id, int64_t n,
+                             double *out, void *stream) {
+    BLMI_REQUIRE(n >= 0 && out && (n == 0 || (dt && dE)), "beam_statistics: bad arguments");
+    return launch_moments<float>(dt, dE, id, n, out, S(stream));
+}
+
 // --------------------------------------------------------------- engine ----
 void *blmi_engine_create(int64_t n, int n_rf, int n_turns, int solver) {
     if (n < 0 || n_rf < 0 || n_turns < 1 || solver < 0 || solver > 2) {

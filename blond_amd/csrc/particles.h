@@ -55,6 +55,11 @@ template <typename T>
 int launch_smooth_histogram(const T *dt, int64_t n, T *profile, T cut_left, T cut_right,
                             int n_slices, long long *scratch, hipStream_t s);
 
+// bunch moments: res[5] = {n_alive, mean_dt, sigma_dt, mean_dE, sigma_dE} (device)
+template <typename T>
+int launch_moments(const T *dt, const T *dE, const int64_t *id, int64_t n, double *res,
+                   hipStream_t s);
+
 // library workspace (capi.hip): device scratch owned by the library, grown on demand
 void *workspace(size_t bytes, int slot);
 

@@ -299,4 +299,101 @@ BLMI_INST(double)
 BLMI_INST(float)
 #undef BLMI_INST
 
+
+// ---------------------------------------------------------------------------
+// Bunch moments (beam.py:430-458: mean and r.m.s. with ddof = 0 over the
+// alive particles, id != 0), two passes like numpy's std: sums, then the
+// squared deviations from the mean.  Per-thread sums in grid-stride order,
+// a fixed tree inside each workgroup and a fixed order over workgroups:
+// the result does not depend on scheduling.
+// ---------------------------------------------------------------------------
+constexpr int kMomBlock = 256;
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+    return s;
+}
+
+template <typename T, bool DEV>
+__global__ void __launch_bounds__(kMomBlock) moments_pass_kernel(const T *__restrict__ dt,
+                                                                 const T *__restrict__ dE,
+                                                                 const int64_t *__restrict__ id,
+                                                                 int64_t n, const double *res,
+                                                                 double *__restrict__ part) {
+    __shared__ double red[kMomBlock / 64];
+    double a = 0.0, b = 0.0, c = 0.0;
+    double mt = 0.0, me = 0.0;
+    if (DEV) {
+        mt = res[1];
+        me = res[3];
+    }
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (id && id[i] == 0) continue;
+        const double x = (double)dt[i], e = (double)dE[i];
+        if (DEV) {
+            a += (x - mt) * (x - mt);
+            b += (e - me) * (e - me);
+        } else {
+            a += x;
+            b += e;
+            c += 1.0;
+        }
+    }
+    a = block_sum(a, red);
+    b = block_sum(b, red);
+    c = block_sum(c, red);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = a;
+        part[3 * blockIdx.x + 1] = b;
+        part[3 * blockIdx.x + 2] = c;
+    }
+}
+
+// res = {n_alive, mean_dt, sigma_dt, mean_dE, sigma_dE}
+__global__ void moments_final_kernel(const double *__restrict__ part, int grid, bool dev,
+                                     double *__restrict__ res) {
+    if (threadIdx.x != 0) return;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int w = 0; w < grid; ++w) {
+        a += part[3 * w];
+        b += part[3 * w + 1];
+        c += part[3 * w + 2];
+    }
+    if (!dev) {
+        res[0] = c;
+        res[1] = c > 0.0 ? a / c : 0.0;
+        res[3] = c > 0.0 ? b / c : 0.0;
+    } else {
+        const double nn = res[0];
+        res[2] = nn > 0.0 ? sqrt(a / nn) : 0.0;
+        res[4] = nn > 0.0 ? sqrt(b / nn) : 0.0;
+    }
+}
+
+template <typename T>
+int launch_moments(const T *dt, const T *dE, const int64_t *id, int64_t n, double *res,
+                   hipStream_t s) {
+    int grid = stream_grid(n, kMomBlock);
+    double *part = static_cast<double *>(workspace(sizeof(double) * 3 * (size_t)grid, 3));
+    if (!part) return blmi_set_error(BLMI_ERR_ALLOC, "moments: workspace");
+    hipLaunchKernelGGL((moments_pass_kernel<T, false>), dim3(grid), dim3(kMomBlock), 0, s, dt, dE,
+                       id, n, res, part);
+    hipLaunchKernelGGL(moments_final_kernel, dim3(1), dim3(64), 0, s, part, grid, false, res);
+    hipLaunchKernelGGL((moments_pass_kernel<T, true>), dim3(grid), dim3(kMomBlock), 0, s, dt, dE,
+                       id, n, res, part);
+    hipLaunchKernelGGL(moments_final_kernel, dim3(1), dim3(64), 0, s, part, grid, true, res);
+    return blmi_check_launch("moments");
+}
+template int launch_moments<double>(const double *, const double *, const int64_t *, int64_t,
+                                    double *, hipStream_t);
+template int launch_moments<float>(const float *, const float *, const int64_t *, int64_t, double *,
+                                   hipStream_t);
+
 }  // namespace blmi

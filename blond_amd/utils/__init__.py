@@ -28,6 +28,7 @@ class MI355XBackend:
         self.irfft = _k.irfft
         self.rfftfreq = _k.rfftfreq
         self.induced_voltage_1turn = _k.induced_voltage_1turn
+        self.beam_statistics = _k.beam_statistics
 
     def use_gpu(self, gpu_id: int = 0):
         """Select the GPU (bmath_backends.py:312-344)."""

@@ -240,6 +240,28 @@ def rfftfreq(n: int, d: float = 1.0):
     return np.fft.rfftfreq(int(n), d)
 
 
+def beam_statistics(dt, dE, id=None):
+    """beam.py:430-458: (n_alive, mean_dt, sigma_dt, mean_dE, sigma_dE) of
+    the particles with id != 0 (all when id is None), std with ddof = 0."""
+    torch = _torch()
+    x = dt if isinstance(dt, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(dt)).cuda()
+    e = dE if isinstance(dE, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(dE)).cuda()
+    if x.dtype != e.dtype or x.dtype not in (torch.float64, torch.float32):
+        raise TypeError("dt and dE must both be float64 or float32")
+    x, e = x.contiguous(), e.contiguous()
+    ids = None
+    if id is not None:
+        ids = id if isinstance(id, torch.Tensor) else torch.from_numpy(np.asarray(id)).cuda()
+        ids = ids.to(torch.int64).contiguous()
+    out = torch.empty(5, dtype=torch.float64, device="cuda")
+    name = "blmi_beam_statistics" + ("_f32" if x.dtype == torch.float32 else "")
+    _lib.call(name, ct.c_void_p(x.data_ptr()), ct.c_void_p(e.data_ptr()),
+              ct.c_void_p(ids.data_ptr()) if ids is not None else None, int(x.numel()),
+              ct.c_void_p(out.data_ptr()), _stream(torch))
+    r = out.cpu().numpy()
+    return int(r[0]), float(r[1]), float(r[2]), float(r[3]), float(r[4])
+
+
 def induced_voltage_1turn(profile, total_impedance, n_fft: int, scale: float, n_out: int = 0):
     """-(scale * irfft(Z * rfft(profile, n_fft)))[:n_out]: one launch on one
     workgroup, or three (four-step FFT) beyond its LDS (impedance.py:457-489 with scale = charge * e * ratio; n_out defaults to

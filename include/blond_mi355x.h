@@ -113,6 +113,15 @@ int blmi_fft_max_lds_length(int64_t *n);
 int blmi_rfft(const double *in, int64_t n_in, double *out, int64_t n, void *stream);
 int blmi_irfft(const double *in, int64_t m_in, double *out, int64_t n, void *stream);
 
+/* Bunch moments of the alive particles (id != 0; id may be NULL for all),
+ * replacing Beam.statistics, blond/beam/beam.py:430-458 (numpy mean and std
+ * with ddof = 0, two passes): out is a device double[5] = {n_alive,
+ * mean_dt, sigma_dt, mean_dE, sigma_dE}.  Deterministic reduction order. */
+int blmi_beam_statistics(const double *dt, const double *dE, const int64_t *id, int64_t n,
+                         double *out, void *stream);
+int blmi_beam_statistics_f32(const float *dt, const float *dE, const int64_t *id, int64_t n,
+                             double *out, void *stream);
+
 /* One-turn induced voltage: volt = -(scale * irfft(Z * rfft(profile, n_fft)))[:n_out]
  * with scale = charge * e * ratio and n_out = n_induced_voltage (or n_slices
  * for the cut of impedance.py:113-131).  Replaces

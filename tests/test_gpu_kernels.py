@@ -276,3 +276,25 @@ def test_f32_histogram_and_interp_kick_vs_reference_single(single):
     gdE = dev(dE)
     bm.linear_interp_kick(dev(dt), gdE, dev(V), dev(bc), 1.0, 0.0)
     np.testing.assert_allclose(host(gdE), rdE, rtol=0, atol=2e-7 * np.abs(rdE).max())
+
+
+# ---------------------------------------------------------------------------
+# bunch moments (beam.py:430-458)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 1000, 1000003])
+def test_beam_statistics_vs_numpy(n):
+    dt, dE = bunch(n, seed=9)
+    ids = np.arange(1, n + 1, dtype=np.int64)
+    ids[::7] = 0  # lost particles are excluded
+    got = bm.beam_statistics(dev(dt), dev(dE), dev(ids))
+    a = ids != 0
+    want = (int(a.sum()), np.mean(dt[a]), np.std(dt[a]), np.mean(dE[a]), np.std(dE[a]))
+    assert got[0] == want[0]
+    if want[0]:
+        np.testing.assert_allclose(got[1:], want[1:], rtol=1e-13, atol=0)
+    # without ids every particle counts; repeated calls are identical
+    g1 = bm.beam_statistics(dev(dt), dev(dE))
+    g2 = bm.beam_statistics(dev(dt), dev(dE))
+    assert g1 == g2 and g1[0] == n
+    np.testing.assert_allclose(g1[1:], (np.mean(dt), np.std(dt), np.mean(dE), np.std(dE)),
+                               rtol=1e-13, atol=0)
