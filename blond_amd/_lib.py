@@ -41,6 +41,10 @@ SIGNATURES = {
     "blmi_rfft": [_p, _i64, _p, _i64, _p],
     "blmi_irfft": [_p, _i64, _p, _i64, _p],
     "blmi_induced_voltage": [_p, _i, _p, _i64, _d, _p, _i, _p],
+    "blmi_interp": [_p, _i64, _p, _p, _i64, _d, _d, _p, _p],
+    "blmi_gradient": [_p, _i64, _d, _p, _p],
+    "blmi_diff": [_p, _i64, _p, _p],
+    "blmi_mtw_shift_add": [_p, _p, _p, _i64, _d, _p, _i64, _p],
     "blmi_beam_statistics": [_p, _p, _p, _i64, _p, _p],
     "blmi_beam_statistics_f32": [_p, _p, _p, _i64, _p, _p],
     "blmi_engine_create": [_i64, _i, _i, _i],
@@ -49,6 +53,7 @@ SIGNATURES = {
     "blmi_engine_set_programs": [_p, _p, _p, _p, _d, _d, _p, _p, _p, _p, _p, _p, _p, _p, _d],
     "blmi_engine_set_slices": [_p, _i, _d, _d, _p],
     "blmi_engine_set_induced": [_p, _i64, _p, _d, _p, _p],
+    "blmi_engine_set_wake": [_p, _i64, _i64, _p, _p, _p, _i],
     "blmi_engine_set_launch": [_p, _i, _i, _i],
     "blmi_engine_set_turn": [_p, _i],
     "blmi_engine_get_turn": [_p, ct.POINTER(_i)],

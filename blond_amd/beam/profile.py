@@ -149,6 +149,28 @@ class Profile:
         """profile.py:899-903."""
         self.beam_spectrum = bm.rfft(self.n_macroparticles, n_sampling_fft)
 
+    def beam_profile_derivative(self, mode: str = "gradient"):
+        """profile.py:905-937: (bin_centers, d(profile)/dt) with the spacing
+        dist = bin_centers[1] - bin_centers[0].  "gradient" is
+        numpy.gradient(profile, dist); "diff" interpolates diff(profile)/dist
+        from the bin edges' midpoints back onto the bin centres.  "filter1d"
+        is CPU-only in the reference (a scipy filter) and raises as its GPU
+        backend does."""
+        bin_centers = self.bin_centers
+        bc2 = _host(bin_centers[:2])
+        dist_centers = bc2[1] - bc2[0]
+        if mode == "filter1d":
+            raise RuntimeError("filter1d mode is not supported in GPU.")
+        if mode == "gradient":
+            derivative = bm.gradient(self.n_macroparticles, dist_centers)
+        elif mode == "diff":
+            derivative = bm.diff(self.n_macroparticles) / dist_centers
+            diff_centers = bin_centers[0:-1] + dist_centers / 2
+            derivative = bm.interp(bin_centers, diff_centers, derivative)
+        else:
+            raise RuntimeError("Option for derivative is not recognized.")
+        return bin_centers, derivative
+
     def to_gpu(self, recursive=True):
         """profile.py:924-951."""
         if self._device == "GPU":

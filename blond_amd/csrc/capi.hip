@@ -20,6 +20,7 @@
 #include "adjoint.h"
 #include "particles.h"
 #include "spectral.h"
+#include "wake.h"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -287,6 +288,34 @@ int blmi_irfft(const double *in, int64_t m_in, double *out, int64_t n, void *str
     return launch_irfft(reinterpret_cast<const double2 *>(in), m_in, out, n, p, S(stream));
 }
 
+int blmi_interp(const double *x, int64_t n, const double *xp, const double *fp, int64_t m,
+                double left, double right, double *out, void *stream) {
+    BLMI_REQUIRE(n >= 0 && m >= 1, "interp: xp must not be empty (m = %lld)", (long long)m);
+    BLMI_REQUIRE(n == 0 || (x && xp && fp && out), "interp: null array");
+    return launch_interp(x, n, xp, fp, m, left, right, out, S(stream));
+}
+
+int blmi_gradient(const double *f, int64_t n, double h, double *out, void *stream) {
+    BLMI_REQUIRE(n >= 2, "gradient: at least 2 values (edge order 1), got %lld", (long long)n);
+    BLMI_REQUIRE(f && out, "gradient: null array");
+    return launch_gradient(f, n, h, out, S(stream));
+}
+
+int blmi_diff(const double *f, int64_t n, double *out, void *stream) {
+    BLMI_REQUIRE(n >= 0, "diff: negative length");
+    BLMI_REQUIRE(n < 2 || (f && out), "diff: null array");
+    return launch_diff(f, n, out, S(stream));
+}
+
+int blmi_mtw_shift_add(const double *mem_in, double *mem_out, const double *time, int64_t n_mem,
+                       double t_rev, const double *add, int64_t n_add, void *stream) {
+    BLMI_REQUIRE(n_mem >= 0 && n_add >= 0 && n_add <= n_mem, "mtw_shift_add: bad lengths");
+    BLMI_REQUIRE(n_mem == 0 || (mem_in && mem_out && time), "mtw_shift_add: null array");
+    BLMI_REQUIRE(mem_in != mem_out || n_mem == 0, "mtw_shift_add: in-place shift is not supported");
+    BLMI_REQUIRE(n_add == 0 || add, "mtw_shift_add: null voltage");
+    return launch_mtw_shift_add(mem_in, mem_out, time, n_mem, t_rev, add, n_add, S(stream));
+}
+
 int blmi_induced_voltage(const double *profile, int n_slices, const double *Z, int64_t n_fft,
                          double scale, double *volt, int n_out, void *stream) {
     BLMI_REQUIRE(n_slices >= 1 && profile && Z && volt, "induced_voltage: bad arguments");
@@ -430,8 +459,17 @@ int blmi_engine_set_induced(void *h, int64_t n_fft, const double *Z_host, double
     Engine *e = reinterpret_cast<Engine *>(h);
     BLMI_REQUIRE(e, "engine: null handle");
     BLMI_REQUIRE(e->n_slices >= 2, "engine: set slices (>= 2) before the induced voltage");
-    BLMI_REQUIRE(volt && bin_centers && Z_host, "engine: null induced-voltage arrays");
+    BLMI_REQUIRE(volt && bin_centers, "engine: null induced-voltage arrays");
+    BLMI_REQUIRE((Z_host && n_fft >= 2) || (!Z_host && n_fft == 0),
+                 "engine: Z with n_fft >= 2, or no Z with n_fft = 0 (no convolution)");
     return e->set_induced(n_fft, Z_host, scale, volt, bin_centers, e->charge);
+}
+
+int blmi_engine_set_wake(void *h, int64_t n_mem, int64_t n_add, const double *time, double *memory,
+                         const double *ind_coef_host, int deriv_mode) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    return e->set_wake(n_mem, n_add, time, memory, ind_coef_host, deriv_mode);
 }
 
 int blmi_engine_set_launch(void *h, int block, int blocks_per_cu, int graph_chunk) {

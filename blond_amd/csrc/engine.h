@@ -7,6 +7,7 @@
 #include "blmi_common.h"
 #include "particles.h"
 #include "spectral.h"
+#include "wake.h"
 
 namespace blmi {
 
@@ -64,6 +65,20 @@ struct Engine {
     double *volt = nullptr;        // borrowed: induced voltage seen by the caller
     const double *bc = nullptr;    // borrowed: bin centres
     double *d_table = nullptr;     // (slope, intercept)[n_slices] as double2, then (bc0, inv_bw)
+    double *d_trev = nullptr;      // [n_turns + 1] revolution period program
+    // wake step after the convolution (multi-turn memory and/or an inductive
+    // Z/n source): the convolution then writes d_v1 and the step writes the
+    // caller's voltage and the table
+    bool have_conv = true;         // false: no impedance convolution (inductive sources only)
+    int64_t n_mem = 0;             // multi-turn memory length (0: none)
+    int64_t n_add = 0;             // the turn's voltage enters the memory at i < n_add
+    double *d_mem = nullptr;       // [2][n_mem], by turn parity
+    double *d_time = nullptr;      // [n_mem] memory time axis
+    double *mem_user = nullptr;    // borrowed: the caller's memory, loaded at track start and
+                                   // published after every chunk
+    double *d_v1 = nullptr;        // [max(n_mem, n_slices)] convolution output
+    double *d_indc = nullptr;      // [n_turns + 1] inductive coefficients (null: none)
+    int deriv_mode = kDerivGradient;
     // FP32 mode: float particles and programs; the slice axis (profile,
     // induced voltage, table) stays FP64 in engine-owned buffers and is
     // converted to the caller's float arrays at the end of every chunk
@@ -76,7 +91,7 @@ struct Engine {
     double *d_prof64 = nullptr, *d_volt64 = nullptr, *d_bc64 = nullptr;
     // bin-ordered working set: engine-owned double-buffered copies of the
     // particles grouped by coarse bin, re-grouped every sort_every turns
-    int sort_every = 8;            // 0: off
+    int sort_every = 0;            // 0: off (the default: slower at 1e7 particles and 65536 slices)
     double *d_ord = nullptr;       // [2][2][n]: dt, dE per buffer
     unsigned *d_perm = nullptr;    // [2][n]: caller's index of each slot
     int *d_win = nullptr;          // [grid]: window base per workgroup
@@ -110,6 +125,10 @@ struct Engine {
     int set_induced_f32(int64_t n_fft, const double *Z_host, double scale, float *volt,
                         const float *bc, double charge);
     int set_method(int method);
+    int set_wake(int64_t n_mem, int64_t n_add, const double *time, double *memory,
+                 const double *ind_coef_host, int deriv_mode);
+    bool wake_on() const { return n_mem > 0 || d_indc != nullptr; }
+    WakeArgs wake_args(int off) const;
     int set_hist(int groups, int max_peel);
     int alloc_counts();
     int set_turn(int turn);

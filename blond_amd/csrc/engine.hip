@@ -334,6 +334,14 @@ void Engine::release() {
     (void)hipFree(d_prof64);
     (void)hipFree(d_volt64);
     (void)hipFree(d_bc64);
+    (void)hipFree(d_trev);
+    (void)hipFree(d_mem);
+    (void)hipFree(d_time);
+    (void)hipFree(d_v1);
+    (void)hipFree(d_indc);
+    d_trev = d_mem = d_time = d_v1 = d_indc = nullptr;
+    n_mem = 0;
+    mem_user = nullptr;
     d_rf32 = d_acc32 = nullptr;
     d_drift32 = nullptr;
     d_prof64 = d_volt64 = d_bc64 = nullptr;
@@ -359,6 +367,7 @@ int Engine::init(int64_t n_particles, int n_rf_, int n_turns_, int solver_) {
     BLMI_HIP_TRY(hipMalloc(&d_rf, sizeof(double) * 3 * n_rf * (size_t)(n_turns + 1)));
     BLMI_HIP_TRY(hipMalloc(&d_acc, sizeof(double) * (size_t)(n_turns + 1)));
     BLMI_HIP_TRY(hipMalloc(&d_drift, sizeof(DriftCoeffs<double>) * (size_t)(n_turns + 1)));
+    BLMI_HIP_TRY(hipMalloc(&d_trev, sizeof(double) * (size_t)(n_turns + 1)));
     // d_base[0] = turn base, d_base[1..2] = turn held by each count buffer (-1: none)
     BLMI_HIP_TRY(hipMalloc(&d_base, sizeof(int) * 4));
     BLMI_HIP_TRY(hipMemset(d_base, 0, sizeof(int)));
@@ -377,6 +386,7 @@ int Engine::set_programs(const double *rf_host, const double *acc_host, const do
     const size_t nt = (size_t)n_turns + 1;
     BLMI_HIP_TRY(hipMemcpy(d_rf, rf_host, sizeof(double) * 3 * n_rf * nt, hipMemcpyHostToDevice));
     BLMI_HIP_TRY(hipMemcpy(d_acc, acc_host, sizeof(double) * nt, hipMemcpyHostToDevice));
+    BLMI_HIP_TRY(hipMemcpy(d_trev, t_rev, sizeof(double) * nt, hipMemcpyHostToDevice));
     std::vector<DriftCoeffs<double>> dc(nt);
     for (size_t i = 0; i < nt; ++i)
         dc[i] = make_drift_coeffs<double>(solver, t_rev[i], length_ratio, alpha_order, eta0[i],
@@ -482,6 +492,27 @@ int Engine::set_induced(int64_t n_fft_, const double *Z_host, double scale_, dou
     (void)hipFree(d_gl);
     d_Z = nullptr;
     d_table = d_gl = nullptr;
+    have_fft = have_direct = have_fourstep = false;
+    // a new source set drops the wake step (set_wake follows when needed)
+    (void)hipFree(d_mem);
+    (void)hipFree(d_time);
+    (void)hipFree(d_v1);
+    (void)hipFree(d_indc);
+    d_mem = d_time = d_v1 = d_indc = nullptr;
+    n_mem = 0;
+    mem_user = nullptr;
+    if (n_fft == 0 && !Z_host) {
+        // no impedance convolution: the induced voltage comes from the wake
+        // step alone (inductive sources, set_wake)
+        BLMI_HIP_TRY(hipMalloc(&d_table, sizeof(double) * (2 * (size_t)n_slices + 2)));
+        BLMI_HIP_TRY(hipMemset(d_table, 0, sizeof(double) * (2 * (size_t)n_slices + 2)));
+        have_conv = false;
+        method = -1;
+        intensity = true;
+        invalidate_graph();
+        return BLMI_OK;
+    }
+    have_conv = true;
     BLMI_HIP_TRY(hipMalloc(&d_Z, sizeof(double2) * (size_t)nz));
     BLMI_HIP_TRY(hipMemcpy(d_Z, Z_host, sizeof(double2) * (size_t)nz, hipMemcpyHostToDevice));
     BLMI_HIP_TRY(hipMalloc(&d_table, sizeof(double) * (2 * (size_t)n_slices + 2)));
@@ -538,6 +569,11 @@ int Engine::set_method(int method_) {
         return blmi_set_error(BLMI_ERR_UNSUPPORTED, "four-step FFT not available (n_fft %lld)",
                               (long long)n_fft);
     if (method_ < 0 || method_ > 2) return blmi_set_error(BLMI_ERR_ARG, "method must be 0, 1 or 2");
+    if (method_ == 1 && n_mem > n_slices)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED,
+                              "direct convolution writes n_slices values; the multi-turn memory "
+                              "needs %lld", (long long)n_mem);
+    if (!have_conv) return blmi_set_error(BLMI_ERR_ARG, "no impedance convolution is set");
     method = method_;
     invalidate_graph();
     return BLMI_OK;
@@ -790,7 +826,6 @@ InducedArgs Engine::induced_args(int off) const {
     ia.n_out = n_slices;
     ia.bc = bc;
     ia.charge = charge;
-    // the four-step path leaves the slope/intercept to the particle kernel
     ia.tab = reinterpret_cast<double2 *>(d_table);
     ia.tab_params = d_table + 2 * n_slices;
     ia.gl = d_gl;
@@ -798,10 +833,108 @@ InducedArgs Engine::induced_args(int off) const {
 }
 
 int Engine::launch_induced(hipStream_t s, int off, const KernelEvents &ev) {
-    InducedArgs ia = induced_args(off);
-    if (method == 1) return launch_induced_direct(ia, s, ev);
-    if (method == 2) return launch_induced_fourstep(ia, plan2, s, ev);
-    return launch_induced_voltage_lds(ia, plan, s, ev);
+    const bool w = wake_on();
+    if (!have_conv && !w) return blmi_set_error(BLMI_ERR_ARG, "induced voltage: no source is set");
+    if (have_conv) {
+        InducedArgs ia = induced_args(off);
+        KernelEvents e1 = ev;
+        if (w) {
+            // the convolution of this turn goes to d_v1; the wake step
+            // forms the caller's voltage and the table
+            ia.volt = d_v1;
+            ia.n_out = (int)(n_mem > n_slices ? n_mem : n_slices);
+            ia.tab = nullptr;
+            ia.tab_params = nullptr;
+            e1.stop = nullptr;
+        }
+        int rc = method == 1   ? launch_induced_direct(ia, s, e1)
+                 : method == 2 ? launch_induced_fourstep(ia, plan2, s, e1)
+                               : launch_induced_voltage_lds(ia, plan, s, e1);
+        if (rc || !w) return rc;
+    }
+    KernelEvents e2;
+    e2.start = have_conv ? nullptr : ev.start;
+    e2.stop = ev.stop;
+    return launch_wake_finish(wake_args(off), s, e2);
+}
+
+WakeArgs Engine::wake_args(int off) const {
+    WakeArgs a{};
+    a.turn_base = d_base;
+    a.turn_off = off;
+    a.v1 = have_conv ? d_v1 : nullptr;
+    a.n_mem = n_mem;
+    a.n_add = n_add;
+    a.mem0 = d_mem;
+    a.mem1 = d_mem ? d_mem + n_mem : nullptr;
+    a.time = d_time;
+    a.t_rev = d_trev;
+    a.ind_coef = d_indc;
+    a.deriv_mode = deriv_mode;
+    a.profile = profile;
+    a.counts0 = d_counts;
+    a.counts1 = d_counts + (size_t)groups * n_slices;
+    a.groups = groups;
+    a.n_slices = n_slices;
+    a.bc = bc;
+    a.charge = charge;
+    a.volt = volt;
+    a.tab = reinterpret_cast<double2 *>(d_table);
+    a.tab_params = d_table + 2 * n_slices;
+    return a;
+}
+
+int Engine::set_wake(int64_t n_mem_, int64_t n_add_, const double *time, double *memory,
+                     const double *ind_host, int deriv_mode_) {
+    if (!intensity) return blmi_set_error(BLMI_ERR_ARG, "set_wake: set the induced voltage first");
+    if (f32)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED,
+                              "multi-turn wakes and inductive sources run with FP64 particles");
+    if (n_mem_ < 0 || (n_mem_ > 0 && (!time || !memory)))
+        return blmi_set_error(BLMI_ERR_ARG, "set_wake: memory needs its time axis and buffer");
+    if (n_mem_ > 0 && !have_conv)
+        return blmi_set_error(BLMI_ERR_ARG, "set_wake: a multi-turn memory needs a convolution");
+    if (n_mem_ > 0 && (n_mem_ < n_slices || n_mem_ > n_fft))
+        return blmi_set_error(BLMI_ERR_ARG, "set_wake: memory length %lld outside [n_slices %d, n_fft %lld]",
+                              (long long)n_mem_, n_slices, (long long)n_fft);
+    if (n_add_ < 0 || n_add_ > n_mem_)
+        return blmi_set_error(BLMI_ERR_ARG, "set_wake: n_add %lld outside [0, n_mem]", (long long)n_add_);
+    if (deriv_mode_ != kDerivGradient && deriv_mode_ != kDerivDiff)
+        return blmi_set_error(BLMI_ERR_ARG, "set_wake: derivative mode 0 (gradient) or 1 (diff)");
+    if (ind_host && n_slices < 2)
+        return blmi_set_error(BLMI_ERR_ARG, "set_wake: the profile derivative needs 2 slices");
+    if (!have_conv && !ind_host)
+        return blmi_set_error(BLMI_ERR_ARG, "set_wake: no induced-voltage source");
+    (void)hipFree(d_mem);
+    (void)hipFree(d_time);
+    (void)hipFree(d_v1);
+    (void)hipFree(d_indc);
+    d_mem = d_time = d_v1 = d_indc = nullptr;
+    n_mem = n_mem_;
+    n_add = n_add_;
+    mem_user = memory;
+    deriv_mode = deriv_mode_;
+    if (n_mem > 0) {
+        BLMI_HIP_TRY(hipMalloc(&d_mem, sizeof(double) * 2 * (size_t)n_mem));
+        BLMI_HIP_TRY(hipMemset(d_mem, 0, sizeof(double) * 2 * (size_t)n_mem));
+        BLMI_HIP_TRY(hipMalloc(&d_time, sizeof(double) * (size_t)n_mem));
+        BLMI_HIP_TRY(hipMemcpy(d_time, time, sizeof(double) * (size_t)n_mem, hipMemcpyDefault));
+        if (method == 1 && n_mem > n_slices) method = have_fft ? 0 : 2;
+        if (method == 2 && !have_fourstep)
+            return blmi_set_error(BLMI_ERR_UNSUPPORTED, "multi-turn memory: no FFT for n_fft %lld",
+                                  (long long)n_fft);
+    }
+    if (have_conv) {
+        const size_t nv = (size_t)(n_mem > n_slices ? n_mem : n_slices);
+        BLMI_HIP_TRY(hipMalloc(&d_v1, sizeof(double) * nv));
+    }
+    if (ind_host) {
+        const size_t nt = (size_t)n_turns + 1;
+        BLMI_HIP_TRY(hipMalloc(&d_indc, sizeof(double) * nt));
+        BLMI_HIP_TRY(hipMemcpy(d_indc, ind_host, sizeof(double) * nt, hipMemcpyHostToDevice));
+    }
+    invalidate_graph();
+    return BLMI_OK;
 }
 
 int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const KernelEvents &ev) {
@@ -852,7 +985,12 @@ int Engine::finish_chunk(hipStream_t s, int count) {
                                (const double *)volt, volt32, n_slices);
     }
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, d_base, count);
-    return blmi_check_launch("advance");
+    int rc = blmi_check_launch("advance");
+    if (rc) return rc;
+    // the multi-turn memory of the new turn base, for the caller
+    if (n_mem > 0 && mem_user)
+        return launch_mtw_publish(d_base, d_mem, d_mem + n_mem, n_mem, mem_user, s);
+    return BLMI_OK;
 }
 
 int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *ms_turn) {
@@ -876,6 +1014,10 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
         prc = order_in(own_stream);
         if (prc) return prc;
         ord_since = 0;
+    }
+    if (n_mem > 0 && mem_user) {
+        prc = launch_mtw_load(d_base, d_mem, d_mem + n_mem, n_mem, mem_user, own_stream);
+        if (prc) return prc;
     }
     std::vector<hipEvent_t> ev(4 * (size_t)turns, nullptr);
     for (auto &x : ev) BLMI_HIP_TRY(hipEventCreate(&x));
@@ -939,6 +1081,11 @@ int Engine::track(int turns, hipStream_t user, bool use_graph) {
         prc = order_in(own_stream);
         if (prc) return prc;
         ord_since = 0;
+    }
+    // the caller's multi-turn memory (it may have been changed between calls)
+    if (n_mem > 0 && mem_user) {
+        prc = launch_mtw_load(d_base, d_mem, d_mem + n_mem, n_mem, mem_user, own_stream);
+        if (prc) return prc;
     }
     int done = 0;
     int rc = BLMI_OK;

@@ -1,2 +1,3 @@
-from .impedance import InducedVoltageFreq, TotalInducedVoltage
+from .impedance import (InducedVoltageFreq, InducedVoltageTime, InductiveImpedance,
+                        TotalInducedVoltage)
 from .impedance_sources import Resonators

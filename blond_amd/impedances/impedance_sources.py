@@ -1,5 +1,6 @@
 """Impedance sources on the tracking path: resonators
-(blond/impedances/impedance_sources.py:239-471)."""
+(blond/impedances/impedance_sources.py:239-471), in the frequency domain
+(imped_calc) and the time domain (wake_calc)."""
 from __future__ import annotations
 
 import numpy as np
@@ -44,6 +45,20 @@ class Resonators(_ImpedanceObject):
     def omega_R(self, omega_R):
         self.__frequency_R = omega_R / 2 / np.pi
         self.__omega_R = omega_R
+
+    def wake_calc(self, time_array):
+        """impedance_sources.py:345-380: the resonators' wake in Ohm/s, a
+        set-up array formed on the host with numpy as in the reference
+        (bm.cos / bm.sin are numpy's there)."""
+        self.time_array = time_array
+        self.wake = np.zeros(self.time_array.shape, dtype=bm.precision.real_t, order="C")
+        for i in range(0, self.n_resonators):
+            alpha = self.omega_R[i] / (2 * self.Q[i])
+            omega_bar = np.sqrt(self.omega_R[i] ** 2 - alpha ** 2)
+            self.wake += ((np.sign(self.time_array) + 1) * self.R_S[i] * alpha
+                          * np.exp(-alpha * self.time_array)
+                          * (np.cos(omega_bar * self.time_array)
+                             - alpha / omega_bar * np.sin(omega_bar * self.time_array)))
 
     def imped_calc(self, frequency_array):
         """impedance_sources.py:382-414: resonator impedance on the GPU

@@ -22,7 +22,7 @@ import numpy as np
 
 from .. import _lib
 from ..constants import e
-from ..impedances.impedance import InducedVoltageFreq
+from ..impedances.impedance import InducedVoltageFreq, InducedVoltageTime, InductiveImpedance
 from ..utils import bmath as bm
 
 _SOLVER = {"simple": 0, "legacy": 1, "exact": 2}
@@ -75,26 +75,71 @@ def ring_program_arrays(rp) -> list:
              rp.beta, rp.energy)]
 
 
-def total_impedance(tiv) -> tuple[np.ndarray, int]:
-    """Sum of the InducedVoltageFreq sources' impedance tables and their n_fft."""
+_CONV = (InducedVoltageFreq, InducedVoltageTime)
+_DERIV = {"gradient": 0, "diff": 1}
+
+
+def _conv_sources(tiv) -> list:
     srcs = tiv.induced_voltage_list
-    if not srcs or not all(isinstance(s, InducedVoltageFreq) for s in srcs):
-        raise NotImplementedError("the fused turn supports InducedVoltageFreq sources")
-    n_fft = srcs[0].n_fft
-    if any(s.n_fft != n_fft for s in srcs):
-        raise NotImplementedError("all sources must share n_fft")
+    if not srcs or not all(isinstance(s, _CONV + (InductiveImpedance,)) for s in srcs):
+        raise NotImplementedError("the fused turn supports InducedVoltageFreq, InducedVoltageTime "
+                                  "and InductiveImpedance sources")
+    return [s for s in srcs if isinstance(s, _CONV)]
+
+
+def total_impedance(tiv) -> tuple[np.ndarray | None, int]:
+    """Sum of the convolution sources' impedance tables (InducedVoltageFreq:
+    Z/bin_size; InducedVoltageTime: rfft of the wake) and their common n_fft;
+    (None, 0) when there is none."""
+    conv = _conv_sources(tiv)
+    if not conv:
+        return None, 0
+    n_fft = conv[0].n_fft
+    if any(s.n_fft != n_fft for s in conv):
+        raise NotImplementedError("all convolution sources must share n_fft")
     z = 0
-    for s in srcs:
+    for s in conv:
         zi = s.total_impedance
         zi = zi.cpu().numpy() if hasattr(zi, "cpu") else np.asarray(zi)
         z = z + zi
     return np.ascontiguousarray(z, dtype=np.complex128), int(n_fft)
 
 
+def wake_setup(tiv, n_turns: int):
+    """The fused turn's wake step for ``tiv``: (mtw source or None, inductive
+    coefficient program [n_turns+1] or None, derivative mode).  A multi-turn
+    source must be the only convolution source (its own voltage alone enters
+    the memory); inductive sources must share a derivative mode (their
+    coefficients are summed)."""
+    conv = _conv_sources(tiv)
+    mtw = [s for s in conv if s.multi_turn_wake]
+    if mtw and len(conv) > 1:
+        raise NotImplementedError("a multi-turn wake source must be the only convolution source")
+    ind = [s for s in tiv.induced_voltage_list if isinstance(s, InductiveImpedance)]
+    coef, mode = None, 0
+    if ind:
+        modes = {s.deriv_mode for s in ind}
+        if len(modes) != 1 or next(iter(modes)) not in _DERIV:
+            raise NotImplementedError("inductive sources: one derivative mode, 'gradient' or "
+                                      "'diff' ('filter1d' is not supported on the GPU)")
+        mode = _DERIV[next(iter(modes))]
+        coef = np.zeros(n_turns + 1, dtype=np.float64)
+        for s in ind:
+            for t in range(n_turns + 1):
+                try:
+                    coef[t] += float(s.coefficient(t))
+                except IndexError:
+                    # the last entry is never used: turn n_turns is not tracked
+                    if t < n_turns:
+                        raise ValueError(f"Z_over_n has no entry for turn {t}") from None
+                    coef[t] = np.nan
+    return (mtw[0] if mtw else None), coef, mode
+
+
 class FusedTracker:
     def __init__(self, tracker, profile=None, total_induced_voltage=None, use_graph=True,
                  block=1024, blocks_per_cu=1, graph_chunk=50, groups=0, max_peel=0,
-                 method=None, sort_every=8):
+                 method=None, sort_every=0):
         import torch
 
         if tracker.interpolation:
@@ -197,6 +242,9 @@ class FusedTracker:
     def _setup_induced(self):
         torch = self._torch
         self._Z, n_fft = total_impedance(self.tiv)
+        mtw, coef, mode = wake_setup(self.tiv, self.n_turns)
+        if (mtw is not None or coef is not None) and self.f32:
+            raise NotImplementedError("multi-turn wakes and inductive sources need FP64 particles")
         scale = self.beam.particle.charge * e * self.beam.ratio
         p = self.profile
         iv = self.tiv.induced_voltage
@@ -210,7 +258,31 @@ class FusedTracker:
             bc = torch.as_tensor(np.asarray(_host_array(bc)), device="cuda").to(dtype).contiguous()
             self._bc = bc
         _lib.call("blmi_engine_set_induced_f32" if self.f32 else "blmi_engine_set_induced",
-                  self._h, int(n_fft), _dptr(self._Z), float(scale), _ptr(iv), _ptr(bc))
+                  self._h, int(n_fft), _dptr(self._Z) if self._Z is not None else None,
+                  float(scale), _ptr(iv), _ptr(bc))
+        if mtw is None and coef is None:
+            return
+        n_mem = n_add = 0
+        time_ptr = mem_ptr = None
+        if mtw is not None:
+            # the engine reads and publishes the source's own memory buffer
+            mem = mtw.mtw_memory
+            if not (hasattr(mem, "is_cuda") and mem.is_cuda and mem.dtype == torch.float64
+                    and mem.is_contiguous()):
+                mem = torch.as_tensor(np.asarray(_host_array(mem)), dtype=torch.float64,
+                                      device="cuda").contiguous()
+                mtw.mtw_memory = mem
+            tm = torch.as_tensor(np.asarray(_host_array(mtw.time_mtw)), dtype=torch.float64,
+                                 device="cuda").contiguous()
+            self._mtw = (mtw, mem, tm)
+            n_mem = int(mtw.n_mtw_memory)
+            n_iv = int(mtw.n_induced_voltage)
+            n_add = slice(n_iv - mtw.front_wake_buffer, None).indices(n_iv)[0]
+            time_ptr, mem_ptr = _ptr(tm), _ptr(mem)
+            mtw.induced_voltage = mem[:n_iv]
+        self._coef = coef
+        _lib.call("blmi_engine_set_wake", self._h, n_mem, n_add, time_ptr, mem_ptr,
+                  _dptr(coef) if coef is not None else None, int(mode))
 
     # ------------------------------------------------------------------
     def invalidate(self):

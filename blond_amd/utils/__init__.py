@@ -29,6 +29,10 @@ class MI355XBackend:
         self.rfftfreq = _k.rfftfreq
         self.induced_voltage_1turn = _k.induced_voltage_1turn
         self.beam_statistics = _k.beam_statistics
+        self.interp = _k.interp
+        self.gradient = _k.gradient
+        self.diff = _k.diff
+        self.mtw_shift_add = _k.mtw_shift_add
 
     def use_gpu(self, gpu_id: int = 0):
         """Select the GPU (bmath_backends.py:312-344)."""

@@ -13,6 +13,7 @@ numpy FFTs the backend keeps (blond/utils/bmath_backends.py:447-449):
     rf_volt_comp(voltages, omega_rf, phi_rf, bin_centers) -> array
     fast_resonator(R_S, Q, frequency_array, frequency_R, impedance=None) -> array
     rfft(a, n=0), irfft(a, n=0), rfftfreq(n, d=1.0)
+    interp(x, xp, fp, left=None, right=None), gradient(f, h), diff(a)
 
 Arrays are torch tensors resident on the GPU (the MI355X counterpart of the
 reference GPU backend's cupy arrays, bmath_backends.py:717-776).  numpy
@@ -238,6 +239,85 @@ def irfft(a, n: int = 0):
 def rfftfreq(n: int, d: float = 1.0):
     """numpy.fft.rfftfreq (host arithmetic: a setup-time frequency axis)."""
     return np.fft.rfftfreq(int(n), d)
+
+
+# ---------------------------------------------------------------------------
+# numpy helpers of the wake and profile-derivative steps (float64 on the device)
+# ---------------------------------------------------------------------------
+def _f64(torch, a, name):
+    """Device float64 contiguous copy/view of a numpy array or tensor."""
+    if isinstance(a, torch.Tensor):
+        if not a.is_cuda:
+            raise TypeError(f"{name}: torch tensor must be on the GPU (use to_gpu())")
+        return a.to(torch.float64).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda")
+
+
+def _like(torch, out, ref):
+    return out if isinstance(ref, torch.Tensor) else out.cpu().numpy()
+
+
+def interp(x, xp, fp, left=None, right=None):
+    """numpy.interp (bm.interp, bmath_backends.py:373) in float64: xp
+    increasing; left/right default to fp[0] / fp[-1]."""
+    torch = _torch()
+    X, XP, FP = _f64(torch, x, "x"), _f64(torch, xp, "xp"), _f64(torch, fp, "fp")
+    m = int(XP.numel())
+    if m == 0:
+        raise ValueError("array of sample points is empty")
+    if int(FP.numel()) != m:
+        raise ValueError("fp and xp are not of the same length.")
+    lv = float(FP[0].item()) if left is None else float(left)
+    rv = float(FP[-1].item()) if right is None else float(right)
+    out = torch.empty_like(X)
+    _lib.call("blmi_interp", ct.c_void_p(X.data_ptr()), int(X.numel()), ct.c_void_p(XP.data_ptr()),
+              ct.c_void_p(FP.data_ptr()), m, lv, rv, ct.c_void_p(out.data_ptr()), _stream(torch))
+    return _like(torch, out, x)
+
+
+def gradient(f, h=1.0):
+    """numpy.gradient(f, h) for a 1-D array and scalar spacing, edge order 1
+    (bm.gradient, bmath_backends.py:429), float64."""
+    torch = _torch()
+    F = _f64(torch, f, "f")
+    n = int(F.numel())
+    if n < 2:
+        raise ValueError("Shape of array too small to calculate a numerical gradient, "
+                         "at least (edge_order + 1) elements are required.")
+    out = torch.empty_like(F)
+    _lib.call("blmi_gradient", ct.c_void_p(F.data_ptr()), n, float(h), ct.c_void_p(out.data_ptr()),
+              _stream(torch))
+    return _like(torch, out, f)
+
+
+def diff(a):
+    """numpy.diff(a) of a 1-D array (bm.diff, bmath_backends.py:441), float64."""
+    torch = _torch()
+    A = _f64(torch, a, "a")
+    n = int(A.numel())
+    out = torch.empty(max(n - 1, 0), dtype=torch.float64, device="cuda")
+    _lib.call("blmi_diff", ct.c_void_p(A.data_ptr()), n, ct.c_void_p(out.data_ptr()),
+              _stream(torch))
+    return _like(torch, out, a)
+
+
+def mtw_shift_add(memory, time, t_rev, voltage, n_add=None):
+    """One multi-turn-wake update (impedance.py:491-512, 533-550):
+    interp(time + t_rev, time, memory, 0, 0) with voltage[:n_add] added to
+    its first n_add points.  Returns a new float64 array of the kind of
+    ``memory``."""
+    torch = _torch()
+    M, T = _f64(torch, memory, "memory"), _f64(torch, time, "time")
+    n = int(M.numel())
+    if int(T.numel()) != n:
+        raise ValueError("memory and time differ in length")
+    V = _f64(torch, voltage, "voltage") if voltage is not None else None
+    k = 0 if V is None else min(int(V.numel()), n if n_add is None else int(n_add))
+    out = torch.empty_like(M)
+    _lib.call("blmi_mtw_shift_add", ct.c_void_p(M.data_ptr()), ct.c_void_p(out.data_ptr()),
+              ct.c_void_p(T.data_ptr()), n, float(t_rev),
+              ct.c_void_p(V.data_ptr()) if V is not None else None, k, _stream(torch))
+    return _like(torch, out, memory)
 
 
 def beam_statistics(dt, dE, id=None):

@@ -19,6 +19,9 @@
       narrowband resonators drawn as in c2 (n_fft = 1024).  Seeds: counts,
       V, phi from default_rng(seed); bunch b from bigaussian(seed*1e5 + 2b),
       its resonators from default_rng(seed*1e5 + 5e4 + b).
+  wake workloads (build_wake_workload): a c2-ring bunch (intensity 1e12,
+      256 slices over [0, t_rf]) with a multi-turn wake or an inductive
+      source; see that function.
 
 Bunch draws follow bigaussian(seed=1): dt from default_rng(1), dE from
 default_rng(2).
@@ -33,7 +36,8 @@ from .beam.beam import Beam, Proton
 from .beam.distributions import bigaussian
 from .beam.profile import CutOptions, OtherSlicesOptions, Profile
 from .constants import e
-from .impedances.impedance import InducedVoltageFreq, TotalInducedVoltage
+from .impedances.impedance import (InducedVoltageFreq, InducedVoltageTime, InductiveImpedance,
+                                   TotalInducedVoltage)
 from .impedances.impedance_sources import Resonators
 from .input_parameters.rf_parameters import RFStation
 from .input_parameters.ring import Ring
@@ -121,18 +125,39 @@ class Workload:
                       bin_centers=_h(p.bin_centers).copy(),
                       profile=_h(p.n_macroparticles).copy())
         if self.tiv is not None:
-            src = self.tiv.induced_voltage_list[0]
-            z = 0
-            for s in self.tiv.induced_voltage_list:
-                z = z + _h(s.total_impedance)
-            st.update(Z=np.asarray(z, dtype=np.complex128), n_fft=int(src.n_fft),
-                      n_induced_voltage=int(src.n_induced_voltage),
-                      ratio=float(self.beam.ratio),
+            srcs = self.tiv.induced_voltage_list
+            conv = [s for s in srcs if isinstance(s, (InducedVoltageFreq, InducedVoltageTime))]
+            st.update(ratio=float(self.beam.ratio),
                       induced_voltage=_h(self.tiv.induced_voltage).copy())
+            if conv:
+                z = 0
+                for s in conv:
+                    z = z + _h(s.total_impedance)
+                st.update(Z=np.asarray(z, dtype=np.complex128), n_fft=int(conv[0].n_fft),
+                          n_induced_voltage=int(conv[0].n_induced_voltage))
+            if len(conv) != len(srcs) or any(s.multi_turn_wake for s in conv):
+                # per-source form: multi-turn memories and inductive sources
+                st["bin_size"] = float(self.profile.bin_size)
+                st["sources"] = [_source_state(s) for s in srcs]
         return st
 
     def oracle_copy(self) -> dict:
         return self.snapshot()
+
+
+def _source_state(s) -> dict:
+    """Host copy of one induced-voltage source for a CPU restatement."""
+    if isinstance(s, InductiveImpedance):
+        return {"kind": "inductive", "Z_over_n": np.asarray(s.Z_over_n, dtype=np.float64),
+                "deriv_mode": s.deriv_mode}
+    d = {"kind": "convolution", "Z": np.asarray(_h(s.total_impedance), dtype=np.complex128),
+         "n_fft": int(s.n_fft), "n_induced_voltage": int(s.n_induced_voltage),
+         "multi_turn_wake": bool(s.multi_turn_wake)}
+    if s.multi_turn_wake:
+        d.update(memory=np.array(_h(s.mtw_memory), dtype=np.float64),
+                 n_mtw_memory=int(s.n_mtw_memory), wake_length=float(s.wake_length),
+                 front_wake_buffer=int(s.front_wake_buffer))
+    return d
 
 
 def _h(a):
@@ -195,6 +220,64 @@ def build_workload(name: str, n_particles: int | None = None, n_turns: int | Non
         w.meta["n_fft"] = ivf.n_fft
     else:
         raise ValueError(f"unknown workload {name!r}")
+    w.meta.update(n_particles=n_particles, n_turns=n_turns, n_slices=n_slices)
+    if to_gpu:
+        w.to_gpu()
+    return w
+
+
+WAKE_KINDS = ("mtw", "mtw_front", "mtw_time", "zovern", "zovern_diff", "mixed")
+
+
+def build_wake_workload(kind: str, n_particles: int = 50_000, n_turns: int = 40,
+                        n_slices: int = 256, memory_turns: int = 4, seed: int = 1,
+                        to_gpu: bool = True) -> Workload:
+    """A c2-ring bunch (h = 1, 8 kV, Gaussian 4 sigma = 400 ns, intensity
+    1e12, n_slices over [0, t_rf] = one revolution) whose induced voltage
+    comes from:
+
+      mtw          InducedVoltageFreq with a multi-turn wake (mode 'time',
+                   impedance.py:491-550) over `memory_turns` revolutions
+                   (frequency resolution 1/(memory_turns * t_rev)), of a
+                   broadband resonator (5 kOhm, 1 GHz, Q = 1) and a
+                   long-range one (20 kOhm at 1.5 f_rev, Q = 200)
+      mtw_front    the same with a front-wake cut of 8 bins (impedance.py:867-871)
+      mtw_time     InducedVoltageTime with a multi-turn wake of the same
+                   resonators, wake length memory_turns * t_rf
+      zovern       InductiveImpedance alone, Z/n = -100 Ohm every turn,
+                   derivative 'gradient'
+      zovern_diff  the same with derivative 'diff'
+      mixed        InducedVoltageFreq (single turn, the resonators above)
+                   plus InductiveImpedance (Z/n = 50 Ohm, 'diff')
+    Map: profile, induced voltage, tracker."""
+    if kind not in WAKE_KINDS:
+        raise ValueError(f"unknown wake workload {kind!r}")
+    ring = Ring(157.08, 1 / 4.4 ** 2, 310e6, Proton(), n_turns)
+    rf = RFStation(ring, [1], [8e3], [np.pi])
+    beam = Beam(ring, n_particles, 1e12)
+    bigaussian(ring, rf, beam, sigma_dt=C2_SIGMA_DT, seed=seed)
+    prof = _profile(beam, 0.0, float(rf.t_rf[0, 0]), n_slices)
+    f_rev = float(ring.f_rev[0])
+    res = Resonators([5e3, 2e4], [1e9, 1.5 * f_rev], [1.0, 200.0])
+    f_res = (1 + 1e-9) / (memory_turns * n_slices * prof.bin_size)
+    srcs = []
+    if kind in ("mtw", "mtw_front"):
+        srcs.append(InducedVoltageFreq(beam, prof, [res], frequency_resolution=f_res,
+                                       multi_turn_wake=True, rf_station=rf, mtw_mode="time",
+                                       front_wake_length=8 * prof.bin_size if kind == "mtw_front"
+                                       else 0))
+    elif kind == "mtw_time":
+        srcs.append(InducedVoltageTime(beam, prof, [res],
+                                       wake_length=memory_turns * float(rf.t_rf[0, 0]),
+                                       multi_turn_wake=True, rf_station=rf))
+    elif kind in ("zovern", "zovern_diff"):
+        srcs.append(InductiveImpedance(beam, prof, [-100.0] * (n_turns + 1), rf,
+                                       deriv_mode="gradient" if kind == "zovern" else "diff"))
+    else:
+        srcs.append(InducedVoltageFreq(beam, prof, [res], frequency_resolution=f_res))
+        srcs.append(InductiveImpedance(beam, prof, [50.0] * (n_turns + 1), rf, deriv_mode="diff"))
+    tiv = TotalInducedVoltage(beam, prof, srcs)
+    w = Workload(kind, ring, rf, beam, RingAndRFTracker(rf, beam), prof, tiv, True)
     w.meta.update(n_particles=n_particles, n_turns=n_turns, n_slices=n_slices)
     if to_gpu:
         w.to_gpu()

@@ -133,6 +133,31 @@ int blmi_beam_statistics_f32(const float *dt, const float *dE, const int64_t *id
 int blmi_induced_voltage(const double *profile, int n_slices, const double *Z, int64_t n_fft,
                          double scale, double *volt, int n_out, void *stream);
 
+/* numpy.interp(x, xp, fp, left, right) -> out[n] (xp increasing, m >= 1;
+ * pass fp[0] / fp[m-1] for numpy's default left / right).  Replaces bm.interp
+ * (blond/utils/bmath_backends.py:373 np.interp, 484 cp.interp) as called by
+ * _InducedVoltage.shift_trev_time (blond/impedances/impedance.py:533-550) and
+ * the "diff" profile derivative (blond/beam/profile.py:929-932). */
+int blmi_interp(const double *x, int64_t n, const double *xp, const double *fp, int64_t m,
+                double left, double right, double *out, void *stream);
+
+/* numpy.gradient(f, h), uniform spacing, edge order 1 -> out[n] (n >= 2).
+ * Replaces bm.gradient (bmath_backends.py:429) in
+ * Profile.beam_profile_derivative("gradient") (profile.py:927-928). */
+int blmi_gradient(const double *f, int64_t n, double h, double *out, void *stream);
+
+/* numpy.diff(f) -> out[n-1].  Replaces bm.diff (bmath_backends.py:441) in
+ * Profile.beam_profile_derivative("diff") (profile.py:929-932). */
+int blmi_diff(const double *f, int64_t n, double *out, void *stream);
+
+/* Multi-turn wake memory, time mode, in one pass:
+ *   mem_out[i] = interp(time[i] + t_rev, time, mem_in, left=0, right=0)
+ *                + (i < n_add ? add[i] : 0),  i < n_mem,
+ * i.e. _InducedVoltage.shift_trev_time followed by the memory update of
+ * induced_voltage_mtw (impedance.py:491-512, 533-550).  mem_in != mem_out. */
+int blmi_mtw_shift_add(const double *mem_in, double *mem_out, const double *time, int64_t n_mem,
+                       double t_rev, const double *add, int64_t n_add, void *stream);
+
 /* Fused turn engine.  Runs the reference's map
  *   with intensity effects:  [Profile, TotalInducedVoltage, RingAndRFTracker]
  *                            (EX_18_robinson_instability.py:226; tracker.py:521-615,
@@ -168,14 +193,33 @@ int blmi_engine_set_induced_f32(void *engine, int64_t n_fft, const double *Z_hos
                                 float *volt, const float *bin_centers);
 int blmi_engine_set_induced(void *engine, int64_t n_fft, const double *Z_host, double scale,
                             double *volt, const double *bin_centers);
+/* Wake step of the fused turn, after the impedance convolution.
+ *   Multi-turn wake (mtw_mode "time"; induced_voltage_mtw, impedance.py:491-512,
+ *   shift_trev_time 533-550): n_mem > 0 points with the memory time axis
+ *   `time` and the caller's memory buffer `memory` (device arrays of n_mem
+ *   doubles, n_slices <= n_mem <= n_fft); the turn's voltage enters the
+ *   memory at i < n_add (n_mem minus the front-wake buffer, impedance.py:506-508).
+ *   The memory is read from `memory`
+ *   at the start of every blmi_engine_track / time_kernels call and written
+ *   back after every chunk; the shift uses the t_rev program of set_programs.
+ *   Inductive source (InductiveImpedance.induced_voltage_1turn,
+ *   impedance.py:986-1006): ind_coef_host[n_turns+1] holds, per turn,
+ *   charge*e/(2 pi)*ratio*Z_over_n[t]*t_rev[t]/bin_size, and deriv_mode picks
+ *   the profile derivative (0 "gradient", 1 "diff"; profile.py:905-937);
+ *   NULL for none.
+ * Call after blmi_engine_set_induced, which drops it; set_induced with
+ * Z_host = NULL and n_fft = 0 means no convolution (inductive sources only).
+ * FP64 particles only. */
+int blmi_engine_set_wake(void *engine, int64_t n_mem, int64_t n_add, const double *time,
+                         double *memory, const double *ind_coef_host, int deriv_mode);
 /* Launch shape of the particle kernel (block in {64..1024}, workgroups per CU)
  * and the number of turns per captured graph. */
 int blmi_engine_set_launch(void *engine, int block, int blocks_per_cu, int graph_chunk);
 /* Induced-voltage method: 0 = one-workgroup LDS FFT (rfft, x Z, irfft),
  * 1 = direct convolution over all CUs (n_slices <= 8192), 2 = four-step FFT
- * over many workgroups (any 5-smooth n_fft with a factor 16; the particle
- * kernel then evaluates the interpolation slope and intercept from V and the
- * bin centres itself).  set_induced picks 1, else 0, else 2. */
+ * over many workgroups (any 5-smooth n_fft with a factor 16).  set_induced
+ * picks 1, else 0, else 2; a multi-turn memory longer than n_slices moves 1
+ * to 0 (or 2). */
 int blmi_engine_set_method(void *engine, int method);
 int blmi_engine_get_method(void *engine, int *method);
 /* Histogram: `groups` copies of the global counts (1..16, or 0 = automatic:
@@ -186,10 +230,12 @@ int blmi_engine_get_method(void *engine, int *method);
 int blmi_engine_set_hist(void *engine, int groups, int max_peel);
 /* Bin-ordered working set for slice counts above 4096 (FP64): the engine
  * tracks copies of the particles grouped by coarse bin (a counting sort
- * every `sort_every` turns, 0 = off, default 8) so that each workgroup's
+ * every `sort_every` turns; 0 = off, the default) so that each workgroup's
  * histogram and interpolation table fit a 4096-bin LDS window; the caller's
  * arrays are copied in and written back in their own order by every
- * track().  Results are bit-identical to the unordered engine. */
+ * track().  Results are bit-identical to the unordered engine.  Off by
+ * default: at 1e7 particles and 65536 slices it measured slower (the sort
+ * and the window kernel cost more than the gathers they save). */
 int blmi_engine_set_order(void *engine, int sort_every);
 int blmi_engine_set_turn(void *engine, int turn);   /* synchronous */
 int blmi_engine_get_turn(void *engine, int *turn);  /* synchronous */

@@ -13,9 +13,14 @@ Three layers:
   linear_interp_kick.cpp:21, kick.cpp:37, fast_resonator.cpp:19).
 * numpy restatements of the slice-axis steps that the reference does in
   Python/numpy: the induced voltage of one turn
-  (blond/impedances/impedance.py:457-489, 113-131), the FFT sizing rule
+  (blond/impedances/impedance.py:457-489, 113-131), the multi-turn wake
+  memory (impedance.py:405-453, 491-550, mode "time"), the inductive Z/n
+  source (impedance.py:986-1006 with profile.py:905-937), the FFT sizing rule
   (next_regular.py:9-59, impedance.py:349-403, 846-874) and the turn loops of
-  tracker.py:521-615, profile.py:700-719 and impedance.py:133-146.
+  tracker.py:521-615, profile.py:700-719 and impedance.py:133-146.  numpy's
+  interp, gradient, diff and FFTs are the functions the reference itself
+  calls there (bm.interp / bm.gradient / bm.diff / bm.rfft / bm.irfft,
+  bmath_backends.py:373-449).
 * ``CpuAdjoint``: a float64 reverse-mode restatement of the differentiable
   turn, used as the CPU reference gradient.
 
@@ -179,6 +184,68 @@ def induced_voltage_1turn(profile, total_impedance, n_fft, charge, ratio,
     return np.ascontiguousarray(v[:n_induced_voltage][:n_slices])
 
 
+def mtw_time_axis(wake_length: float, n_mtw_memory: int) -> np.ndarray:
+    """impedance.py:440-447: linspace(0, wake_length, n, endpoint=False)."""
+    return np.linspace(0, wake_length, n_mtw_memory, endpoint=False, dtype=np.float64)
+
+
+def mtw_update(memory, time_mtw, t_rev, voltage, n_induced_voltage, front_wake_buffer):
+    """induced_voltage_mtw (impedance.py:491-516) after the one-turn voltage:
+    shift_trev_time (533-550), the front-wake tail of this turn's voltage set
+    to 0, the voltage added to the first n_induced_voltage points.  Returns
+    (new memory, induced voltage = memory[:n_induced_voltage])."""
+    v = np.array(voltage, dtype=np.float64, copy=True)
+    v[n_induced_voltage - front_wake_buffer:] = 0
+    mem = np.interp(time_mtw + t_rev, time_mtw, memory, left=0, right=0)
+    mem[:n_induced_voltage] += v
+    return mem, mem[:n_induced_voltage]
+
+
+def profile_derivative(profile, bin_centers, mode: str):
+    """Profile.beam_profile_derivative (profile.py:905-937), modes
+    'gradient' and 'diff'."""
+    dist_centers = bin_centers[1] - bin_centers[0]
+    if mode == "gradient":
+        return np.gradient(profile, dist_centers)
+    if mode == "diff":
+        derivative = np.diff(profile) / dist_centers
+        diff_centers = bin_centers[0:-1] + dist_centers / 2
+        return np.interp(bin_centers, diff_centers, derivative)
+    raise RuntimeError("Option for derivative is not recognized.")
+
+
+def inductive_voltage(profile, bin_centers, Z_over_n, t_rev, turn, charge, ratio, bin_size,
+                      mode):
+    """InductiveImpedance.induced_voltage_1turn (impedance.py:986-1006)."""
+    return -(charge * E_CHARGE / (2 * np.pi) * ratio * Z_over_n[turn] * t_rev[turn] / bin_size
+             * profile_derivative(profile, bin_centers, mode))
+
+
+def induced_voltage_sources(state: dict, profile, turn: int, memories: list):
+    """TotalInducedVoltage.induced_voltage_sum (impedance.py:113-131) over
+    state["sources"] in list order; multi-turn memories in `memories` are
+    replaced by their updated arrays."""
+    ns = int(state["n_slices"])
+    total = 0
+    for k, s in enumerate(state["sources"]):
+        if s["kind"] == "inductive":
+            v = inductive_voltage(profile, state["bin_centers"], s["Z_over_n"], state["t_rev"],
+                                  turn, state["charge"], state["ratio"], state["bin_size"],
+                                  s["deriv_mode"])
+            v = v[:ns]
+        else:
+            n_iv = s["n_induced_voltage"]
+            spec = np.fft.rfft(profile, s["n_fft"])
+            y = np.fft.irfft(s["Z"] * spec)
+            v = -(state["charge"] * E_CHARGE * state["ratio"] * y)[:n_iv]
+            if s["multi_turn_wake"]:
+                tm = mtw_time_axis(s["wake_length"], s["n_mtw_memory"])
+                memories[k], v = mtw_update(memories[k], tm, state["t_rev"][turn], v, n_iv,
+                                            s["front_wake_buffer"])
+        total = total + v[:ns]
+    return np.ascontiguousarray(total, dtype=np.float64)
+
+
 def resonator_impedance(freq, R_S, f_r, Q):
     """Closed form Z(f) = sum_k R_k / (1 + j Q_k (f/f_rk - f_rk/f)), Z(0) = 0."""
     z = np.zeros(len(freq), dtype=np.complex128)
@@ -204,12 +271,17 @@ def track(state: dict, n_turns: int, lib: Lib | None = None) -> dict:
     prof = np.array(state["profile"], dtype=r, copy=True) if "profile" in state else None
     volt_ind = np.array(state["induced_voltage"], copy=True) if state.get("intensity") else None
     t0 = int(state["turn"])
+    sources = state.get("sources")
+    memories = [s.get("memory") for s in sources] if sources else None
     for t in range(t0, t0 + n_turns):
         if state.get("intensity"):
             lib.slice_beam(dt, prof, state["cut_left"], state["cut_right"])
-            volt_ind = induced_voltage_1turn(prof, state["Z"], state["n_fft"], state["charge"],
-                                             state["ratio"], state["n_induced_voltage"],
-                                             state["n_slices"])
+            if sources:
+                volt_ind = induced_voltage_sources(state, prof, t, memories)
+            else:
+                volt_ind = induced_voltage_1turn(prof, state["Z"], state["n_fft"],
+                                                 state["charge"], state["ratio"],
+                                                 state["n_induced_voltage"], state["n_slices"])
             lib.linear_interp_kick(dt, dE, volt_ind, state["bin_centers"], state["charge"], 0.0)
         lib.kick(dt, dE, state["voltage"][:, t], state["omega_rf"][:, t], state["phi_rf"][:, t],
                  state["charge"], state["n_rf"], state["acc"][t])
@@ -225,4 +297,7 @@ def track(state: dict, n_turns: int, lib: Lib | None = None) -> dict:
         st["profile"] = prof
     if volt_ind is not None:
         st["induced_voltage"] = volt_ind
+    if sources:
+        st["sources"] = [dict(s, memory=m) if m is not None else dict(s)
+                         for s, m in zip(sources, memories)]
     return st
