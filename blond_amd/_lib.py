@@ -45,6 +45,9 @@ SIGNATURES = {
     "blmi_gradient": [_p, _i64, _d, _p, _p],
     "blmi_diff": [_p, _i64, _p, _p],
     "blmi_mtw_shift_add": [_p, _p, _p, _i64, _d, _p, _i64, _p],
+    "blmi_beam_phase": [_p, _p, _i, _d, _d, _d, _d, _p, _p],
+    "blmi_beam_phase_fast": [_p, _p, _i, _d, _d, _d, _p, _p],
+    "blmi_strided_mean": [_p, _i64, _i64, _p, _p],
     "blmi_beam_statistics": [_p, _p, _p, _i64, _p, _p],
     "blmi_beam_statistics_f32": [_p, _p, _p, _i64, _p, _p],
     "blmi_engine_create": [_i64, _i, _i, _i],
@@ -54,6 +57,9 @@ SIGNATURES = {
     "blmi_engine_set_slices": [_p, _i, _d, _d, _p],
     "blmi_engine_set_induced": [_p, _i64, _p, _d, _p, _p],
     "blmi_engine_set_wake": [_p, _i64, _i64, _p, _p, _p, _i],
+    "blmi_engine_set_feedback": [_p, _i, _i, _i, _d, _d, _p, _p, _p, _p, _p, _i, _p],
+    "blmi_engine_get_feedback": [_p, _p],
+    "blmi_engine_get_programs": [_p, _i, _i, _p],
     "blmi_engine_set_launch": [_p, _i, _i, _i],
     "blmi_engine_set_turn": [_p, _i],
     "blmi_engine_get_turn": [_p, ct.POINTER(_i)],

@@ -132,6 +132,41 @@ __device__ __forceinline__ void cephes_sincos(double x, double &s, double &c) {
     if (x < 0.0) s = -s;
 }
 
+// Exponential by the Cephes method (exp.c), arranged as in the VDT library
+// the reference vendors (blond/cpp_routines/exp.h:70-113): n = floor'(x
+// log2(e) + 1/2) with floor'(y) = trunc(y) - signbit((float)y)
+// (vdtcore_common.h:218-224), the two-part reduction x - n C1 - n C2, the
+// Pade form 1 + 2 x P(x^2) / (Q(x^2) - x P(x^2)) with Cephes's P and Q, times
+// 2^n; +inf above 708 and 0 below -708.
+__device__ __forceinline__ double cephes_exp(double x0) {
+    double x = x0;
+    const double t = 1.4426950408889634073599 * x + 0.5;
+    double px = (double)((int)t - (signbit((float)t) ? 1 : 0));
+    const int n = (int)px;
+    x -= px * 6.93145751953125E-1;
+    x -= px * 1.42860682030941723212E-6;
+    const double xx = x * x;
+    px = 1.26177193074810590878E-4;
+    px *= xx;
+    px += 3.02994407707441961300E-2;
+    px *= xx;
+    px += 9.99999999999999999910E-1;
+    px *= x;
+    double qx = 3.00198505138664455042E-6;
+    qx *= xx;
+    qx += 2.52448340349684104192E-3;
+    qx *= xx;
+    qx += 2.27265548208155028766E-1;
+    qx *= xx;
+    qx += 2.00000000000000000009E0;
+    x = px / (qx - px);
+    x = 1.0 + 2.0 * x;
+    x *= __longlong_as_double((long long)((((uint64_t)(int64_t)n) + 1023) << 52));
+    if (x0 > 708.0) x = __longlong_as_double(0x7ff0000000000000LL);
+    if (x0 < -708.0) x = 0.0;
+    return x;
+}
+
 // float version (sincos.h:166-233, Cephes sinf)
 __device__ __forceinline__ float cephes_sinf(float x) {
     const float kFourOverPiF = (float)(4.0 / 3.14159265358979323846);

@@ -21,6 +21,7 @@
 #include "particles.h"
 #include "spectral.h"
 #include "wake.h"
+#include "feedback.h"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -288,6 +289,26 @@ int blmi_irfft(const double *in, int64_t m_in, double *out, int64_t n, void *str
     return launch_irfft(reinterpret_cast<const double2 *>(in), m_in, out, n, p, S(stream));
 }
 
+int blmi_beam_phase(const double *bin_centers, const double *profile, int n_bins, double alpha,
+                    double omega_rf, double phi_rf, double bin_size, double *coeff, void *stream) {
+    BLMI_REQUIRE(n_bins >= 1 && bin_centers && profile && coeff, "beam_phase: bad arguments");
+    return launch_beam_phase(bin_centers, profile, n_bins, alpha, omega_rf, phi_rf, bin_size, false,
+                             coeff, S(stream));
+}
+
+int blmi_beam_phase_fast(const double *bin_centers, const double *profile, int n_bins,
+                         double omega_rf, double phi_rf, double bin_size, double *coeff,
+                         void *stream) {
+    BLMI_REQUIRE(n_bins >= 1 && bin_centers && profile && coeff, "beam_phase_fast: bad arguments");
+    return launch_beam_phase(bin_centers, profile, n_bins, 0.0, omega_rf, phi_rf, bin_size, true,
+                             coeff, S(stream));
+}
+
+int blmi_strided_mean(const double *x, int64_t n, int64_t stride, double *out, void *stream) {
+    BLMI_REQUIRE(x && out && n >= 1 && stride >= 1, "strided_mean: bad arguments");
+    return launch_strided_mean(x, n, stride, out, S(stream));
+}
+
 int blmi_interp(const double *x, int64_t n, const double *xp, const double *fp, int64_t m,
                 double left, double right, double *out, void *stream) {
     BLMI_REQUIRE(n >= 0 && m >= 1, "interp: xp must not be empty (m = %lld)", (long long)m);
@@ -463,6 +484,29 @@ int blmi_engine_set_induced(void *h, int64_t n_fft, const double *Z_host, double
     BLMI_REQUIRE((Z_host && n_fft >= 2) || (!Z_host && n_fft == 0),
                  "engine: Z with n_fft >= 2, or no Z with n_fft = 0 (no convolution)");
     return e->set_induced(n_fft, Z_host, scale, volt, bin_centers, e->charge);
+}
+
+int blmi_engine_set_feedback(void *h, int machine, int delay, int first_bin, double alpha,
+                             double reference, const double *bin_centers,
+                             const double *harmonic_host, const double *omega_rf_d_host,
+                             const double *prog_host, const double *on_time_host, int n_on_time,
+                             const double *state_host) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    return e->set_feedback(machine, delay, first_bin, alpha, reference, bin_centers, harmonic_host,
+                           omega_rf_d_host, prog_host, on_time_host, n_on_time, state_host);
+}
+
+int blmi_engine_get_feedback(void *h, double *state_host) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e && state_host, "engine: null handle");
+    return e->get_feedback_state(state_host);
+}
+
+int blmi_engine_get_programs(void *h, int turn0, int turns, double *rf_host) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e && rf_host, "engine: null handle");
+    return e->get_programs(turn0, turns, rf_host);
 }
 
 int blmi_engine_set_wake(void *h, int64_t n_mem, int64_t n_add, const double *time, double *memory,

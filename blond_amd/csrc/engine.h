@@ -8,6 +8,7 @@
 #include "particles.h"
 #include "spectral.h"
 #include "wake.h"
+#include "feedback.h"
 
 namespace blmi {
 
@@ -79,6 +80,17 @@ struct Engine {
     double *d_v1 = nullptr;        // [max(n_mem, n_slices)] convolution output
     double *d_indc = nullptr;      // [n_turns + 1] inductive coefficients (null: none)
     int deriv_mode = kDerivGradient;
+    // beam phase loop (feedback.h): one workgroup per turn after the induced
+    // voltage, before the particle kernel; it updates the next turn's
+    // omega_rf and phi_rf in d_rf
+    bool fb_on = false;
+    int fb_machine = 0, fb_delay = 0, fb_first_bin = 0, fb_n_on = 0;
+    double fb_alpha = 0, fb_reference = 0;
+    const double *fb_bc = nullptr;  // borrowed: bin centres
+    double *d_fb = nullptr;         // harmonic, omega_rf_d [nt][n_rf]; prog [nt][kFbCols];
+                                    // on_time [n_on]; state [kFbState + n_rf]
+    double *fb_h = nullptr, *fb_wd = nullptr, *fb_prog = nullptr, *fb_on_time = nullptr,
+           *fb_state = nullptr;
     // FP32 mode: float particles and programs; the slice axis (profile,
     // induced voltage, table) stays FP64 in engine-owned buffers and is
     // converted to the caller's float arrays at the end of every chunk
@@ -129,6 +141,13 @@ struct Engine {
                  const double *ind_coef_host, int deriv_mode);
     bool wake_on() const { return n_mem > 0 || d_indc != nullptr; }
     WakeArgs wake_args(int off) const;
+    int set_feedback(int machine, int delay, int first_bin, double alpha, double reference,
+                     const double *bc, const double *harmonic, const double *omega_d,
+                     const double *prog, const double *on_time, int n_on_time,
+                     const double *state);
+    int get_feedback_state(double *host);
+    int get_programs(int turn0, int turns, double *host);
+    int launch_feedback_turn(hipStream_t s, int off);
     int set_hist(int groups, int max_peel);
     int alloc_counts();
     int set_turn(int turn);

@@ -339,6 +339,9 @@ void Engine::release() {
     (void)hipFree(d_time);
     (void)hipFree(d_v1);
     (void)hipFree(d_indc);
+    (void)hipFree(d_fb);
+    d_fb = nullptr;
+    fb_on = false;
     d_trev = d_mem = d_time = d_v1 = d_indc = nullptr;
     n_mem = 0;
     mem_user = nullptr;
@@ -858,6 +861,99 @@ int Engine::launch_induced(hipStream_t s, int off, const KernelEvents &ev) {
     return launch_wake_finish(wake_args(off), s, e2);
 }
 
+int Engine::set_feedback(int machine, int delay, int first_bin, double alpha, double reference,
+                         const double *bc_dev, const double *harmonic, const double *omega_d,
+                         const double *prog, const double *on_time, int n_on_time,
+                         const double *state) {
+    (void)hipFree(d_fb);
+    d_fb = nullptr;
+    fb_on = false;
+    invalidate_graph();
+    if (machine < 0) return BLMI_OK;
+    if (machine > kFbPSB) return blmi_set_error(BLMI_ERR_ARG, "feedback: machine 0 (LHC), 1 (LHC_F) or 2 (PSB)");
+    if (f32) return blmi_set_error(BLMI_ERR_UNSUPPORTED, "feedback: FP64 particles only");
+    if (n_slices < 1 || !profile || !bc_dev)
+        return blmi_set_error(BLMI_ERR_ARG, "feedback: needs the profile and its bin centres");
+    if (first_bin < 0 || first_bin >= n_slices)
+        return blmi_set_error(BLMI_ERR_ARG, "feedback: first bin %d outside [0, %d)", first_bin, n_slices);
+    if (n_rf > 1024) return blmi_set_error(BLMI_ERR_ARG, "feedback: at most 1024 RF systems");
+    if (n_on_time < 0 || (n_on_time > 0 && !on_time) || !harmonic || !omega_d || !prog || !state)
+        return blmi_set_error(BLMI_ERR_ARG, "feedback: null program");
+    const size_t nt = (size_t)n_turns + 1;
+    const size_t nh = nt * (size_t)n_rf, np = nt * kFbCols, ns = (size_t)kFbState + n_rf;
+    const size_t total = 2 * nh + np + (size_t)n_on_time + ns;
+    BLMI_HIP_TRY(hipMalloc(&d_fb, sizeof(double) * total));
+    fb_h = d_fb;
+    fb_wd = fb_h + nh;
+    fb_prog = fb_wd + nh;
+    fb_on_time = fb_prog + np;
+    fb_state = fb_on_time + n_on_time;
+    BLMI_HIP_TRY(hipMemcpy(fb_h, harmonic, sizeof(double) * nh, hipMemcpyHostToDevice));
+    BLMI_HIP_TRY(hipMemcpy(fb_wd, omega_d, sizeof(double) * nh, hipMemcpyHostToDevice));
+    BLMI_HIP_TRY(hipMemcpy(fb_prog, prog, sizeof(double) * np, hipMemcpyHostToDevice));
+    if (n_on_time > 0)
+        BLMI_HIP_TRY(hipMemcpy(fb_on_time, on_time, sizeof(double) * n_on_time, hipMemcpyHostToDevice));
+    BLMI_HIP_TRY(hipMemcpy(fb_state, state, sizeof(double) * ns, hipMemcpyHostToDevice));
+    fb_machine = machine;
+    fb_delay = delay;
+    fb_first_bin = first_bin;
+    fb_alpha = alpha;
+    fb_reference = reference;
+    fb_n_on = n_on_time;
+    fb_bc = bc_dev;
+    fb_on = true;
+    return BLMI_OK;
+}
+
+int Engine::get_feedback_state(double *host) {
+    if (!fb_on) return blmi_set_error(BLMI_ERR_ARG, "feedback: not set");
+    BLMI_HIP_TRY(hipStreamSynchronize(own_stream));
+    BLMI_HIP_TRY(hipMemcpy(host, fb_state, sizeof(double) * ((size_t)kFbState + n_rf),
+                           hipMemcpyDeviceToHost));
+    return BLMI_OK;
+}
+
+int Engine::get_programs(int turn0, int turns, double *host) {
+    if (turn0 < 0 || turns < 0 || turn0 + turns > n_turns + 1)
+        return blmi_set_error(BLMI_ERR_ARG, "get_programs: turns [%d, %d) outside [0, %d]", turn0,
+                              turn0 + turns, n_turns + 1);
+    BLMI_HIP_TRY(hipStreamSynchronize(own_stream));
+    BLMI_HIP_TRY(hipMemcpy(host, d_rf + (size_t)turn0 * 3 * n_rf,
+                           sizeof(double) * 3 * n_rf * (size_t)turns, hipMemcpyDeviceToHost));
+    return BLMI_OK;
+}
+
+int Engine::launch_feedback_turn(hipStream_t s, int off) {
+    FeedbackArgs a{};
+    a.turn_base = d_base;
+    a.turn_off = off;
+    a.machine = fb_machine;
+    a.delay = fb_delay;
+    a.first_bin = fb_first_bin;
+    a.n_slices = n_slices;
+    a.n_rf = n_rf;
+    a.alpha = fb_alpha;
+    a.bin_size = (cut_right - cut_left) / (double)n_slices;
+    a.reference = fb_reference;
+    a.bc = fb_bc;
+    a.profile = profile;
+    // the plain map leaves the turn's histogram in the counts until the
+    // particle kernel publishes it; with intensity effects the induced
+    // voltage step has written the profile already
+    a.counts0 = d_counts;
+    a.counts1 = d_counts + (size_t)groups * n_slices;
+    a.pending = intensity ? nullptr : d_base + 1;
+    a.groups = groups;
+    a.rf = d_rf;
+    a.harmonic = fb_h;
+    a.omega_d = fb_wd;
+    a.prog = fb_prog;
+    a.on_time = fb_on_time;
+    a.n_on_time = fb_n_on;
+    a.state = fb_state;
+    return launch_feedback(a, s);
+}
+
 WakeArgs Engine::wake_args(int off) const {
     WakeArgs a{};
     a.turn_base = d_base;
@@ -955,6 +1051,10 @@ int Engine::launch_turns(hipStream_t s, int count, bool advance) {
             int rc = launch_induced(s, k, none);
             if (rc) return rc;
         }
+        if (fb_on) {
+            int rc = launch_feedback_turn(s, k);
+            if (rc) return rc;
+        }
         int rc = launch_turn_kernel(s, k, intensity, none);
         if (!rc) rc = launch_reduce(s, k + 1);
         if (!rc && ordered() && ++ord_since == sort_every) {
@@ -1025,6 +1125,7 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
     for (int k = 0; k < turns && rc == BLMI_OK; ++k) {
         KernelEvents ei{ev[4 * k], ev[4 * k + 1]}, et{ev[4 * k + 2], ev[4 * k + 3]};
         if (intensity) rc = launch_induced(own_stream, k, ei);
+        if (rc == BLMI_OK && fb_on) rc = launch_feedback_turn(own_stream, k);
         if (rc == BLMI_OK) rc = launch_turn_kernel(own_stream, k, intensity, et);
         if (rc == BLMI_OK) rc = launch_reduce(own_stream, k + 1);
         if (rc == BLMI_OK && ordered() && ++ord_since == sort_every) {

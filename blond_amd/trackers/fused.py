@@ -147,6 +147,9 @@ class FusedTracker:
         self.tracker = tracker
         self.profile = profile
         self.tiv = total_induced_voltage
+        self.fb = tracker.beamFB
+        if self.fb is not None and profile is None:
+            raise ValueError("the beam phase loop needs the profile")
         self.beam = tracker.beam
         self.rf = tracker.rf_params
         self.use_graph = bool(use_graph)
@@ -186,6 +189,8 @@ class FusedTracker:
                       _ptr(p.n_macroparticles))
         if self.tiv is not None:
             self._setup_induced()
+        if self.fb is not None:
+            self._setup_feedback()
         _lib.call("blmi_engine_set_launch", self._h, int(block), int(blocks_per_cu),
                   int(graph_chunk))
         _lib.call("blmi_engine_set_hist", self._h, int(groups), int(max_peel))
@@ -222,6 +227,10 @@ class FusedTracker:
 
     # ------------------------------------------------------------------
     def _phase_program(self):
+        if self.fb is not None:
+            # the device loop accumulates the phase offsets turn by turn
+            self._dphi_prog, self._phase_moves = None, False
+            return np.array(self.rf.host_array("phi_rf"), dtype=np.float64, copy=True)
         phi, self._dphi_prog, self._phase_moves = phase_program(self.rf)
         return phi
 
@@ -284,6 +293,42 @@ class FusedTracker:
         _lib.call("blmi_engine_set_wake", self._h, n_mem, n_add, time_ptr, mem_ptr,
                   _dptr(coef) if coef is not None else None, int(mode))
 
+    def _setup_feedback(self):
+        fb, rp = self.fb, self.rf
+        if self.f32:
+            raise NotImplementedError("the beam phase loop needs FP64 particles")
+        machine, prog, on_time = fb.device_programs(self.n_turns)
+        h = np.ascontiguousarray(np.asarray(rp.harmonic, dtype=np.float64).T)
+        wd = np.ascontiguousarray(np.asarray(rp.omega_rf_d, dtype=np.float64).T)
+        st = fb.device_state(rp.n_rf)
+        bc = self.profile.bin_centers
+        torch = self._torch
+        if not (hasattr(bc, "is_cuda") and bc.dtype == torch.float64 and bc.is_contiguous()):
+            bc = torch.as_tensor(np.asarray(_host_array(bc)), dtype=torch.float64,
+                                 device="cuda").contiguous()
+        self._fb_keep = (h, wd, prog, on_time, st, bc)
+        _lib.call("blmi_engine_set_feedback", self._h, int(machine), int(fb.delay),
+                  int(fb.first_bin()), float(fb.alpha), float(fb.reference), _ptr(bc), _dptr(h),
+                  _dptr(wd), _dptr(prog), _dptr(on_time) if len(on_time) else None,
+                  len(on_time), _dptr(st))
+
+    def _fetch_feedback(self, start: int, end: int):
+        """The loop's RF program rows start+1..end and its state, back to the
+        host objects (tracker.py:551-565 leaves them in RFStation)."""
+        rp, n_rf = self.rf, self.rf.n_rf
+        rows = np.empty((end - start, 3, n_rf), dtype=np.float64)
+        _lib.call("blmi_engine_get_programs", self._h, start + 1, end - start, _dptr(rows))
+        for name, k in (("omega_rf", 1), ("phi_rf", 2)):
+            host = rp.host_array(name)
+            host[:, start + 1:end + 1] = rows[:, k, :].T
+            if rp._device == "GPU":
+                dev = getattr(rp, name)
+                dev[:, start + 1:end + 1] = self._torch.from_numpy(
+                    np.ascontiguousarray(host[:, start + 1:end + 1])).to(dev.device)
+        st = np.empty(16 + n_rf, dtype=np.float64)
+        _lib.call("blmi_engine_get_feedback", self._h, _dptr(st))
+        self.fb.set_device_state(st)
+
     # ------------------------------------------------------------------
     def invalidate(self):
         """Call after changing beam.dt outside the tracker: the staged
@@ -305,6 +350,8 @@ class FusedTracker:
 
     def _advance(self, start: int, end: int):
         rp = self.rf
+        if self.fb is not None and end > start:
+            self._fetch_feedback(start, end)
         # host-side state the map leaves behind (tracker.py:556-615)
         if end > start and self._phase_moves:
             host_phi = rp.host_array("phi_rf")

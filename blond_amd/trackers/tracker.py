@@ -3,9 +3,10 @@
 ``RingAndRFTracker.track()`` calls ``bm.kick`` (or ``bm.linear_interp_kick``
 with interpolation) then ``bm.drift`` exactly as the reference; both are
 HIP kernels.  The per-turn scalars come from the host programs (no device
-synchronisation per turn).  Feedback loops, phase noise/modulation,
-periodicity and xsuite coupling are outside the tracking path and raise
-NotImplementedError.
+synchronisation per turn).  A beam phase loop (BeamFeedback) runs before the
+phase update as in the reference; phase noise/modulation, cavity and noise
+feedbacks, periodicity and xsuite coupling are outside the tracking path and
+raise NotImplementedError.
 """
 from __future__ import annotations
 
@@ -20,8 +21,8 @@ class RingAndRFTracker:
     def __init__(self, rf_station, beam, solver="simple", beam_feedback=None, noise_feedback=None,
                  cavity_feedback=None, periodicity=False, interpolation=False, with_xsuite=False,
                  profile=None, total_induced_voltage=None):
-        if beam_feedback is not None or noise_feedback is not None or cavity_feedback is not None:
-            raise NotImplementedError("RF feedback loops are outside the tracking path")
+        if noise_feedback is not None or cavity_feedback is not None:
+            raise NotImplementedError("noise and cavity feedback loops are outside the tracking path")
         if periodicity:
             raise NotImplementedError("periodicity=True is not supported")
         if with_xsuite:
@@ -36,7 +37,7 @@ class RingAndRFTracker:
                                "recognised!")
         if self.rf_params.alpha_order > 1:
             self.solver = "exact"
-        self.beamFB = None
+        self.beamFB = beam_feedback
         self.noiseFB = None
         self.cavityFB = None
         self.periodicity = False
@@ -89,6 +90,8 @@ class RingAndRFTracker:
     def track(self):
         """tracker.py:521-615."""
         turn = self.counter[0]
+        if self.beamFB is not None and turn >= self.beamFB.delay:
+            self.beamFB.track()
         self._update_phase(turn)
         if self.rf_params.empty is False:
             if self.interpolation:

@@ -33,6 +33,9 @@ class MI355XBackend:
         self.gradient = _k.gradient
         self.diff = _k.diff
         self.mtw_shift_add = _k.mtw_shift_add
+        self.beam_phase = _k.beam_phase
+        self.beam_phase_fast = _k.beam_phase_fast
+        self.strided_mean = _k.strided_mean
 
     def use_gpu(self, gpu_id: int = 0):
         """Select the GPU (bmath_backends.py:312-344)."""

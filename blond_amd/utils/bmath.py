@@ -14,6 +14,8 @@ numpy FFTs the backend keeps (blond/utils/bmath_backends.py:447-449):
     fast_resonator(R_S, Q, frequency_array, frequency_R, impedance=None) -> array
     rfft(a, n=0), irfft(a, n=0), rfftfreq(n, d=1.0)
     interp(x, xp, fp, left=None, right=None), gradient(f, h), diff(a)
+    beam_phase(bin_centers, profile, alpha, omega_rf, phi_rf, bin_size)
+    beam_phase_fast(bin_centers, profile, omega_rf, phi_rf, bin_size)
 
 Arrays are torch tensors resident on the GPU (the MI355X counterpart of the
 reference GPU backend's cupy arrays, bmath_backends.py:717-776).  numpy
@@ -318,6 +320,40 @@ def mtw_shift_add(memory, time, t_rev, voltage, n_add=None):
               ct.c_void_p(T.data_ptr()), n, float(t_rev),
               ct.c_void_p(V.data_ptr()) if V is not None else None, k, _stream(torch))
     return _like(torch, out, memory)
+
+
+def beam_phase(bin_centers, profile, alpha, omega_rf, phi_rf, bin_size) -> float:
+    """beam_phase.cpp:6-39 via butils_wrap_cpp.py:785-808: scoeff / ccoeff of
+    the exponentially windowed profile (float64, one synchronisation)."""
+    torch = _torch()
+    bc, p = _f64(torch, bin_centers, "bin_centers"), _f64(torch, profile, "profile")
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    _lib.call("blmi_beam_phase", ct.c_void_p(bc.data_ptr()), ct.c_void_p(p.data_ptr()),
+              int(p.numel()), float(alpha), float(omega_rf), float(phi_rf), float(bin_size),
+              ct.c_void_p(out.data_ptr()), _stream(torch))
+    return float(out.item())
+
+
+def beam_phase_fast(bin_centers, profile, omega_rf, phi_rf, bin_size) -> float:
+    """beam_phase.cpp:42-65 via butils_wrap_cpp.py:811-832 (no window)."""
+    torch = _torch()
+    bc, p = _f64(torch, bin_centers, "bin_centers"), _f64(torch, profile, "profile")
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    _lib.call("blmi_beam_phase_fast", ct.c_void_p(bc.data_ptr()), ct.c_void_p(p.data_ptr()),
+              int(p.numel()), float(omega_rf), float(phi_rf), float(bin_size),
+              ct.c_void_p(out.data_ptr()), _stream(torch))
+    return float(out.item())
+
+
+def strided_mean(x, stride: int = 1) -> float:
+    """numpy.mean(x[::stride]) in float64 with a fixed summation order (the
+    radial loop's mean energy offset, beam_feedback.py:375-378)."""
+    torch = _torch()
+    X = _f64(torch, x, "x")
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    _lib.call("blmi_strided_mean", ct.c_void_p(X.data_ptr()), int(X.numel()), int(stride),
+              ct.c_void_p(out.data_ptr()), _stream(torch))
+    return float(out.item())
 
 
 def beam_statistics(dt, dE, id=None):

@@ -19,6 +19,8 @@
       narrowband resonators drawn as in c2 (n_fft = 1024).  Seeds: counts,
       V, phi from default_rng(seed); bunch b from bigaussian(seed*1e5 + 2b),
       its resonators from default_rng(seed*1e5 + 5e4 + b).
+  feedback workloads (build_feedback_workload): a displaced bunch under a
+      beam phase loop; see that function.
   wake workloads (build_wake_workload): a c2-ring bunch (intensity 1e12,
       256 slices over [0, t_rf]) with a multi-turn wake or an inductive
       source; see that function.
@@ -40,6 +42,7 @@ from .impedances.impedance import (InducedVoltageFreq, InducedVoltageTime, Induc
                                    TotalInducedVoltage)
 from .impedances.impedance_sources import Resonators
 from .input_parameters.rf_parameters import RFStation
+from .llrf.beam_feedback import BeamFeedback
 from .input_parameters.ring import Ring
 from .trackers.tracker import RingAndRFTracker
 
@@ -124,6 +127,9 @@ class Workload:
             st.update(n_slices=p.n_slices, cut_left=float(p.cut_left), cut_right=float(p.cut_right),
                       bin_centers=_h(p.bin_centers).copy(),
                       profile=_h(p.n_macroparticles).copy())
+        fb = self.tracker.beamFB
+        if fb is not None:
+            st["feedback"] = _feedback_state(fb, rp, self.profile)
         if self.tiv is not None:
             srcs = self.tiv.induced_voltage_list
             conv = [s for s in srcs if isinstance(s, (InducedVoltageFreq, InducedVoltageTime))]
@@ -143,6 +149,42 @@ class Workload:
 
     def oracle_copy(self) -> dict:
         return self.snapshot()
+
+
+def _feedback_state(fb, rp, profile) -> dict:
+    """Host copy of a BeamFeedback's configuration and state for a CPU
+    restatement of the loop."""
+    ring = fb.ring
+    d = dict(machine=fb.machine, alpha=float(fb.alpha), first_bin=fb.first_bin(),
+             delay=int(fb.delay), reference=float(fb.reference),
+             bin_size=float(profile.bin_size),
+             phi_s=np.array(rp.phi_s, dtype=np.float64),
+             harmonic=np.array(rp.harmonic, dtype=np.float64),
+             omega_rf_d=np.array(rp.omega_rf_d, dtype=np.float64),
+             dphi_rf=np.array(rp.dphi_rf, dtype=np.float64),
+             phi_beam=float(fb.phi_beam), dphi=float(fb.dphi), domega_rf=float(fb.domega_rf),
+             alpha_0=np.array(ring.alpha_0[0], dtype=np.float64),
+             gamma=np.array(rp.gamma, dtype=np.float64))
+    if fb.machine == "PSB":
+        d.update(gain=np.array(fb.gain, dtype=np.float64),
+                 gain2=[np.array(fb.gain2[0], dtype=np.float64),
+                        np.array(fb.gain2[1], dtype=np.float64)],
+                 on_time=np.array(fb.on_time, dtype=np.float64), PL_counter=int(fb.PL_counter),
+                 dphi_sum=float(fb.dphi_sum), dphi_av=float(fb.dphi_av),
+                 dphi_av_prev=float(fb.dphi_av_prev), domega_PL=float(fb.domega_PL),
+                 domega_RL=float(fb.domega_RL), dR_over_R=float(fb.dR_over_R),
+                 dR_over_R_prev=float(fb.dR_over_R_prev))
+    else:
+        d.update(gain=float(fb.gain), gain2=float(fb.gain2))
+    if fb.machine == "LHC":
+        d.update(lhc_a=np.array(fb.lhc_a, dtype=np.float64),
+                 lhc_t=np.array(fb.lhc_t, dtype=np.float64), lhc_y=float(fb.lhc_y))
+    if fb.machine == "SPS_RL":
+        d.update(sample_dE=int(fb.sample_dE), ring_radius=float(ring.ring_radius),
+                 beta=np.array(ring.beta[0], dtype=np.float64),
+                 energy=np.array(ring.energy[0], dtype=np.float64),
+                 eta_0=np.array(rp.eta_0, dtype=np.float64))
+    return d
 
 
 def _source_state(s) -> dict:
@@ -281,6 +323,77 @@ def build_wake_workload(kind: str, n_particles: int = 50_000, n_turns: int = 40,
     w.meta.update(n_particles=n_particles, n_turns=n_turns, n_slices=n_slices)
     if to_gpu:
         w.to_gpu()
+    return w
+
+
+FEEDBACK_KINDS = ("LHC", "LHC_SL", "LHC_F", "PSB", "SPS_RL")
+
+
+def build_feedback_workload(kind: str, n_particles: int = 50_000, n_turns: int = 60,
+                            n_slices: int = 1024, intensity: bool = False, seed: int = 1234,
+                            to_gpu: bool = True) -> Workload:
+    """A bunch displaced from the synchronous phase under a beam phase loop.
+
+      LHC, LHC_F, SPS_RL: the SPS-like case of the reference's loop test
+          (unittests/llrf/test_beam_feedback.py:24-60): 6911.5038 m,
+          gamma_t 17.95142852, 25.92 GeV/c, h = 4620, 4.5 MV, Gaussian
+          sigma = 0.05 ns moved by 0.1 ns, n_slices over [0, t_rf];
+          PL_gain 1000/s; LHC_F adds a frequency loop (FL_gain 0.05 per turn),
+          SPS_RL a radial loop (RL_gain 1e4, every 7th particle).
+      LHC_SL: the c1 LHC ring (h = 35640, 6 MV), bunch 4 sigma = 1 ns moved
+          by 0.1 ns, phase and synchro loops as EX_28_phase_loop_lhc.py
+          (PL_gain 1/(5 t_rev), SL_gain PL_gain/10).
+      PSB: the c2 ring (157.08 m, 310 MeV/c, h = 1, 8 kV, phi = pi) with
+          the loop of EX_08_Phase_Loop.py:78 (PL_gain 1/25 us, period
+          10 us) and a radial loop (RL_gain [1e-3, 1e2]), bunch of 100 ns
+          rms moved by 50 ns.
+    Map without intensity: tracker (with the loop), profile.  With
+    ``intensity`` a broadband resonator adds induced voltage and the map is
+    profile, induced voltage, tracker."""
+    if kind not in FEEDBACK_KINDS:
+        raise ValueError(f"unknown feedback workload {kind!r}")
+    if kind == "PSB":
+        ring = Ring(157.08, 1 / 4.4 ** 2, 310e6, Proton(), n_turns)
+        rf = RFStation(ring, [1], [8e3], [np.pi])
+        beam = Beam(ring, n_particles, 1e12)
+        bigaussian(ring, rf, beam, sigma_dt=C2_SIGMA_DT, seed=seed)
+        offset = 50e-9
+        config = {"machine": "PSB", "PL_gain": 1.0 / 25.0e-6, "period": 10.0e-6,
+                  "RL_gain": [1e-3, 1e2]}
+    elif kind == "LHC_SL":
+        ring = Ring(26658.883, 1 / 55.759505 ** 2, 450e9, Proton(), n_turns)
+        rf = RFStation(ring, [35640], [6e6], [0.0])
+        beam = Beam(ring, n_particles, 1.15e11 if intensity else 1e9)
+        bigaussian(ring, rf, beam, sigma_dt=1.0e-9 / 4, seed=seed)
+        offset = 0.1e-9
+        pl = 1 / (5 * float(ring.t_rev[0]))
+        config = {"machine": "LHC", "PL_gain": pl, "SL_gain": pl / 10}
+    else:
+        ring = Ring(6911.5038, 1.0 / 17.95142852 ** 2, 25.92e9, Proton(), n_turns)
+        rf = RFStation(ring, [4620], [4.5e6], [0.0])
+        beam = Beam(ring, n_particles, 1.2e11 if intensity else 1.2e6)
+        bigaussian(ring, rf, beam, sigma_dt=0.05e-9, seed=seed)
+        offset = 0.1e-9
+        config = {"machine": kind, "PL_gain": 1000.0}
+        if kind == "LHC_F":
+            config["FL_gain"] = 0.05
+        elif kind == "SPS_RL":
+            config.update(RL_gain=1e4, sample_dE=7)
+    beam.dt += offset
+    prof = _profile(beam, 0.0, float(rf.t_rf[0, 0]), n_slices)
+    fb = BeamFeedback(ring, rf, prof, config)
+    tiv = None
+    if intensity:
+        res = Resonators([5e5], [1e9], [1.0])
+        ivf = InducedVoltageFreq(beam, prof, [res],
+                                 frequency_resolution=(1 + 1e-9) / (4 * n_slices * prof.bin_size))
+        tiv = TotalInducedVoltage(beam, prof, [ivf])
+    tr = RingAndRFTracker(rf, beam, beam_feedback=fb, profile=prof)
+    w = Workload(f"fb_{kind}", ring, rf, beam, tr, prof, tiv, intensity)
+    w.meta.update(n_particles=n_particles, n_turns=n_turns, n_slices=n_slices)
+    if to_gpu:
+        w.to_gpu()
+        prof.track()  # the loop reads the profile of the current coordinates
     return w
 
 

@@ -158,6 +158,23 @@ int blmi_diff(const double *f, int64_t n, double *out, void *stream);
 int blmi_mtw_shift_add(const double *mem_in, double *mem_out, const double *time, int64_t n_mem,
                        double t_rev, const double *add, int64_t n_add, void *stream);
 
+/* Beam phase of a profile: *coeff = scoeff / ccoeff, the trapezoid sums of
+ * exp(alpha bc) p sin(w bc + phi) and ... cos(...) (beam_phase) or of
+ * p sin / p cos (beam_phase_fast); coeff is a device double.  Replaces
+ * beam_phase() / beam_phase_fast(), blond/cpp_routines/beam_phase.cpp:6-65
+ * (bm.beam_phase, butils_wrap_cpp.py:785-832).  Sines, cosines and the
+ * exponential are the VDT (Cephes) functions the reference uses; the
+ * interior sum runs in a fixed tree order. */
+int blmi_beam_phase(const double *bin_centers, const double *profile, int n_bins, double alpha,
+                    double omega_rf, double phi_rf, double bin_size, double *coeff, void *stream);
+int blmi_beam_phase_fast(const double *bin_centers, const double *profile, int n_bins,
+                         double omega_rf, double phi_rf, double bin_size, double *coeff,
+                         void *stream);
+
+/* *out = mean(x[::stride]) (device double), fixed summation order: the
+ * radial loop's bm.mean(beam.dE[::sample_dE]), beam_feedback.py:375-378. */
+int blmi_strided_mean(const double *x, int64_t n, int64_t stride, double *out, void *stream);
+
 /* Fused turn engine.  Runs the reference's map
  *   with intensity effects:  [Profile, TotalInducedVoltage, RingAndRFTracker]
  *                            (EX_18_robinson_instability.py:226; tracker.py:521-615,
@@ -212,6 +229,34 @@ int blmi_engine_set_induced(void *engine, int64_t n_fft, const double *Z_host, d
  * FP64 particles only. */
 int blmi_engine_set_wake(void *engine, int64_t n_mem, int64_t n_add, const double *time,
                          double *memory, const double *ind_coef_host, int deriv_mode);
+/* Beam phase loop of the fused turn (BeamFeedback with RingAndRFTracker,
+ * blond/llrf/beam_feedback.py:236-251, 275-367, 414-441, 514-622;
+ * tracker.py:551-565): one workgroup per turn, after the induced voltage and
+ * before the particle kernel, computes the beam phase of the turn's profile
+ * over bins [first_bin, n_slices) (window coefficient alpha), the machine's
+ * frequency correction (machine 0 LHC, 1 LHC_F, 2 PSB; -1 switches the loop
+ * off), the next turn's omega_rf and the accumulated phase offset into the
+ * next turn's phi_rf, in place in the engine's RF program.  The programs of
+ * set_programs must then hold phi_rf without the accumulated offsets.
+ *   bin_centers: device [n_slices];
+ *   harmonic_host, omega_rf_d_host: [n_turns+1][n_rf];
+ *   prog_host: [n_turns+1][8] = PL gain, gain2 (SL / FL / PSB RL P), PSB RL I,
+ *              LHC a, LHC tau, phi_s, alpha_0, gamma per turn;
+ *   on_time_host: PSB loop turns (beam_feedback.py:253-273);
+ *   state_host: [16 + n_rf] = lhc_y, dphi_sum, dphi_av, dphi_av_prev,
+ *              domega_PL, domega_RL, dR_over_R, dR_over_R_prev, PL_counter,
+ *              domega_rf, phi_beam, dphi, 4 spare, then dphi_rf[n_rf].
+ * FP64 particles only; turns the bin-ordered mode off.
+ * blmi_engine_get_feedback copies the state back (synchronous);
+ * blmi_engine_get_programs copies `turns` rows [3][n_rf] of the RF program
+ * from turn0 (synchronous). */
+int blmi_engine_set_feedback(void *engine, int machine, int delay, int first_bin, double alpha,
+                             double reference, const double *bin_centers,
+                             const double *harmonic_host, const double *omega_rf_d_host,
+                             const double *prog_host, const double *on_time_host, int n_on_time,
+                             const double *state_host);
+int blmi_engine_get_feedback(void *engine, double *state_host);
+int blmi_engine_get_programs(void *engine, int turn0, int turns, double *rf_host);
 /* Launch shape of the particle kernel (block in {64..1024}, workgroups per CU)
  * and the number of turns per captured graph. */
 int blmi_engine_set_launch(void *engine, int block, int blocks_per_cu, int graph_chunk);

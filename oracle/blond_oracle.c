@@ -25,6 +25,9 @@
  *   linear_interp_kick   golden vectors; numpy.interp closed form.
  *   rf_volt_comp         golden vectors; closed form.
  *   fast_resonator_real_imag  golden vectors; complex closed form R/(1+jQ(f/fr-fr/f)).
+ *   beam_phase, beam_phase_fast  the reference build run on one thread (bit
+ *                        for bit); the closed form tan(w t + phi) of a one-bin
+ *                        profile; cos and exp against libm.
  */
 #include <math.h>
 #include <stdint.h>
@@ -101,6 +104,110 @@ float oracle_sinf(float x) {
     if (flip) out = -out;
     if (x < 0.0f) out = -out;
     return out;
+}
+
+/* Cosine with the sine's reduction and polynomials (VDT fast_sincos,
+ * cpp_routines/sincos.h:132-160, cos.h:22): the other polynomial of the
+ * octant pair, negated when bit 2 of (j - 2) is clear. */
+double oracle_cos(double x) {
+    double ax = fabs(x);
+    int oct = (int)(ORC_PIO4_INV * ax);
+    oct = (oct + 1) & ~1;
+    double y = (double)oct;
+    double r = ((ax - y * ORC_DP1) - y * ORC_DP2) - y * ORC_DP3;
+    double zz = r * r;
+    double s = r + r * zz * oracle_poly6(ORC_SIN, zz);
+    double c = 1.0 - zz * 0.5 + zz * zz * oracle_poly6(ORC_COS, zz);
+    int use_sin = ((oct - 2) & 2) == 0;
+    double out = use_sin ? s : c;
+    if (((oct - 2) & 4) == 0) out = -out;
+    return out;
+}
+
+/* Exponential: Cephes exp.c as arranged in VDT (cpp_routines/exp.h:70-113):
+ * n = trunc(y) - signbit((float)y) with y = x log2(e) + 1/2, x - n C1 - n C2,
+ * 1 + 2 x P / (Q - x P) with Cephes's P(x^2), Q(x^2), times 2^n. */
+double oracle_exp(double x0) {
+    double x = x0;
+    const double t = 1.4426950408889634073599 * x + 0.5;
+    float tf = (float)t;
+    uint32_t tb;
+    memcpy(&tb, &tf, sizeof tb);
+    double px = (double)((int32_t)t - (int32_t)(tb >> 31));
+    const int32_t n = (int32_t)px;
+    x -= px * 6.93145751953125E-1;
+    x -= px * 1.42860682030941723212E-6;
+    const double xx = x * x;
+    px = 1.26177193074810590878E-4;
+    px *= xx;
+    px += 3.02994407707441961300E-2;
+    px *= xx;
+    px += 9.99999999999999999910E-1;
+    px *= x;
+    double qx = 3.00198505138664455042E-6;
+    qx *= xx;
+    qx += 2.52448340349684104192E-3;
+    qx *= xx;
+    qx += 2.27265548208155028766E-1;
+    qx *= xx;
+    qx += 2.00000000000000000009E0;
+    x = px / (qx - px);
+    x = 1.0 + 2.0 * x;
+    uint64_t bits = (((uint64_t)(int64_t)n) + 1023) << 52;
+    double scale;
+    memcpy(&scale, &bits, sizeof scale);
+    x *= scale;
+    if (x0 > 708.0) x = INFINITY;
+    if (x0 < -708.0) x = 0.0;
+    return x;
+}
+
+/* trapz_const_delta (blondmath.cpp:496-512) as its OpenMP reduction runs on
+ * one thread: the interior summed into a zero-initialised private copy,
+ * which is then added to (f[0] + f[n-1]) / 2. */
+static double oracle_trapz_const_delta(const double *f, double dx, int n) {
+    double psum = (f[0] + f[n - 1]) / 2.;
+    double priv = 0.0;
+    for (int i = 1; i < n - 1; ++i) priv += f[i];
+    psum = psum + priv;
+    return dx * psum;
+}
+
+/* beam_phase.cpp:6-39 */
+double beam_phase(const double *bin_centers, const double *profile, double alpha,
+                  double omega_rf, double phi_rf, double bin_size, int n_bins) {
+    if (n_bins < 1) return NAN;
+    double *a1 = (double *)malloc(sizeof(double) * (size_t)n_bins);
+    double *a2 = (double *)malloc(sizeof(double) * (size_t)n_bins);
+    for (int i = 0; i < n_bins; ++i) {
+        const double base = oracle_exp(alpha * bin_centers[i]) * profile[i];
+        const double a = omega_rf * bin_centers[i] + phi_rf;
+        a1[i] = base * oracle_sin(a);
+        a2[i] = base * oracle_cos(a);
+    }
+    const double s = oracle_trapz_const_delta(a1, bin_size, n_bins);
+    const double c = oracle_trapz_const_delta(a2, bin_size, n_bins);
+    free(a1);
+    free(a2);
+    return s / c;
+}
+
+/* beam_phase.cpp:42-65 */
+double beam_phase_fast(const double *bin_centers, const double *profile, double omega_rf,
+                       double phi_rf, double bin_size, int n_bins) {
+    if (n_bins < 1) return NAN;
+    double *a1 = (double *)malloc(sizeof(double) * (size_t)n_bins);
+    double *a2 = (double *)malloc(sizeof(double) * (size_t)n_bins);
+    for (int i = 0; i < n_bins; ++i) {
+        const double a = omega_rf * bin_centers[i] + phi_rf;
+        a1[i] = profile[i] * oracle_sin(a);
+        a2[i] = profile[i] * oracle_cos(a);
+    }
+    const double s = oracle_trapz_const_delta(a1, bin_size, n_bins);
+    const double c = oracle_trapz_const_delta(a2, bin_size, n_bins);
+    free(a1);
+    free(a2);
+    return s / c;
 }
 
 /* kick.cpp:16-35 : dE[i] += charge * sum_j V_j sin(w_j dt_i + phi_j) + acc */

@@ -113,6 +113,23 @@ class Lib:
                               _ptr(out, self.ptr))
         return out
 
+    # beam_phase.cpp:6-39 and 42-65 (double only)
+    def beam_phase(self, bin_centers, profile, alpha, omega_rf, phi_rf, bin_size):
+        bc = np.ascontiguousarray(bin_centers, dtype=np.float64)
+        p = np.ascontiguousarray(profile, dtype=np.float64)
+        f = self.lib.beam_phase
+        f.restype = ct.c_double
+        return f(_ptr(bc), _ptr(p), ct.c_double(alpha), ct.c_double(omega_rf),
+                 ct.c_double(phi_rf), ct.c_double(bin_size), ct.c_int(len(p)))
+
+    def beam_phase_fast(self, bin_centers, profile, omega_rf, phi_rf, bin_size):
+        bc = np.ascontiguousarray(bin_centers, dtype=np.float64)
+        p = np.ascontiguousarray(profile, dtype=np.float64)
+        f = self.lib.beam_phase_fast
+        f.restype = ct.c_double
+        return f(_ptr(bc), _ptr(p), ct.c_double(omega_rf), ct.c_double(phi_rf),
+                 ct.c_double(bin_size), ct.c_int(len(p)))
+
     # fast_resonator.cpp:19-73 (double only)
     def fast_resonator(self, R_S, Q, freq, f_r):
         R = np.ascontiguousarray(R_S, dtype=np.float64)
@@ -246,6 +263,69 @@ def induced_voltage_sources(state: dict, profile, turn: int, memories: list):
     return np.ascontiguousarray(total, dtype=np.float64)
 
 
+def phase_loop_turn(fb: dict, t: int, omega, phi, profile, bin_centers, bin_size, lib, dE=None):
+    """The machine function of BeamFeedback.track at turn t
+    (beam_feedback.py:275-367, 414-622) on the loop state ``fb`` (updated in
+    place); returns domega_rf.  The beam phase is beam_phase.cpp through
+    ``lib``; omega / phi are the [n_rf, n_turns+1] programs as modified so far."""
+    m = fb["machine"]
+    b0 = fb["first_bin"]
+    coeff = lib.beam_phase(bin_centers[b0:], profile[b0:], fb["alpha"], omega[0, t], phi[0, t],
+                           bin_size)
+    fb["phi_beam"] = np.arctan(coeff) + np.pi
+    fb["dphi"] = fb["phi_beam"] - fb["phi_s"][t]
+    dphi = fb["dphi"]
+    if m == "LHC":
+        x = fb["dphi_rf"][0] + fb["reference"]
+        fb["domega_rf"] = -fb["gain"] * dphi - fb["gain2"] * (fb["lhc_y"] + fb["lhc_a"][t] * x)
+        fb["lhc_y"] = (1 - fb["lhc_t"][t]) * fb["lhc_y"] + (1 - fb["lhc_a"][t]) * fb["lhc_t"][t] * x
+    elif m == "LHC_F":
+        fb["domega_rf"] = -fb["gain"] * dphi - fb["gain2"] * (
+            omega[0, t] - fb["omega_rf_d"][0, t] + fb["reference"])
+    elif m == "SPS_RL":
+        avg = np.mean(dE[:: fb["sample_dE"]])
+        drho = (fb["alpha_0"][t] * fb["ring_radius"] * avg
+                / (fb["beta"][t] ** 2.0 * fb["energy"][t]))
+        domega_dphi = -fb["gain"] * dphi
+        domega_dR = (-np.sign(fb["eta_0"][t]) * fb["gain2"] * (fb["reference"] - drho)
+                     / fb["ring_radius"])
+        fb["domega_rf"] = domega_dphi + domega_dR
+    elif m == "PSB":
+        fb["dphi_sum"] += dphi
+        pc = fb["PL_counter"]
+        on = fb["on_time"]
+        if t == on[pc] and t >= fb["delay"]:
+            fb["dphi_av"] = fb["dphi_sum"] / (on[pc] - on[pc - 1])
+            fb["domega_PL"] = 0.99803799 * fb["domega_PL"] + fb["gain"][t] * (
+                0.99901903 * fb["dphi_av"] - 0.99901003 * fb["dphi_av_prev"])
+            fb["dphi_av_prev"] = fb["dphi_av"]
+            fb["dphi_sum"] = 0.0
+            wd = fb["omega_rf_d"][0, t]
+            fb["dR_over_R"] = (omega[0, t] - wd) / (
+                wd * (1.0 / (fb["alpha_0"][t] * fb["gamma"][t] ** 2) - 1.0))
+            fb["domega_RL"] = (fb["domega_RL"] + fb["gain2"][0][t]
+                               * (fb["dR_over_R"] - fb["dR_over_R_prev"])
+                               + fb["gain2"][1][t] * fb["dR_over_R"])
+            fb["dR_over_R_prev"] = fb["dR_over_R"]
+            fb["PL_counter"] = pc + 1
+        fb["domega_rf"] = -fb["domega_PL"] - fb["domega_RL"]
+    else:
+        raise ValueError(f"machine {m!r}")
+    return fb["domega_rf"]
+
+
+def phase_update(fb: dict, t: int, omega, phi, domega):
+    """beam_feedback.py:246-251 (when the loop ran: domega is not None) then
+    tracker.py:551-565: next turn's frequency, accumulated phase offset and
+    next turn's phase."""
+    h = fb["harmonic"]
+    if domega is not None:
+        omega[:, t + 1] += domega * h[:, t + 1] / h[0, t + 1]
+    wd = fb["omega_rf_d"]
+    fb["dphi_rf"] += 2.0 * np.pi * h[:, t + 1] * (omega[:, t + 1] - wd[:, t + 1]) / wd[:, t + 1]
+    phi[:, t + 1] += fb["dphi_rf"]
+
+
 def resonator_impedance(freq, R_S, f_r, Q):
     """Closed form Z(f) = sum_k R_k / (1 + j Q_k (f/f_rk - f_rk/f)), Z(0) = 0."""
     z = np.zeros(len(freq), dtype=np.complex128)
@@ -271,6 +351,15 @@ def track(state: dict, n_turns: int, lib: Lib | None = None) -> dict:
     prof = np.array(state["profile"], dtype=r, copy=True) if "profile" in state else None
     volt_ind = np.array(state["induced_voltage"], copy=True) if state.get("intensity") else None
     t0 = int(state["turn"])
+    fb = state.get("feedback")
+    if fb is not None:
+        # the loop mutates the RF programs and its own state
+        fb = {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v)
+              for k, v in fb.items()}
+        omega_p = np.array(state["omega_rf"], dtype=np.float64, copy=True)
+        phi_p = np.array(state["phi_rf"], dtype=np.float64, copy=True)
+    else:
+        omega_p, phi_p = state["omega_rf"], state["phi_rf"]
     sources = state.get("sources")
     memories = [s.get("memory") for s in sources] if sources else None
     for t in range(t0, t0 + n_turns):
@@ -283,7 +372,14 @@ def track(state: dict, n_turns: int, lib: Lib | None = None) -> dict:
                                                  state["charge"], state["ratio"],
                                                  state["n_induced_voltage"], state["n_slices"])
             lib.linear_interp_kick(dt, dE, volt_ind, state["bin_centers"], state["charge"], 0.0)
-        lib.kick(dt, dE, state["voltage"][:, t], state["omega_rf"][:, t], state["phi_rf"][:, t],
+        if fb is not None:
+            # RingAndRFTracker.track (tracker.py:551-565) before the kick
+            domega = None
+            if t >= fb["delay"]:
+                domega = phase_loop_turn(fb, t, omega_p, phi_p, prof, state["bin_centers"],
+                                         fb["bin_size"], fb.get("lib") or lib, dE)
+            phase_update(fb, t, omega_p, phi_p, domega)
+        lib.kick(dt, dE, state["voltage"][:, t], omega_p[:, t], phi_p[:, t],
                  state["charge"], state["n_rf"], state["acc"][t])
         k = t + 1
         lib.drift(dt, dE, state["solver"], state["t_rev"][k], state["length_ratio"],
@@ -300,4 +396,6 @@ def track(state: dict, n_turns: int, lib: Lib | None = None) -> dict:
     if sources:
         st["sources"] = [dict(s, memory=m) if m is not None else dict(s)
                          for s, m in zip(sources, memories)]
+    if fb is not None:
+        st.update(feedback=fb, omega_rf=omega_p, phi_rf=phi_p)
     return st

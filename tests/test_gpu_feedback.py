@@ -1,0 +1,149 @@
+"""GPU parity of the beam phase loop: the beam phase reduction against the
+oracle's (reference-order) beam_phase, and whole turns with the LHC, LHC
+with synchro loop, LHC_F, PSB and SPS_RL loops -- object by object and in
+the fused turn (one loop workgroup per turn) -- against the oracle's turn
+loop: coordinates, profiles, the RF programs the loop rewrites and the loop
+state."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("no GPU", allow_module_level=True)
+
+from blond_amd.utils import bmath as bm  # noqa: E402
+from blond_amd.workloads import build_feedback_workload  # noqa: E402
+from tests.test_gpu_tracking import _check_intensity, host  # noqa: E402
+
+ORC = orc.oracle_lib()
+FUSED_KINDS = ("LHC", "LHC_SL", "LHC_F", "PSB")
+
+
+def _phase_tol(bc, p, alpha, w, phi, coeff):
+    """Rounding bound of scoeff/ccoeff when the two sums are evaluated in
+    different orders: a few ulps times their condition numbers."""
+    base = np.exp(alpha * bc) * p
+    fs, fc = base * np.sin(w * bc + phi), base * np.cos(w * bc + phi)
+    cond = np.abs(fs).sum() / abs(fs.sum()) + np.abs(fc).sum() / abs(fc.sum())
+    return 8 * np.finfo(float).eps * cond * abs(coeff) + 1e-300
+
+
+@pytest.mark.parametrize("n", [1, 2, 100, 1024, 65536])
+@pytest.mark.parametrize("shape", ["random", "bunch"])
+def test_beam_phase_vs_oracle(n, shape):
+    rng = np.random.default_rng(n)
+    bc = (np.arange(n) + 0.5) * (2.5e-9 / n)
+    if shape == "random":
+        p = rng.integers(0, 300, n).astype(np.float64)
+    else:
+        p = np.round(1e4 * np.exp(-0.5 * ((bc - 1.4e-9) / 0.2e-9) ** 2))
+    w = 2 * np.pi / 2.5e-9
+    for alpha in (0.0, -2e8):
+        a = bm.beam_phase(torch.from_numpy(bc).cuda(), torch.from_numpy(p).cuda(), alpha, w,
+                          0.2, 2.5e-9 / n)
+        b = ORC.beam_phase(bc, p, alpha, w, 0.2, 2.5e-9 / n)
+        assert abs(a - b) <= _phase_tol(bc, p, alpha, w, 0.2, b)
+        if shape == "bunch" and n > 2:
+            assert a == pytest.approx(b, rel=1e-13)
+    a = bm.beam_phase_fast(bc, p, w, 0.2, 2.5e-9 / n)
+    b = ORC.beam_phase_fast(bc, p, w, 0.2, 2.5e-9 / n)
+    assert abs(a - b) <= _phase_tol(bc, p, 0.0, w, 0.2, b)
+
+
+def test_strided_mean():
+    x = np.random.default_rng(3).standard_normal(100003) * 1e6
+    for s in (1, 7, 100003, 200000):
+        assert bm.strided_mean(torch.from_numpy(x).cuda(), s) == pytest.approx(
+            np.mean(x[::s]), rel=1e-13, abs=1e-9)
+
+
+def _check(w, ref):
+    """Coordinates and profiles as in the intensity turns; RF programs and
+    the loop's outputs to 1e-12 relative (the beam phase sums in a different
+    order than the reference's reduction)."""
+    if w.tiv is not None:
+        _check_intensity(w, ref, 1e-9)
+    else:
+        dt, dE = host(w.beam.dt), host(w.beam.dE)
+        for a, b in ((dt, ref["dt"]), (dE, ref["dE"])):
+            sc = np.abs(b).max()
+            err = np.abs(a - b) / sc
+            assert np.mean(err > 1e-9) <= 1e-3
+            assert err.max() <= 1e-6
+            assert abs(a.mean() - b.mean()) <= 1e-9 * max(abs(b.mean()), b.std())
+        dp = np.abs(host(w.profile.n_macroparticles) - ref["profile"]).sum()
+        assert dp <= 8
+    rp = w.rf
+    t = rp.counter[0]
+    for name in ("omega_rf", "phi_rf"):
+        np.testing.assert_allclose(rp.host_array(name)[:, : t + 1], ref[name][:, : t + 1],
+                                   rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(host(getattr(rp, name))[:, : t + 1], ref[name][:, : t + 1],
+                                   rtol=1e-12, atol=1e-12)
+    fb, rf_ = w.tracker.beamFB, ref["feedback"]
+    assert fb.phi_beam == pytest.approx(rf_["phi_beam"], rel=1e-12)
+    sc = max(abs(rf_["domega_rf"]), 1e-3)
+    assert abs(fb.domega_rf - rf_["domega_rf"]) <= 1e-7 * sc
+    np.testing.assert_allclose(rp.dphi_rf, rf_["dphi_rf"], rtol=1e-10, atol=1e-12)
+
+
+def _ref(w, turns):
+    st = w.snapshot()
+    return orc.track(st, turns, ORC)
+
+
+@pytest.mark.parametrize("kind", FUSED_KINDS + ("SPS_RL",))
+def test_feedback_turns_object_by_object(kind):
+    w = build_feedback_workload(kind, n_particles=40001, n_turns=40)
+    ref = _ref(w, 25)
+    for _ in range(25):
+        for m in w.map():
+            m.track()
+    _check(w, ref)
+
+
+@pytest.mark.parametrize("kind", FUSED_KINDS)
+def test_feedback_turns_fused(kind):
+    w = build_feedback_workload(kind, n_particles=40001, n_turns=60)
+    ref = _ref(w, 45)
+    w.fused(45)
+    _check(w, ref)
+
+
+@pytest.mark.parametrize("kind", ("LHC", "PSB"))
+def test_feedback_turns_fused_with_intensity(kind):
+    w = build_feedback_workload(kind, n_particles=40001, n_turns=40, intensity=True)
+    ref = _ref(w, 25)
+    ft = w.fused(25)
+    assert ft.method is not None
+    _check(w, ref)
+
+
+def test_feedback_fused_graph_vs_eager_and_object_by_object():
+    a = build_feedback_workload("PSB", n_particles=30001, n_turns=80)
+    b = build_feedback_workload("PSB", n_particles=30001, n_turns=80)
+    from blond_amd.trackers.fused import FusedTracker
+
+    fa = FusedTracker(a.tracker, a.profile, None, graph_chunk=10)
+    fa.track(33)
+    fa.track(30)
+    fb = FusedTracker(b.tracker, b.profile, None, use_graph=False)
+    for _ in range(63):
+        fb.track(1)
+    for x, y in ((a.beam.dt, b.beam.dt), (a.beam.dE, b.beam.dE),
+                 (a.profile.n_macroparticles, b.profile.n_macroparticles)):
+        np.testing.assert_array_equal(host(x), host(y))
+    np.testing.assert_array_equal(a.rf.host_array("phi_rf"), b.rf.host_array("phi_rf"))
+    assert a.tracker.beamFB.PL_counter == b.tracker.beamFB.PL_counter > 0
+    # the fused loop continues from where object-by-object tracking left off
+    c = build_feedback_workload("PSB", n_particles=30001, n_turns=80)
+    ref = _ref(c, 20)
+    for _ in range(8):
+        for m in c.map():
+            m.track()
+    c.fused(12)
+    _check(c, ref)
