@@ -23,6 +23,12 @@ Prints one JSON line with the contract keys plus
                   adjoint c5: 1e6 macroparticles x 500 turns, forward +
                           checkpointed reverse pass, 112 B per
                           macroparticle-turn
+                  wake    c2 bunch (4e6, 4096 slices) with a multi-turn
+                          wake over 4 revolutions (memory 16384 points,
+                          n_fft 16384) plus an inductive Z/n source
+                  feedback c2 bunch (4e6, 4096 slices, broadband resonator)
+                          under the PSB beam phase loop, one loop workgroup
+                          per turn
 """
 from __future__ import annotations
 
@@ -57,7 +63,7 @@ def parse():
     ap.add_argument("--groups", type=int, default=0, help="count copies (0: automatic)")
     ap.add_argument("--max-peel", type=int, default=0)
     ap.add_argument("--method", choices=["auto", "fft", "direct", "fourstep"], default="auto")
-    ap.add_argument("--forms", default="large,batched,adjoint",
+    ap.add_argument("--forms", default="large,batched,adjoint,wake,feedback",
                     help="comma list of extra forms to measure ('' for none)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (the known-size kernel "
@@ -106,6 +112,49 @@ def form_large(torch, turns=200) -> dict:
            "whole_turn_frac": BYTES_PER_PARTICLE_TURN * n / (el / turns) / 1e9 / HBM_PEAK_GBS}
     ft.close()
     return out
+
+
+def _form_turns(torch, w, turns, label) -> dict:
+    from blond_amd.trackers.fused import FusedTracker
+
+    n = w.beam.n_macroparticles
+    ft = FusedTracker(w.tracker, w.profile, w.tiv)
+    ft.track(10)
+    el = _timed(lambda: ft.track(turns), torch.cuda.synchronize)
+    ms_ind, ms_turn = ft.time_kernels(30)
+    out = {"workload": label, "value": n * turns / el, "unit": "macroparticle-turns/s",
+           "turns": turns, "ms_per_turn": 1e3 * el / turns, "induced_method": ft.method,
+           "particle_kernel_ms": ms_turn, "induced_ms": ms_ind,
+           "particle_kernel_frac": BYTES_PER_PARTICLE_TURN * n / (ms_turn * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "whole_turn_frac": BYTES_PER_PARTICLE_TURN * n / (el / turns) / 1e9 / HBM_PEAK_GBS}
+    ft.close()
+    return out
+
+
+def form_wake(torch, turns=200) -> dict:
+    """c2 bunch with a 4-turn multi-turn wake and an inductive source; the
+    induced step is the LDS FFT (n_out = 16384) plus the wake step."""
+    from blond_amd.impedances.impedance import InductiveImpedance
+    from blond_amd.workloads import build_wake_workload
+
+    w = build_wake_workload("mtw", n_particles=4_000_000, n_turns=turns + 60, n_slices=4096)
+    zon = InductiveImpedance(w.beam, w.profile, [-100.0] * (turns + 61), w.rf)
+    zon.to_gpu()
+    w.tiv.induced_voltage_list.append(zon)
+    src = w.tiv.induced_voltage_list[0]
+    out = _form_turns(torch, w, turns, "c2 bunch 4e6, 4096 slices; multi-turn wake over 4 turns "
+                      f"(memory {src.n_mtw_memory}, n_fft {src.n_fft}) + inductive Z/n")
+    return out
+
+
+def form_feedback(torch, turns=200) -> dict:
+    """c2-ring bunch under the PSB phase and radial loops, with intensity."""
+    from blond_amd.workloads import build_feedback_workload
+
+    w = build_feedback_workload("PSB", n_particles=4_000_000, n_turns=turns + 60,
+                                n_slices=4096, intensity=True)
+    return _form_turns(torch, w, turns, "c2-ring bunch 4e6, 4096 slices, broadband resonator, "
+                       "PSB beam phase + radial loop on the device")
 
 
 def form_batched(torch, turns=100, seq_turns=20) -> dict:
@@ -290,7 +339,8 @@ def main():
         line["cpu_baseline"] = cpu_baseline(snap, a.cpu_seconds)
     forms = {}
     for name in [f for f in a.forms.split(",") if f]:
-        fn = {"large": form_large, "batched": form_batched, "adjoint": form_adjoint}[name]
+        fn = {"large": form_large, "batched": form_batched, "adjoint": form_adjoint,
+              "wake": form_wake, "feedback": form_feedback}[name]
         try:
             forms[name] = fn(torch)
         except Exception as exc:  # a form that cannot run is reported, not hidden

@@ -25,11 +25,12 @@ FUSED_KINDS = ("LHC", "LHC_SL", "LHC_F", "PSB")
 
 def _phase_tol(bc, p, alpha, w, phi, coeff):
     """Rounding bound of scoeff/ccoeff when the two sums are evaluated in
-    different orders: a few ulps times their condition numbers."""
+    different orders (the oracle's left-to-right sum against the device's
+    tree): (2 sqrt(n) + 8) ulps times their condition numbers."""
     base = np.exp(alpha * bc) * p
     fs, fc = base * np.sin(w * bc + phi), base * np.cos(w * bc + phi)
     cond = np.abs(fs).sum() / abs(fs.sum()) + np.abs(fc).sum() / abs(fc.sum())
-    return 8 * np.finfo(float).eps * cond * abs(coeff) + 1e-300
+    return (2 * np.sqrt(len(p)) + 8) * np.finfo(float).eps * cond * abs(coeff) + 1e-300
 
 
 @pytest.mark.parametrize("n", [1, 2, 100, 1024, 65536])
