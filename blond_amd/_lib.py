@@ -30,6 +30,8 @@ SIGNATURES = {
     "blmi_drift_f32": [_p, _p, _i, _f, _f, _f, _f, _f, _f, _f, _f, _f, _f, _f, _i64, _p],
     "blmi_histogram": [_p, _p, _d, _d, _i, _i64, _p],
     "blmi_histogram_f32": [_p, _p, _f, _f, _i, _i64, _p],
+    "blmi_sparse_histogram": [_p, _p, _p, _p, _p, _i, _i, _i, _i64, _p],
+    "blmi_sparse_histogram_f32": [_p, _p, _p, _p, _p, _i, _i, _i, _i64, _p],
     "blmi_smooth_histogram": [_p, _p, _d, _d, _i, _i64, _p],
     "blmi_smooth_histogram_f32": [_p, _p, _f, _f, _i, _i64, _p],
     "blmi_linear_interp_kick": [_p, _p, _p, _p, _d, _i, _i64, _d, _p],
@@ -60,6 +62,8 @@ SIGNATURES = {
     "blmi_engine_set_feedback": [_p, _i, _i, _i, _d, _d, _p, _p, _p, _p, _p, _i, _p],
     "blmi_engine_get_feedback": [_p, _p],
     "blmi_engine_get_programs": [_p, _i, _i, _p],
+    "blmi_engine_set_statistics": [_p, _i, _d, _d],
+    "blmi_engine_get_statistics": [_p, _i, _i, _p],
     "blmi_engine_set_launch": [_p, _i, _i, _i],
     "blmi_engine_set_turn": [_p, _i],
     "blmi_engine_get_turn": [_p, ct.POINTER(_i)],
@@ -129,10 +133,16 @@ def load(path: str | None = None) -> ct.CDLL:
     except OSError as exc:  # pragma: no cover - depends on the machine
         _LOAD_ERROR = f"cannot load {p}: {exc}"
         raise LibraryNotAvailable(_LOAD_ERROR) from exc
+    missing = []
     for name, args in SIGNATURES.items():
-        fn = getattr(lib, name)
+        fn = getattr(lib, name, None)
+        if fn is None:
+            # a library built from older sources: the entry fails when called
+            missing.append(name)
+            continue
         fn.argtypes = args
         fn.restype = _RESTYPES.get(name, ct.c_int)
+    lib.blmi_missing = frozenset(missing)
     if path is None:
         _LIB = lib
     return lib
@@ -145,6 +155,9 @@ def last_error() -> str:
 
 def call(name: str, *args) -> None:
     """Call ``name`` and raise BlmiError on a non-zero status."""
-    rc = getattr(load(), name)(*args)
+    lib = load()
+    if name in lib.blmi_missing:
+        raise LibraryNotAvailable(f"{name} is not in {LIB_PATH}: rebuild the library")
+    rc = getattr(lib, name)(*args)
     if rc != 0:
         raise BlmiError(f"{name} failed ({rc}): {last_error()}")

@@ -192,6 +192,31 @@ int blmi_histogram_f32(const float *dt, float *profile, float cut_left, float cu
     return launch_histogram<float>(dt, n, profile, cut_left, cut_right, n_slices, S(stream));
 }
 
+int blmi_sparse_histogram(const double *dt, double *profile, const double *cut_left,
+                          const double *cut_right, const double *bunch_indexes,
+                          int n_slices_bucket, int n_filled_buckets, int n_pattern, int64_t n,
+                          void *stream) {
+    BLMI_REQUIRE(n >= 0 && n_slices_bucket >= 0 && n_filled_buckets >= 1 && n_pattern >= 1,
+                 "sparse_histogram: bad sizes");
+    BLMI_REQUIRE(profile && cut_left && cut_right && bunch_indexes && (n == 0 || dt),
+                 "sparse_histogram: null array");
+    return launch_sparse_histogram<double>(dt, n, profile, cut_left, cut_right, bunch_indexes,
+                                           n_slices_bucket, n_filled_buckets, n_pattern,
+                                           S(stream));
+}
+
+int blmi_sparse_histogram_f32(const float *dt, float *profile, const float *cut_left,
+                              const float *cut_right, const float *bunch_indexes,
+                              int n_slices_bucket, int n_filled_buckets, int n_pattern, int64_t n,
+                              void *stream) {
+    BLMI_REQUIRE(n >= 0 && n_slices_bucket >= 0 && n_filled_buckets >= 1 && n_pattern >= 1,
+                 "sparse_histogram_f32: bad sizes");
+    BLMI_REQUIRE(profile && cut_left && cut_right && bunch_indexes && (n == 0 || dt),
+                 "sparse_histogram_f32: null array");
+    return launch_sparse_histogram<float>(dt, n, profile, cut_left, cut_right, bunch_indexes,
+                                          n_slices_bucket, n_filled_buckets, n_pattern, S(stream));
+}
+
 int blmi_smooth_histogram(const double *dt, double *profile, double cut_left, double cut_right,
                           int n_slices, int64_t n, void *stream) {
     BLMI_REQUIRE(n >= 0 && n_slices >= 0, "smooth_histogram: negative size");
@@ -507,6 +532,18 @@ int blmi_engine_get_programs(void *h, int turn0, int turns, double *rf_host) {
     Engine *e = reinterpret_cast<Engine *>(h);
     BLMI_REQUIRE(e && rf_host, "engine: null handle");
     return e->get_programs(turn0, turns, rf_host);
+}
+
+int blmi_engine_set_statistics(void *h, int enable, double shift_dt, double shift_dE) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    return e->set_statistics(enable != 0, shift_dt, shift_dE);
+}
+
+int blmi_engine_get_statistics(void *h, int turn0, int turns, double *moments_host) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e && moments_host, "engine: null handle");
+    return e->get_statistics(turn0, turns, moments_host);
 }
 
 int blmi_engine_set_wake(void *h, int64_t n_mem, int64_t n_add, const double *time, double *memory,

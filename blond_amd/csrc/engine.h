@@ -80,6 +80,13 @@ struct Engine {
     double *d_v1 = nullptr;        // [max(n_mem, n_slices)] convolution output
     double *d_indc = nullptr;      // [n_turns + 1] inductive coefficients (null: none)
     int deriv_mode = kDerivGradient;
+    // bunch moments after every turn, accumulated by the particle kernel
+    // (per-workgroup rows) and reduced once per chunk
+    bool stats_on = false;
+    double stat_shift_t = 0, stat_shift_e = 0;
+    int stats_grid = 0;
+    double *d_stats = nullptr;      // [n_turns + 1][stats_grid][4]
+    double *d_moments = nullptr;    // [n_turns + 1][5]
     // beam phase loop (feedback.h): one workgroup per turn after the induced
     // voltage, before the particle kernel; it updates the next turn's
     // omega_rf and phi_rf in d_rf
@@ -147,6 +154,9 @@ struct Engine {
                      const double *state);
     int get_feedback_state(double *host);
     int get_programs(int turn0, int turns, double *host);
+    int set_statistics(bool on, double shift_t, double shift_e);
+    int ensure_stats();
+    int get_statistics(int turn0, int turns, double *host);
     int launch_feedback_turn(hipStream_t s, int off);
     int set_hist(int groups, int max_peel);
     int alloc_counts();

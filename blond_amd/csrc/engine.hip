@@ -78,7 +78,54 @@ struct TurnArgs {
     // u16-pair histograms: workgroup b stores its LDS words at partials + b * part_words
     unsigned *partials;
     int part_words;
+    // bunch moments of the coordinates after the drift: workgroup b writes
+    // sum(dt - shift_t), sum((dt - shift_t)^2), sum(dE - shift_e),
+    // sum((dE - shift_e)^2) of its particles to stats[(t + 1) * grid + b]
+    double *stats;
+    double shift_t, shift_e;
 };
+
+// Workgroup sums of the four moment accumulators in a fixed order (wavefront
+// butterfly, then wavefronts in index order), written by thread 0.
+__device__ __forceinline__ void stats_flush(double s[4], double *__restrict__ out) {
+    __shared__ double wred[1024 / kWave][4];
+    for (int k = 0; k < 4; ++k)
+        for (int off = kWave / 2; off > 0; off >>= 1) s[k] += __shfl_xor(s[k], off);
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    if (lane == 0)
+        for (int k = 0; k < 4; ++k) wred[w][k] = s[k];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double v = 0.0;
+        for (int q = 0; q < (int)blockDim.x / kWave; ++q) v += wred[q][threadIdx.x];
+        out[threadIdx.x] = v;
+    }
+}
+
+// Moments of turns *base + 1 .. *base + gridDim.x from the workgroup rows:
+// moments[t] = (n, mean_dt, sigma_dt, mean_dE, sigma_dE), sigma with ddof 0.
+__global__ void moments_reduce_kernel(const double *__restrict__ part, int grid, const int *base,
+                                      double n, double shift_t, double shift_e,
+                                      double *__restrict__ moments) {
+    __shared__ double s[4];
+    const int t = *base + 1 + blockIdx.x;
+    if (threadIdx.x < 4) {
+        double v = 0.0;
+        for (int g = 0; g < grid; ++g) v += part[((size_t)t * grid + g) * 4 + threadIdx.x];
+        s[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double mt = s[0] / n, me = s[2] / n;
+        const double vt = s[1] / n - mt * mt, ve = s[3] / n - me * me;
+        double *m = moments + (size_t)t * 5;
+        m[0] = n;
+        m[1] = shift_t + mt;
+        m[2] = sqrt(vt > 0.0 ? vt : 0.0);
+        m[3] = shift_e + me;
+        m[4] = sqrt(ve > 0.0 ? ve : 0.0);
+    }
+}
 
 // Copy the workgroup's LDS words to its row of partial histograms (16-byte
 // stores; part_words is a multiple of 4).
@@ -191,6 +238,14 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
         }
         return -1;
     };
+    double st[4] = {0.0, 0.0, 0.0, 0.0};
+    auto moment = [&](T x, T e) {
+        const double u = (double)x - a.shift_t, v = (double)e - a.shift_e;
+        st[0] += u;
+        st[1] += u * u;
+        st[2] += v;
+        st[3] += v * v;
+    };
     auto count = [&](int bin) {
         if (!HIST) return;
         if (LDSH)
@@ -207,6 +262,10 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
         const int b1 = step(x.y, e.y);
         reinterpret_cast<V2 *>(dt)[i] = x;
         reinterpret_cast<V2 *>(dE)[i] = e;
+        if (a.stats) {
+            moment(x.x, e.x);
+            moment(x.y, e.y);
+        }
         count(b0);
         count(b1);
     }
@@ -219,9 +278,11 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
             b = step(x, e);
             dt[n - 1] = x;
             dE[n - 1] = e;
+            if (a.stats) moment(x, e);
         }
         count(b);
     }
+    if (a.stats) stats_flush(st, a.stats + ((size_t)(t + 1) * gridDim.x + blockIdx.x) * 4);
     if (HIST && LDSH) {
         __syncthreads();
         if (LDSH == 2)
@@ -342,6 +403,11 @@ void Engine::release() {
     (void)hipFree(d_fb);
     d_fb = nullptr;
     fb_on = false;
+    (void)hipFree(d_stats);
+    (void)hipFree(d_moments);
+    d_stats = d_moments = nullptr;
+    stats_on = false;
+    stats_grid = 0;
     d_trev = d_mem = d_time = d_v1 = d_indc = nullptr;
     n_mem = 0;
     mem_user = nullptr;
@@ -789,6 +855,9 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
     a.pending = e.d_base + 1;
     a.partials = e.d_partials;
     a.part_words = hist_words(2, e.n_slices);
+    a.stats = e.stats_on ? e.d_stats : nullptr;
+    a.shift_t = e.stat_shift_t;
+    a.shift_e = e.stat_shift_e;
     int grid = 0;
     size_t lds = 0;
     int ldsh = e.hist_layout(e.block_size, &grid, &lds);
@@ -920,6 +989,44 @@ int Engine::get_programs(int turn0, int turns, double *host) {
     BLMI_HIP_TRY(hipStreamSynchronize(own_stream));
     BLMI_HIP_TRY(hipMemcpy(host, d_rf + (size_t)turn0 * 3 * n_rf,
                            sizeof(double) * 3 * n_rf * (size_t)turns, hipMemcpyDeviceToHost));
+    return BLMI_OK;
+}
+
+int Engine::set_statistics(bool on, double shift_t, double shift_e) {
+    invalidate_graph();
+    stats_on = on;
+    stat_shift_t = shift_t;
+    stat_shift_e = shift_e;
+    if (!on) return BLMI_OK;
+    if (!d_moments) {
+        const size_t bytes = sizeof(double) * 5 * ((size_t)n_turns + 1);
+        BLMI_HIP_TRY(hipMalloc(&d_moments, bytes));
+        BLMI_HIP_TRY(hipMemset(d_moments, 0, bytes));
+    }
+    return ensure_stats();
+}
+
+int Engine::ensure_stats() {
+    if (!stats_on) return BLMI_OK;
+    int grid = 0;
+    size_t lds = 0;
+    hist_layout(block_size, &grid, &lds);
+    if (grid == stats_grid && d_stats) return BLMI_OK;
+    (void)hipFree(d_stats);
+    d_stats = nullptr;
+    BLMI_HIP_TRY(hipMalloc(&d_stats, sizeof(double) * 4 * (size_t)grid * ((size_t)n_turns + 1)));
+    stats_grid = grid;
+    return BLMI_OK;
+}
+
+int Engine::get_statistics(int turn0, int turns, double *host) {
+    if (!d_moments) return blmi_set_error(BLMI_ERR_ARG, "statistics: not enabled");
+    if (turn0 < 0 || turns < 0 || turn0 + turns > n_turns + 1)
+        return blmi_set_error(BLMI_ERR_ARG, "statistics: turns [%d, %d) outside [0, %d]", turn0,
+                              turn0 + turns, n_turns + 1);
+    BLMI_HIP_TRY(hipStreamSynchronize(own_stream));
+    BLMI_HIP_TRY(hipMemcpy(host, d_moments + (size_t)turn0 * 5, sizeof(double) * 5 * (size_t)turns,
+                           hipMemcpyDeviceToHost));
     return BLMI_OK;
 }
 
@@ -1084,6 +1191,12 @@ int Engine::finish_chunk(hipStream_t s, int count) {
             hipLaunchKernelGGL(to_float_kernel, dim3((n_slices + 255) / 256), dim3(256), 0, s,
                                (const double *)volt, volt32, n_slices);
     }
+    if (stats_on) {
+        hipLaunchKernelGGL(moments_reduce_kernel, dim3(count), dim3(64), 0, s, d_stats, stats_grid,
+                           d_base, (double)n, stat_shift_t, stat_shift_e, d_moments);
+        int rc = blmi_check_launch("moments_reduce");
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, d_base, count);
     int rc = blmi_check_launch("advance");
     if (rc) return rc;
@@ -1102,6 +1215,7 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
         return blmi_set_error(BLMI_ERR_ARG, "time_kernels: %d turns from turn %d passes n_turns=%d",
                               turns, host_turn, n_turns);
     int prc = ensure_partials();
+    if (!prc) prc = ensure_stats();
     if (prc) return prc;
     const bool ord = ordered();
     if (ord) {
@@ -1165,6 +1279,7 @@ int Engine::track(int turns, hipStream_t user, bool use_graph) {
         return blmi_set_error(BLMI_ERR_ARG, "tracking %d turns from turn %d passes n_turns=%d",
                               turns, host_turn, n_turns);
     int prc = ensure_partials();
+    if (!prc) prc = ensure_stats();
     if (prc) return prc;
     const bool ord = ordered();
     if (ord) {

@@ -321,7 +321,8 @@ __global__ void order_out_kernel(const double *__restrict__ xi, const double *__
 
 // ---------------------------------------------------------------------------
 bool Engine::ordered() const {
-    return sort_every > 0 && !f32 && !fb_on && n_slices > kSortWindow && n < (int64_t)1 << 32 &&
+    return sort_every > 0 && !f32 && !fb_on && !stats_on && n_slices > kSortWindow &&
+           n < (int64_t)1 << 32 &&
            n >= 2;
 }
 

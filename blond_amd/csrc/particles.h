@@ -48,6 +48,11 @@ int launch_rf_volt_comp(const T *V, const T *w, const T *phi, const T *bc, int n
 
 // slices.hip
 template <typename T> int launch_fill(T *p, int64_t n, T v, hipStream_t s);
+// sparse_histogram.cpp:22-87 (profile: [n_filled][n_slices_bucket], overwritten)
+template <typename T>
+int launch_sparse_histogram(const T *dt, int64_t n, T *profile, const T *cut_left,
+                            const T *cut_right, const T *bunch_indexes, int n_slices_bucket,
+                            int n_filled, int n_pattern, hipStream_t s);
 template <typename T>
 int launch_histogram(const T *dt, int64_t n, T *profile, T cut_left, T cut_right, int n_slices,
                      hipStream_t s);

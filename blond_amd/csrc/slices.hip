@@ -117,6 +117,84 @@ int launch_histogram(const T *dt, int64_t n, T *profile, T cut_left, T cut_right
 }
 
 // ---------------------------------------------------------------------------
+// sparse histogram (blond/beam/sparse_histogram.cpp:22-87): one profile of
+// nsb slices per filled bucket of a filling pattern.  The bucket is found
+// relative to the first filled bucket's left cut and looked up in
+// bunch_indexes (the bucket's profile row, -1 when empty), the bin by
+// truncation toward zero from that row's left cut.  Where the reference
+// would read past bunch_indexes (a particle exactly at the last right cut,
+// or NaN) or write outside the profile rows, the particle is skipped.
+// ---------------------------------------------------------------------------
+template <typename T, bool LDS>
+__global__ void __launch_bounds__(256)
+sparse_hist_kernel(const T *__restrict__ dt, int64_t n, T *__restrict__ out,
+                   const T *__restrict__ cl, const T *__restrict__ cr,
+                   const T *__restrict__ bidx, int nsb, int nfb, int n_pattern) {
+    extern __shared__ unsigned lds_hist[];
+    const int total = nfb * nsb;
+    if (LDS) {
+        for (int b = threadIdx.x; b < total; b += blockDim.x) lds_hist[b] = 0u;
+        __syncthreads();
+    }
+    const T lo = cl[0], hi = cr[nfb - 1];
+    // sparse_histogram.cpp:40-41 (the 1.0 literal makes the reciprocal a double)
+    const T inv_bucket = (T)(1.0 / (double)(cr[0] - cl[0]));
+    const T inv_bw = inv_bucket * (T)nsb;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const T a = dt[j];
+        if (a < lo || a > hi || isnan(a)) continue;
+        const int fb = (int)((a - lo) * inv_bucket);
+        if (fb >= n_pattern) continue;
+        const int ib = (int)bidx[fb];
+        if (ib == -1) continue;
+        const int bin = ib * nsb + (int)((a - cl[ib]) * inv_bw);
+        if (bin < 0 || bin >= total) continue;
+        if (LDS)
+            atomicAdd(&lds_hist[bin], 1u);
+        else
+            atomic_add_agent(&out[bin], T(1));
+    }
+    if (LDS) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < total; b += blockDim.x) {
+            const unsigned c = lds_hist[b];
+            if (c) atomic_add_agent(&out[b], T(c));
+        }
+    }
+}
+
+template <typename T>
+int launch_sparse_histogram(const T *dt, int64_t n, T *profile, const T *cut_left,
+                            const T *cut_right, const T *bunch_indexes, int n_slices_bucket,
+                            int n_filled, int n_pattern, hipStream_t s) {
+    const int64_t total = (int64_t)n_slices_bucket * n_filled;
+    if (total <= 0) return BLMI_OK;
+    int rc = launch_fill<T>(profile, total, T(0), s);
+    if (rc || n <= 0) return rc;
+    const size_t lds = (size_t)total * sizeof(unsigned);
+    int grid = stream_grid(n, 256);
+    const int cap = device_cu_count() * 4;
+    if (grid > cap) grid = cap;
+    if (lds <= 65536)
+        hipLaunchKernelGGL((sparse_hist_kernel<T, true>), dim3(grid), dim3(256), lds, s, dt, n,
+                           profile, cut_left, cut_right, bunch_indexes, n_slices_bucket, n_filled,
+                           n_pattern);
+    else
+        hipLaunchKernelGGL((sparse_hist_kernel<T, false>), dim3(grid), dim3(256), 0, s, dt, n,
+                           profile, cut_left, cut_right, bunch_indexes, n_slices_bucket, n_filled,
+                           n_pattern);
+    return blmi_check_launch("sparse_histogram");
+}
+
+template int launch_sparse_histogram<double>(const double *, int64_t, double *, const double *,
+                                             const double *, const double *, int, int, int,
+                                             hipStream_t);
+template int launch_sparse_histogram<float>(const float *, int64_t, float *, const float *,
+                                            const float *, const float *, int, int, int,
+                                            hipStream_t);
+
+// ---------------------------------------------------------------------------
 // smooth histogram (histogram.cpp:75-132, including its weight quirk)
 // ---------------------------------------------------------------------------
 template <typename T>

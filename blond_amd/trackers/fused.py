@@ -139,7 +139,7 @@ def wake_setup(tiv, n_turns: int):
 class FusedTracker:
     def __init__(self, tracker, profile=None, total_induced_voltage=None, use_graph=True,
                  block=1024, blocks_per_cu=1, graph_chunk=50, groups=0, max_peel=0,
-                 method=None, sort_every=0):
+                 method=None, sort_every=0, statistics=False, monitor=None):
         import torch
 
         if tracker.interpolation:
@@ -197,6 +197,10 @@ class FusedTracker:
         _lib.call("blmi_engine_set_order", self._h, int(sort_every))
         if method is not None and self.tiv is not None:
             _lib.call("blmi_engine_set_method", self._h, _METHODS[method])
+        self.monitor = monitor
+        self._stats = bool(statistics) or monitor is not None
+        if self._stats:
+            self._setup_statistics()
         _lib.call("blmi_engine_set_turn", self._h, int(rp.counter[0]))
         self._primed = False
 
@@ -312,6 +316,29 @@ class FusedTracker:
                   _dptr(wd), _dptr(prog), _dptr(on_time) if len(on_time) else None,
                   len(on_time), _dptr(st))
 
+    def _setup_statistics(self):
+        """Per-turn moments from the particle kernel, shifted by the current
+        means (bm.beam_statistics) so that the one-pass variance keeps its
+        precision."""
+        from ..utils import bmath as bm
+
+        b = self.beam
+        if b.n_macroparticles_alive != b.n_macroparticles:
+            raise NotImplementedError("fused statistics need every particle alive (the fused "
+                                      "turn tracks no losses)")
+        _, mt, _, me, _ = bm.beam_statistics(b.dt, b.dE)
+        _lib.call("blmi_engine_set_statistics", self._h, 1, float(mt), float(me))
+
+    def statistics(self, turn0: int, turns: int) -> np.ndarray:
+        """Rows {n, mean_dt, sigma_dt, mean_dE, sigma_dE} of turns
+        [turn0, turn0 + turns) as left by the fused turns (row t + 1 is the
+        bunch after turn t; rows not tracked by this engine are zero)."""
+        if not self._stats:
+            raise RuntimeError("statistics were not enabled (FusedTracker(statistics=True))")
+        out = np.zeros((int(turns), 5), dtype=np.float64)
+        _lib.call("blmi_engine_get_statistics", self._h, int(turn0), int(turns), _dptr(out))
+        return out
+
     def _fetch_feedback(self, start: int, end: int):
         """The loop's RF program rows start+1..end and its state, back to the
         host objects (tracker.py:551-565 leaves them in RFStation)."""
@@ -352,6 +379,8 @@ class FusedTracker:
         rp = self.rf
         if self.fb is not None and end > start:
             self._fetch_feedback(start, end)
+        if self.monitor is not None and end > start:
+            self.monitor.record_moments(self.statistics(start + 1, end - start))
         # host-side state the map leaves behind (tracker.py:556-615)
         if end > start and self._phase_moves:
             host_phi = rp.host_array("phi_rf")

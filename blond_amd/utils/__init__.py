@@ -21,6 +21,7 @@ class MI355XBackend:
         self.drift = _k.drift
         self.slice_beam = _k.slice_beam
         self.slice_smooth = _k.slice_smooth
+        self.sparse_histogram = _k.sparse_histogram
         self.linear_interp_kick = _k.linear_interp_kick
         self.rf_volt_comp = _k.rf_volt_comp
         self.fast_resonator = _k.fast_resonator

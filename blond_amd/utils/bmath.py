@@ -153,6 +153,34 @@ def slice_smooth(dt, profile, cut_left, cut_right):
     a_p.finish()
 
 
+def sparse_histogram(dt, profile, cut_left, cut_right, bunch_indexes, n_slices_bucket):
+    """sparse_histogram.cpp:22-87 via butils_wrap_cpp.py:1003-1023: profile
+    [n_filled, n_slices_bucket] overwritten with the per-bucket counts."""
+    torch = _torch()
+    a_dt = _Staged(torch, dt, False, name="dt")
+    shape = tuple(profile.shape)
+    if isinstance(profile, torch.Tensor):
+        if not profile.is_contiguous():
+            raise ValueError("profile: in/out tensor must be contiguous")
+        flat = profile.view(-1)
+    else:
+        flat = np.asarray(profile).reshape(-1)
+        if not np.shares_memory(flat, profile):
+            raise ValueError("profile: in/out array must be C-contiguous")
+    a_p = _Staged(torch, flat, True, name="profile")
+    cl = _Staged(torch, np.asarray(cut_left) if not isinstance(cut_left, torch.Tensor) else cut_left,
+                 False, name="cut_left")
+    cr = _Staged(torch, np.asarray(cut_right) if not isinstance(cut_right, torch.Tensor) else cut_right,
+                 False, name="cut_right")
+    bi = _Staged(torch, np.asarray(bunch_indexes, dtype=precision.real_t)
+                 if not isinstance(bunch_indexes, torch.Tensor) else bunch_indexes,
+                 False, name="bunch_indexes")
+    n_filled = shape[0]
+    _lib.call("blmi_sparse_histogram" + _suffix(), a_dt.ptr, a_p.ptr, cl.ptr, cr.ptr, bi.ptr,
+              int(n_slices_bucket), int(n_filled), _len(bi.dev), _len(dt), _stream(torch))
+    a_p.finish()
+
+
 def linear_interp_kick(dt, dE, voltage, bin_centers, charge, acceleration_kick):
     """linear_interp_kick.cpp:21-71 via butils_wrap_cpp.py:946-968."""
     torch = _torch()

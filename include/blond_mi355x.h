@@ -70,6 +70,22 @@ int blmi_histogram(const double *dt, double *profile, double cut_left, double cu
 int blmi_histogram_f32(const float *dt, float *profile, float cut_left, float cut_right,
                        int n_slices, int64_t n, void *stream);
 
+/* Per-bucket profiles of a filling pattern: profile[n_filled][n_slices_bucket]
+ * (overwritten) from dt, with the filled buckets' cuts cut_left/right[n_filled]
+ * and bunch_indexes[n_pattern] (the row of each RF bucket, -1 when empty; the
+ * reference passes it as real_t).  Replaces sparse_histogram(),
+ * blond/beam/sparse_histogram.cpp:22-87 (butils_wrap_cpp.py:1003-1023); the
+ * extra n_pattern bounds the bucket lookup where the reference reads past its
+ * array (a particle at the last right cut). */
+int blmi_sparse_histogram(const double *dt, double *profile, const double *cut_left,
+                          const double *cut_right, const double *bunch_indexes,
+                          int n_slices_bucket, int n_filled_buckets, int n_pattern, int64_t n,
+                          void *stream);
+int blmi_sparse_histogram_f32(const float *dt, float *profile, const float *cut_left,
+                              const float *cut_right, const float *bunch_indexes,
+                              int n_slices_bucket, int n_filled_buckets, int n_pattern, int64_t n,
+                              void *stream);
+
 /* Two-bin "smooth" histogram with the reference's weight rule.  Replaces
  * smooth_histogram(), histogram.cpp:75-132 (slice_smooth, butils_wrap_cpp.py:987-1000).
  * Weights are summed in 64-bit fixed point: deterministic, within ~1e-13 of
@@ -257,6 +273,16 @@ int blmi_engine_set_feedback(void *engine, int machine, int delay, int first_bin
                              const double *state_host);
 int blmi_engine_get_feedback(void *engine, double *state_host);
 int blmi_engine_get_programs(void *engine, int turn0, int turns, double *rf_host);
+/* Bunch moments after every turn (Beam.statistics, beam.py:430-458, for the
+ * whole bunch: the engine has no lost particles), accumulated inside the
+ * particle kernel as sums of (x - shift) and (x - shift)^2 per workgroup in a
+ * fixed order and reduced once per chunk: no extra pass over the particles.
+ * blmi_engine_get_statistics copies rows [turn0, turn0 + turns) of
+ * {n, mean_dt, sigma_dt, mean_dE, sigma_dE} (synchronous); the row of turn
+ * t + 1 is written by turn t.  Shifts near the means keep the one-pass
+ * variance accurate.  Turns the bin-ordered mode off. */
+int blmi_engine_set_statistics(void *engine, int enable, double shift_dt, double shift_dE);
+int blmi_engine_get_statistics(void *engine, int turn0, int turns, double *moments_host);
 /* Launch shape of the particle kernel (block in {64..1024}, workgroups per CU)
  * and the number of turns per captured graph. */
 int blmi_engine_set_launch(void *engine, int block, int blocks_per_cu, int graph_chunk);

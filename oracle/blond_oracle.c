@@ -25,6 +25,9 @@
  *   linear_interp_kick   golden vectors; numpy.interp closed form.
  *   rf_volt_comp         golden vectors; closed form.
  *   fast_resonator_real_imag  golden vectors; complex closed form R/(1+jQ(f/fr-fr/f)).
+ *   sparse_histogram     the reference build (bit for bit); the uniform
+ *                        histogram over the whole train (the reference's
+ *                        unittests/beam_profile/test_sparse_profile.py).
  *   beam_phase, beam_phase_fast  the reference build run on one thread (bit
  *                        for bit); the closed form tan(w t + phi) of a one-bin
  *                        profile; cos and exp against libm.
@@ -104,6 +107,30 @@ float oracle_sinf(float x) {
     if (flip) out = -out;
     if (x < 0.0f) out = -out;
     return out;
+}
+
+/* sparse_histogram.cpp:22-87: bucket from the first filled bucket's left
+ * cut, its row from bunch_indexes (-1: empty), bin by truncation; right cut
+ * of the last bucket inclusive.  Particles the reference would look up past
+ * bunch_indexes (n_pattern entries) or count outside the rows are skipped. */
+void sparse_histogram_n(const double *input, double *output, const double *cut_left,
+                        const double *cut_right, const double *bunch_indexes,
+                        int n_slices_bucket, int n_filled, int n, int n_pattern) {
+    const double inv_bucket = 1.0 / (cut_right[0] - cut_left[0]);
+    const double inv_bw = inv_bucket * (double)n_slices_bucket;
+    const int total = n_filled * n_slices_bucket;
+    for (int i = 0; i < total; ++i) output[i] = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double a = input[j];
+        if (a < cut_left[0] || a > cut_right[n_filled - 1] || a != a) continue;
+        const int fb = (int)((a - cut_left[0]) * inv_bucket);
+        if (fb >= n_pattern) continue;
+        const int ib = (int)bunch_indexes[fb];
+        if (ib == -1) continue;
+        const int bin = ib * n_slices_bucket + (int)((a - cut_left[ib]) * inv_bw);
+        if (bin < 0 || bin >= total) continue;
+        output[bin] += 1.0;
+    }
 }
 
 /* Cosine with the sine's reduction and polynomials (VDT fast_sincos,

@@ -113,6 +113,22 @@ class Lib:
                               _ptr(out, self.ptr))
         return out
 
+    # sparse_histogram.cpp:22-87 (double only).  The reference's own symbol
+    # takes no pattern length; the oracle's sparse_histogram_n bounds the
+    # bucket lookup by it.
+    def sparse_histogram(self, dt, profile, cut_left, cut_right, bunch_indexes, n_slices_bucket):
+        cl = np.ascontiguousarray(cut_left, dtype=np.float64)
+        cr = np.ascontiguousarray(cut_right, dtype=np.float64)
+        bi = np.ascontiguousarray(bunch_indexes, dtype=np.float64)
+        if hasattr(self.lib, "sparse_histogram_n"):
+            self.lib.sparse_histogram_n(_ptr(dt), _ptr(profile), _ptr(cl), _ptr(cr), _ptr(bi),
+                                        ct.c_int(int(n_slices_bucket)), ct.c_int(len(cl)),
+                                        ct.c_int(len(dt)), ct.c_int(len(bi)))
+        else:
+            self.lib.sparse_histogram(_ptr(dt), _ptr(profile), _ptr(cl), _ptr(cr), _ptr(bi),
+                                      ct.c_int(int(n_slices_bucket)), ct.c_int(len(cl)),
+                                      ct.c_int(len(dt)))
+
     # beam_phase.cpp:6-39 and 42-65 (double only)
     def beam_phase(self, bin_centers, profile, alpha, omega_rf, phi_rf, bin_size):
         bc = np.ascontiguousarray(bin_centers, dtype=np.float64)

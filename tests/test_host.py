@@ -146,3 +146,35 @@ def test_oracle_turn_loop_runs_and_conserves_particles():
     st = orc.track(w.snapshot(), 5)
     assert st["turn"] == 5 and st["profile"].sum() <= 4000
     assert np.all(np.isfinite(st["dt"])) and np.all(np.isfinite(st["dE"]))
+
+
+def test_bunch_monitor_buffering_rows(tmp_path):
+    """monitors.py:117-129, 325-460: row 0 from the constructor, then the
+    buffer of turns [k*b, (k+1)*b) written when i_turn reaches (k+1)*b;
+    float32 datasets; written as npz when h5py is absent."""
+    from blond_amd.monitors.monitors import BunchMonitor, _h5py
+    from blond_amd.workloads import build_workload
+
+    w = build_workload("c1", n_particles=4000, n_turns=12, n_slices=100, to_gpu=False)
+    mon = BunchMonitor(w.ring, w.rf, w.beam, str(tmp_path / "mon"), buffer_time=4)
+    rows = np.zeros((8, 5))
+    rows[:, 0] = 4000
+    rows[:, 1] = 1e-9 + np.arange(8) * 1e-12
+    rows[:, 2] = 2e-10
+    rows[:, 3] = np.arange(8) * 10.0
+    rows[:, 4] = 5e5
+    mon.record_moments(rows)
+    if _h5py is not None:  # pragma: no cover - h5py not installed here
+        return
+    d = np.load(str(tmp_path / "mon.npz"))
+    mean_dt = d["Beam/mean_dt"]
+    assert mean_dt.dtype == np.float32 and mean_dt.shape == (13,)
+    w.beam.statistics()
+    assert mean_dt[0] == np.float32(w.beam.mean_dt)
+    # buffer row i holds turn i; rows 1..7 are the recorded turns 1..7 (turn 8
+    # sits in the buffer until i_turn reaches 12)
+    np.testing.assert_array_equal(mean_dt[1:8], rows[:7, 1].astype(np.float32))
+    assert np.all(mean_dt[8:] == 0)
+    np.testing.assert_array_equal(d["Beam/epsn_rms_l"][1:8],
+                                  (np.pi * rows[:7, 4] * rows[:7, 2]).astype(np.float32))
+    assert mon.i_turn == 9
