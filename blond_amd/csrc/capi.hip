@@ -377,12 +377,13 @@ int blmi_induced_voltage(const double *profile, int n_slices, const double *Z, i
     a.scale = scale;
     a.volt = volt;
     a.n_out = n_out;
-    if (n_fft <= fft_lds_max_real()) {
+    if (n_fft < 8192 && n_fft <= fft_lds_max_real()) {
         FftPlan p;
         int rc = fft_plan_get(n_fft, &p);
         if (rc == BLMI_OK) return launch_induced_voltage_lds(a, p, S(stream));
     }
-    // beyond one workgroup's LDS: the four-step transform; its plan holds
+    // from 8192 points (or beyond one workgroup's LDS): the four-step
+    // transform, faster there than the one-workgroup one; its plan holds
     // scratch, so plans are kept per (device, n_fft, stream)
     static std::mutex mu;
     static std::map<std::tuple<int, int64_t, void *>, Fft2Plan> plans;
@@ -395,7 +396,14 @@ int blmi_induced_voltage(const double *profile, int n_slices, const double *Z, i
         auto it = plans.find(key);
         if (it == plans.end()) {
             int rc = fft2_plan_create(n_fft, &p2);
-            if (rc) return rc;
+            if (rc) {
+                // no four-step factorisation: the one-workgroup transform
+                // when the length fits its LDS
+                FftPlan p;
+                if (n_fft <= fft_lds_max_real() && fft_plan_get(n_fft, &p) == BLMI_OK)
+                    return launch_induced_voltage_lds(a, p, S(stream));
+                return rc;
+            }
             plans[key] = p2;
         } else {
             p2 = it->second;

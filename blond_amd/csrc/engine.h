@@ -144,6 +144,7 @@ struct Engine {
     int set_induced_f32(int64_t n_fft, const double *Z_host, double scale, float *volt,
                         const float *bc, double charge);
     int set_method(int method);
+    int preferred_fft() const;
     int set_wake(int64_t n_mem, int64_t n_add, const double *time, double *memory,
                  const double *ind_coef_host, int deriv_mode);
     bool wake_on() const { return n_mem > 0 || d_indc != nullptr; }
@@ -158,6 +159,9 @@ struct Engine {
     int ensure_stats();
     int get_statistics(int turn0, int turns, double *host);
     int launch_feedback_turn(hipStream_t s, int off);
+    FeedbackArgs feedback_args(int off) const;
+    // the loop runs as one extra workgroup of the direct convolution's launch
+    bool fb_in_induced() const { return fb_on && intensity && have_conv && method == 1 && !wake_on(); }
     int set_hist(int groups, int max_peel);
     int alloc_counts();
     int set_turn(int turn);

@@ -621,10 +621,18 @@ int Engine::set_induced(int64_t n_fft_, const double *Z_host, double scale_, dou
         return blmi_set_error(BLMI_ERR_UNSUPPORTED, "induced voltage: n_fft=%lld is not supported (%s)",
                               (long long)n_fft, why.c_str());
     }
-    method = have_direct ? 1 : have_fft ? 0 : 2;
+    method = have_direct ? 1 : preferred_fft();
     intensity = true;
     invalidate_graph();
     return BLMI_OK;
+}
+
+// The one-workgroup LDS transform wins below 8192 points; from 8192 the
+// four-step transform over many workgroups is faster (8192: 21 against
+// 33 us; 16384: 24 against 60 us per turn, measured on the c2 bunch).
+int Engine::preferred_fft() const {
+    if (have_fourstep && (n_fft >= 8192 || !have_fft)) return 2;
+    return have_fft ? 0 : 2;
 }
 
 int Engine::set_method(int method_) {
@@ -919,6 +927,11 @@ int Engine::launch_induced(hipStream_t s, int off, const KernelEvents &ev) {
             ia.tab_params = nullptr;
             e1.stop = nullptr;
         }
+        if (fb_in_induced()) {
+            ia.with_fb = 1;
+            ia.fb = feedback_args(off);
+            ia.fb.from_counts = 1;  // the profile is being written beside it
+        }
         int rc = method == 1   ? launch_induced_direct(ia, s, e1)
                  : method == 2 ? launch_induced_fourstep(ia, plan2, s, e1)
                                : launch_induced_voltage_lds(ia, plan, s, e1);
@@ -1031,6 +1044,10 @@ int Engine::get_statistics(int turn0, int turns, double *host) {
 }
 
 int Engine::launch_feedback_turn(hipStream_t s, int off) {
+    return launch_feedback(feedback_args(off), s);
+}
+
+FeedbackArgs Engine::feedback_args(int off) const {
     FeedbackArgs a{};
     a.turn_base = d_base;
     a.turn_off = off;
@@ -1058,7 +1075,8 @@ int Engine::launch_feedback_turn(hipStream_t s, int off) {
     a.on_time = fb_on_time;
     a.n_on_time = fb_n_on;
     a.state = fb_state;
-    return launch_feedback(a, s);
+    a.from_counts = 0;
+    return a;
 }
 
 WakeArgs Engine::wake_args(int off) const {
@@ -1122,7 +1140,7 @@ int Engine::set_wake(int64_t n_mem_, int64_t n_add_, const double *time, double 
         BLMI_HIP_TRY(hipMemset(d_mem, 0, sizeof(double) * 2 * (size_t)n_mem));
         BLMI_HIP_TRY(hipMalloc(&d_time, sizeof(double) * (size_t)n_mem));
         BLMI_HIP_TRY(hipMemcpy(d_time, time, sizeof(double) * (size_t)n_mem, hipMemcpyDefault));
-        if (method == 1 && n_mem > n_slices) method = have_fft ? 0 : 2;
+        if (method == 1 && n_mem > n_slices) method = preferred_fft();
         if (method == 2 && !have_fourstep)
             return blmi_set_error(BLMI_ERR_UNSUPPORTED, "multi-turn memory: no FFT for n_fft %lld",
                                   (long long)n_fft);
@@ -1158,7 +1176,7 @@ int Engine::launch_turns(hipStream_t s, int count, bool advance) {
             int rc = launch_induced(s, k, none);
             if (rc) return rc;
         }
-        if (fb_on) {
+        if (fb_on && !fb_in_induced()) {
             int rc = launch_feedback_turn(s, k);
             if (rc) return rc;
         }
@@ -1239,7 +1257,7 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
     for (int k = 0; k < turns && rc == BLMI_OK; ++k) {
         KernelEvents ei{ev[4 * k], ev[4 * k + 1]}, et{ev[4 * k + 2], ev[4 * k + 3]};
         if (intensity) rc = launch_induced(own_stream, k, ei);
-        if (rc == BLMI_OK && fb_on) rc = launch_feedback_turn(own_stream, k);
+        if (rc == BLMI_OK && fb_on && !fb_in_induced()) rc = launch_feedback_turn(own_stream, k);
         if (rc == BLMI_OK) rc = launch_turn_kernel(own_stream, k, intensity, et);
         if (rc == BLMI_OK) rc = launch_reduce(own_stream, k + 1);
         if (rc == BLMI_OK && ordered() && ++ord_since == sort_every) {

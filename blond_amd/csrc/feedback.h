@@ -54,6 +54,8 @@ struct FeedbackArgs {
     const double *profile;
     const unsigned *counts0, *counts1;
     const int *pending;           // null: always `profile`
+    int from_counts;              // 1: always counts[t & 1] (the loop runs beside the
+                                  // launch that writes `profile`)
     int groups;
     double *rf;                   // [n_turns + 1][3][n_rf]: V, omega_rf, phi_rf (updated in place)
     const double *harmonic;       // [n_turns + 1][n_rf]

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "blmi_common.h"
+#include "feedback.h"
 
 namespace blmi {
 
@@ -62,6 +63,10 @@ struct InducedArgs {
     double2 *tab;
     double *tab_params;
     const double *gl; // direct method: g[(k - (ns-1)) mod n_fft], g = -scale * irfft(Z)
+    // direct method: one extra workgroup runs this turn's beam phase loop
+    // (feedback.h) beside the convolution
+    int with_fb;
+    FeedbackArgs fb;
 };
 // Single workgroup: LDS Stockham rfft, impedance product, irfft.
 int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream_t s,

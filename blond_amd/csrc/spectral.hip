@@ -16,6 +16,7 @@
 #include "blmi_common.h"
 #include "particles.h"
 #include "spectral.h"
+#include "feedback_device.h"
 
 namespace blmi {
 
@@ -392,6 +393,11 @@ constexpr int kConvC = 16;
 constexpr int kConvThreads = 512;
 
 __global__ void __launch_bounds__(kConvThreads) induced_direct_kernel(InducedArgs a) {
+    if (a.with_fb && blockIdx.x == gridDim.x - 1) {
+        // the beam phase loop of this turn, independent of the convolution
+        feedback_turn_body(a.fb);
+        return;
+    }
     extern __shared__ double conv_lds[];
     __shared__ double red[kConvThreads / 64][kConvC + 1];
     __shared__ double vout[kConvC + 1];
@@ -505,7 +511,9 @@ static int set_lds_limit(const void *fn, size_t bytes);
 int launch_induced_direct(const InducedArgs &a, hipStream_t s, const KernelEvents &ev) {
     if (!a.gl || a.n_out != a.n_slices || a.n_slices < 1)
         return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: bad arguments");
-    const int blocks = (a.n_slices + kConvC - 1) / kConvC;
+    const int blocks = (a.n_slices + kConvC - 1) / kConvC + (a.with_fb ? 1 : 0);
+    if (a.with_fb && a.fb.n_rf > kConvThreads)
+        return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: too many RF systems for the loop");
     const size_t lds = sizeof(double) * (2 * (size_t)a.n_slices + kConvC);
     if (lds + sizeof(double) * (kConvThreads / 64 + 1) * (kConvC + 1) > (size_t)device_max_lds())
         return blmi_set_error(BLMI_ERR_UNSUPPORTED, "direct induced voltage: %d slices exceed LDS",
