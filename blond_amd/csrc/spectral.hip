@@ -382,27 +382,25 @@ __global__ void induced_lds_batch_kernel(const InducedArgs *__restrict__ args, i
 // ---------------------------------------------------------------------------
 // one-turn induced voltage by direct convolution over all CUs
 //   V[i] = sum_m q[m] gl[i + m],  q[m] = p[ns-1-m],  gl[k] = g[(k-(ns-1)) mod n_fft]
-// Each workgroup produces kConvC outputs plus the next one (needed by the
+// Each workgroup produces C outputs plus the next one (needed by the
 // last table entry).  The profile (reversed) and the workgroup's window of gl
 // are staged in LDS with coalesced loads; thread l then takes samples
 // m = l, l + blockDim, ... so that every LDS read of a wavefront hits consecutive
 // words, and the partial sums are reduced in a fixed order (deterministic).
-// LDS: (2 ns + kConvC) doubles.
+// LDS: (2 ns + C) doubles.
 // ---------------------------------------------------------------------------
-constexpr int kConvC = 16;
-constexpr int kConvThreads = 512;
-
-__global__ void __launch_bounds__(kConvThreads) induced_direct_kernel(InducedArgs a) {
+template <int C, int TH>
+__global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     if (a.with_fb && blockIdx.x == gridDim.x - 1) {
         // the beam phase loop of this turn, independent of the convolution
         feedback_turn_body(a.fb);
         return;
     }
     extern __shared__ double conv_lds[];
-    __shared__ double red[kConvThreads / 64][kConvC + 1];
-    __shared__ double vout[kConvC + 1];
+    __shared__ double red[TH / 64][C + 1];
+    __shared__ double vout[C + 1];
     const int ns = a.n_slices;
-    const int i0 = blockIdx.x * kConvC;
+    const int i0 = blockIdx.x * C;
     const int lim = 2 * ns - 1;
     const unsigned *cin = nullptr;
     unsigned *cclr = nullptr;
@@ -412,7 +410,7 @@ __global__ void __launch_bounds__(kConvThreads) induced_direct_kernel(InducedArg
         cclr = (t & 1) ? a.counts0 : a.counts1;
     }
     double *q = conv_lds;        // q[m] = p[ns - 1 - m]
-    double *wv = conv_lds + ns;  // wv[k] = gl[i0 + k], k < ns + kConvC
+    double *wv = conv_lds + ns;  // wv[k] = gl[i0 + k], k < ns + C
     // staging: every thread issues all its global loads before its LDS
     // stores, so the loads of a pass are in flight together
     constexpr int kStage = 8;
@@ -442,44 +440,44 @@ __global__ void __launch_bounds__(kConvThreads) induced_direct_kernel(InducedArg
             if (m < ns) q[m] = v[u];
         }
     }
-    for (int b0 = 0; b0 < ns + kConvC; b0 += T * kStage) {
+    for (int b0 = 0; b0 < ns + C; b0 += T * kStage) {
         double v[kStage];
 #pragma unroll
         for (int u = 0; u < kStage; ++u) {
             const int k = b0 + u * T + (int)threadIdx.x;
             const int idx = i0 + k;
-            v[u] = (k < ns + kConvC && idx < lim) ? a.gl[idx] : 0.0;
+            v[u] = (k < ns + C && idx < lim) ? a.gl[idx] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < kStage; ++u) {
             const int k = b0 + u * T + (int)threadIdx.x;
-            if (k < ns + kConvC) wv[k] = v[u];
+            if (k < ns + C) wv[k] = v[u];
         }
     }
     __syncthreads();
-    double acc[kConvC + 1];
+    double acc[C + 1];
 #pragma unroll
-    for (int c = 0; c <= kConvC; ++c) acc[c] = 0.0;
+    for (int c = 0; c <= C; ++c) acc[c] = 0.0;
     for (int m = threadIdx.x; m < ns; m += blockDim.x) {
         const double qm = q[m];
 #pragma unroll
-        for (int c = 0; c <= kConvC; ++c) acc[c] = fma(qm, wv[m + c], acc[c]);
+        for (int c = 0; c <= C; ++c) acc[c] = fma(qm, wv[m + c], acc[c]);
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int c = 0; c <= kConvC; ++c) {
+    for (int c = 0; c <= C; ++c) {
         double v = acc[c];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         if (lane == 0) red[wid][c] = v;
     }
     __syncthreads();
-    if (threadIdx.x <= kConvC) {
+    if (threadIdx.x <= C) {
         double v = 0.0;
-        for (int w = 0; w < kConvThreads / 64; ++w) v += red[w][threadIdx.x];
+        for (int w = 0; w < TH / 64; ++w) v += red[w][threadIdx.x];
         vout[threadIdx.x] = v;
     }
     __syncthreads();
-    if (threadIdx.x < kConvC) {
+    if (threadIdx.x < C) {
         const int i = i0 + threadIdx.x;
         if (i < ns) {
             a.volt[i] = vout[threadIdx.x];
@@ -495,7 +493,7 @@ __global__ void __launch_bounds__(kConvThreads) induced_direct_kernel(InducedArg
             a.tab_params[0] = a.bc[0];
             a.tab_params[1] = inv_bw;
         }
-        if (threadIdx.x < kConvC) {
+        if (threadIdx.x < C) {
             const int i = i0 + threadIdx.x;
             if (i < ns - 1) {
                 const double vi = vout[threadIdx.x], vi1 = vout[threadIdx.x + 1];
@@ -508,20 +506,37 @@ __global__ void __launch_bounds__(kConvThreads) induced_direct_kernel(InducedArg
 
 static int set_lds_limit(const void *fn, size_t bytes);
 
+template <int C, int TH>
+static int launch_direct_shape(const InducedArgs &a, hipStream_t s, const KernelEvents &ev) {
+    const int blocks = (a.n_slices + C - 1) / C + (a.with_fb ? 1 : 0);
+    if (a.with_fb && a.fb.n_rf > TH)
+        return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: too many RF systems for the loop");
+    const size_t lds = sizeof(double) * (2 * (size_t)a.n_slices + C);
+    if (lds + sizeof(double) * (TH / 64 + 1) * (C + 1) > (size_t)device_max_lds())
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "direct induced voltage: %d slices exceed LDS",
+                              a.n_slices);
+    int rc = set_lds_limit((const void *)induced_direct_kernel<C, TH>, lds);
+    if (rc) return rc;
+    launch(induced_direct_kernel<C, TH>, dim3(blocks), dim3(TH), lds, s, ev, a);
+    return blmi_check_launch("induced_direct");
+}
+
+// Outputs per workgroup and workgroup size of the direct convolution: 8
+// outputs (twice the workgroups) up to 2048 slices, else 16 (measured on a
+// 4e6 bunch: 1024 slices 8.6 against 11.6 us; 4096 slices 13.9 us with 16,
+// 14.6 with 8; 256- and 1024-thread groups and 4 or 32 outputs were slower).
+static void direct_shape(int n_slices, int *c, int *th) {
+    *c = n_slices <= 2048 ? 8 : 16;
+    *th = 512;
+}
+
 int launch_induced_direct(const InducedArgs &a, hipStream_t s, const KernelEvents &ev) {
     if (!a.gl || a.n_out != a.n_slices || a.n_slices < 1)
         return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: bad arguments");
-    const int blocks = (a.n_slices + kConvC - 1) / kConvC + (a.with_fb ? 1 : 0);
-    if (a.with_fb && a.fb.n_rf > kConvThreads)
-        return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: too many RF systems for the loop");
-    const size_t lds = sizeof(double) * (2 * (size_t)a.n_slices + kConvC);
-    if (lds + sizeof(double) * (kConvThreads / 64 + 1) * (kConvC + 1) > (size_t)device_max_lds())
-        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "direct induced voltage: %d slices exceed LDS",
-                              a.n_slices);
-    int rc = set_lds_limit((const void *)induced_direct_kernel, lds);
-    if (rc) return rc;
-    launch(induced_direct_kernel, dim3(blocks), dim3(kConvThreads), lds, s, ev, a);
-    return blmi_check_launch("induced_direct");
+    int c = 16, th = 512;
+    direct_shape(a.n_slices, &c, &th);
+    if (c == 8 && th == 512) return launch_direct_shape<8, 512>(a, s, ev);
+    return launch_direct_shape<16, 512>(a, s, ev);
 }
 
 // ---------------------------------------------------------------------------
