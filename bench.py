@@ -16,7 +16,8 @@ Prints one JSON line with the contract keys plus
                 cores, same turn loop with numpy FFTs, bounded sample
   forms         the other forms of the metric, each on its own workload:
                   large   c3: 1e7 macroparticles, 2 RF systems, 65536 slices,
-                          FFT 131072 (four-step), fused intensity turn
+                          FFT 131072 (four-step), fused intensity turn; with
+                          3e7 macroparticles and the FP32 mode as variants
                   batched c4: 256 bunches (1e4..1e6 each), one launch pair per
                           turn for all bunches, autotuned, against 256
                           sequential fused runs of the same bunches
@@ -93,24 +94,42 @@ def _timed(fn, sync):
     return time.perf_counter() - t0
 
 
-def form_large(torch, turns=200) -> dict:
-    """c3: 1e7 macroparticles, 65536 slices, FFT 131072, fused intensity turn."""
+def _large(torch, n_particles, f32, turns):
     from blond_amd.trackers.fused import FusedTracker
     from blond_amd.workloads import build_workload
 
-    w = build_workload("c3", n_particles=10_000_000, n_turns=turns + 80)
+    w = build_workload("c3", n_particles=n_particles, n_turns=turns + 80)
+    if f32:
+        # the FP32 mode: float coordinates (the slice axis stays FP64)
+        w.beam.dt = w.beam.dt.to(torch.float32).contiguous()
+        w.beam.dE = w.beam.dE.to(torch.float32).contiguous()
     n = w.beam.n_macroparticles
+    bpp = BYTES_PER_PARTICLE_TURN // 2 if f32 else BYTES_PER_PARTICLE_TURN
     ft = FusedTracker(w.tracker, w.profile, w.tiv)
     ft.track(10)
     el = _timed(lambda: ft.track(turns), torch.cuda.synchronize)
     ms_ind, ms_turn = ft.time_kernels(50)
-    out = {"workload": "c3: 1e7 macroparticles, h=35640+142560, 65536 slices, n_fft 131072",
+    out = {"workload": f"c3: {n_particles:.0e} macroparticles, h=35640+142560, 65536 slices, "
+                       f"n_fft 131072, {'FP32' if f32 else 'FP64'}",
            "value": n * turns / el, "unit": "macroparticle-turns/s", "turns": turns,
            "ms_per_turn": 1e3 * el / turns, "induced_method": ft.method,
+           "bytes_per_macroparticle_turn": bpp,
            "particle_kernel_ms": ms_turn, "induced_ms": ms_ind,
-           "particle_kernel_frac": BYTES_PER_PARTICLE_TURN * n / (ms_turn * 1e-3) / 1e9 / HBM_PEAK_GBS,
-           "whole_turn_frac": BYTES_PER_PARTICLE_TURN * n / (el / turns) / 1e9 / HBM_PEAK_GBS}
+           "particle_kernel_frac": bpp * n / (ms_turn * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "whole_turn_frac": bpp * n / (el / turns) / 1e9 / HBM_PEAK_GBS}
     ft.close()
+    del w
+    torch.cuda.empty_cache()
+    return out
+
+
+def form_large(torch, turns=200) -> dict:
+    """c3: 1e7 and 3e7 macroparticles, 65536 slices, FFT 131072, fused
+    intensity turn, FP64 and FP32 (the 1e7 FP64 figures at the top level)."""
+    out = _large(torch, 10_000_000, False, turns)
+    out["variants"] = {"3e7_f64": _large(torch, 30_000_000, False, turns // 2),
+                       "1e7_f32": _large(torch, 10_000_000, True, turns),
+                       "3e7_f32": _large(torch, 30_000_000, True, turns // 2)}
     return out
 
 

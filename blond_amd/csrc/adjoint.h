@@ -44,6 +44,12 @@ struct Adjoint {
     int grid = 0, block = 256;
     bool forward_done = false;
     int forward_turns = 0;
+    // the forward and reverse passes are captured once per turn count and
+    // replayed as HIP graphs on the adjoint's own stream
+    hipStream_t own = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraphExec_t fwd_exec = nullptr, bwd_exec = nullptr;
+    int fwd_graph_turns = -1, bwd_graph_turns = -1;
 
     ~Adjoint();
     void release();
@@ -56,6 +62,10 @@ struct Adjoint {
                      const double *energy, double charge);
     int set_slices(double cut_left, double cut_right);
     int set_wake(int64_t n_fft, const double *g_host, double scale);
+    void invalidate_graphs();
+    int ensure_stream();
+    int enqueue_forward(int turns, hipStream_t s);
+    int enqueue_backward(int T, hipStream_t s);
     int forward(int turns, double *profile_out, hipStream_t s);
     int backward(const double *dprofile, double *grad_rf_host, double *corr_host, hipStream_t s);
 };

@@ -334,7 +334,40 @@ __global__ void __launch_bounds__(kAdjOneBlock) adj_iv_kernel(AdjArgs a, int t, 
 // ---------------------------------------------------------------------------
 Adjoint::~Adjoint() { release(); }
 
+void Adjoint::invalidate_graphs() {
+    if (fwd_exec) (void)hipGraphExecDestroy(fwd_exec);
+    if (bwd_exec) (void)hipGraphExecDestroy(bwd_exec);
+    fwd_exec = bwd_exec = nullptr;
+    fwd_graph_turns = bwd_graph_turns = -1;
+}
+
+int Adjoint::ensure_stream() {
+    if (own) return BLMI_OK;
+    BLMI_HIP_TRY(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+    BLMI_HIP_TRY(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+    BLMI_HIP_TRY(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+    return BLMI_OK;
+}
+
+// dst[i] = src[i] (a kernel node: the passes are graph-captured)
+__global__ void adj_copy_kernel(double *__restrict__ dst, const double *__restrict__ src,
+                                int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
+static void adj_copy(double *dst, const double *src, int64_t n, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(adj_copy_kernel, dim3(grid), dim3(256), 0, s, dst, src, n);
+}
+
 void Adjoint::release() {
+    invalidate_graphs();
+    if (own) (void)hipStreamDestroy(own);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    own = nullptr;
+    ev_in = ev_out = nullptr;
     void *bufs[] = {d_rf, d_acc, d_drift, d_x0, d_work, d_ckpt, d_slots, d_lam, d_dep,
                     d_prof, d_vt, d_g, d_lv, d_lp, d_corr, d_gpart};
     for (void *p : bufs) (void)hipFree(p);
@@ -389,6 +422,8 @@ int Adjoint::set_programs(const double *rf_host, const double *acc_host, const d
                           const double *alpha1, const double *alpha2, const double *beta,
                           const double *energy, double charge_) {
     const size_t nt = (size_t)n_turns + 1;
+    // the captured passes hold scalars by value: a changed one re-captures
+    if (charge_ != charge) invalidate_graphs();
     charge = charge_;
     BLMI_HIP_TRY(hipMemcpy(d_rf, rf_host, sizeof(double) * 3 * n_rf * nt, hipMemcpyHostToDevice));
     BLMI_HIP_TRY(hipMemcpy(d_acc, acc_host, sizeof(double) * nt, hipMemcpyHostToDevice));
@@ -404,8 +439,10 @@ int Adjoint::set_programs(const double *rf_host, const double *acc_host, const d
 }
 
 int Adjoint::set_slices(double cl, double cr) {
+    const double ibw = (double)n_slices / (cr - cl);
+    if (cl != cut_left || ibw != inv_bw_cut) invalidate_graphs();
     cut_left = cl;
-    inv_bw_cut = (double)n_slices / (cr - cl);
+    inv_bw_cut = ibw;
     forward_done = false;
     return BLMI_OK;
 }
@@ -414,6 +451,7 @@ int Adjoint::set_wake(int64_t n_fft_, const double *g_host, double scale_) {
     if (n_fft_ < n_slices || n_fft_ > 8192)
         return blmi_set_error(BLMI_ERR_UNSUPPORTED, "adjoint: n_fft %lld outside [n_slices, 8192]",
                               (long long)n_fft_);
+    if (n_fft_ != n_fft || scale_ != scale) invalidate_graphs();
     if (n_fft_ != n_fft) {
         (void)hipFree(d_g);
         (void)hipFree(d_corr);
@@ -447,55 +485,39 @@ static AdjArgs adj_args(const Adjoint &j) {
     return a;
 }
 
-int Adjoint::forward(int turns, double *profile_out, hipStream_t s) {
-    if (!d_g) return blmi_set_error(BLMI_ERR_ARG, "adjoint: set the wake first");
-    if (turns < 1 || turns > n_turns)
-        return blmi_set_error(BLMI_ERR_ARG, "adjoint: turns %d outside [1, %d]", turns, n_turns);
+int Adjoint::enqueue_forward(int turns, hipStream_t s) {
     const AdjArgs a = adj_args(*this);
     const size_t pn = 2 * (size_t)n;
     const size_t lds_dep = 2 * sizeof(double) * n_slices;  // deposit + V_t
     const size_t lds_one = sizeof(double) * ((size_t)n_slices + n_fft);
     double *x = d_work, *e = d_work + n;
-    BLMI_HIP_TRY(hipMemcpyAsync(d_work, d_x0, sizeof(double) * pn, hipMemcpyDeviceToDevice, s));
+    adj_copy(d_work, d_x0, (int64_t)pn, grid, s);
     hipLaunchKernelGGL(adj_deposit_kernel, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, n, a);
     hipLaunchKernelGGL(adj_profile_kernel, dim3(1), dim3(kAdjOneBlock), lds_one, s, a, 0, true,
                        d_prof, d_g, (int)n_fft, scale, d_vt);
     for (int t = 0; t < turns; ++t) {
-        if (t % every == 0)
-            BLMI_HIP_TRY(hipMemcpyAsync(d_ckpt + (size_t)(t / every) * pn, d_work,
-                                        sizeof(double) * pn, hipMemcpyDeviceToDevice, s));
+        if (t % every == 0) adj_copy(d_ckpt + (size_t)(t / every) * pn, d_work, (int64_t)pn, grid, s);
         hipLaunchKernelGGL(adj_turn_kernel<true>, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, e, x,
                            e, n, t, a);
         hipLaunchKernelGGL(adj_profile_kernel, dim3(1), dim3(kAdjOneBlock), lds_one, s, a, t + 1,
                            t + 1 < turns, d_prof, d_g, (int)n_fft, scale, d_vt);
     }
-    int rc = blmi_check_launch("adjoint_forward");
-    if (rc) return rc;
-    BLMI_HIP_TRY(hipMemcpyAsync(profile_out, d_prof + (size_t)turns * n_slices,
-                                sizeof(double) * n_slices, hipMemcpyDeviceToDevice, s));
-    forward_done = true;
-    forward_turns = turns;
-    return BLMI_OK;
+    return blmi_check_launch("adjoint_forward");
 }
 
-int Adjoint::backward(const double *dprofile, double *grad_rf_host, double *corr_host,
-                      hipStream_t s) {
-    if (!forward_done) return blmi_set_error(BLMI_ERR_ARG, "adjoint: run forward first");
-    const int T = forward_turns;
+int Adjoint::enqueue_backward(int T, hipStream_t s) {
     const size_t pn = 2 * (size_t)n;
     const size_t lds_dep = 2 * sizeof(double) * n_slices;
     const size_t lds_rev = sizeof(double) * 3 * (size_t)n_slices;
     const size_t lds_one = sizeof(double) * (2 * (size_t)n_slices + n_fft);
     AdjArgs a = adj_args(*this);
-    BLMI_HIP_TRY(hipMemsetAsync(d_lam, 0, sizeof(double) * pn, s));
-    BLMI_HIP_TRY(hipMemsetAsync(d_corr, 0, sizeof(double) * n_fft, s));
-    BLMI_HIP_TRY(hipMemcpyAsync(d_lp, dprofile, sizeof(double) * n_slices, hipMemcpyDeviceToDevice,
-                                s));
+    int rc = launch_fill<double>(d_lam, (int64_t)pn, 0.0, s);
+    if (!rc) rc = launch_fill<double>(d_corr, n_fft, 0.0, s);
+    if (rc) return rc;
     for (int seg = (T - 1) / every; seg >= 0; --seg) {
         const int t0 = seg * every, t1 = (t0 + every < T) ? t0 + every : T;
         // states t0 .. t1-1 into slots 0 .. t1-t0-1
-        BLMI_HIP_TRY(hipMemcpyAsync(d_slots, d_ckpt + (size_t)seg * pn, sizeof(double) * pn,
-                                    hipMemcpyDeviceToDevice, s));
+        adj_copy(d_slots, d_ckpt + (size_t)seg * pn, (int64_t)pn, grid, s);
         for (int t = t0; t < t1 - 1; ++t) {
             double *si = d_slots + (size_t)(t - t0) * pn, *so = si + pn;
             hipLaunchKernelGGL(adj_turn_kernel<false>, dim3(grid), dim3(kAdjBlock), lds_dep, s, si,
@@ -509,13 +531,78 @@ int Adjoint::backward(const double *dprofile, double *grad_rf_host, double *corr
                                d_g, (int)n_fft, scale, d_lp, d_corr);
         }
     }
-    int rc = blmi_check_launch("adjoint_backward");
+    return blmi_check_launch("adjoint_backward");
+}
+
+// Capture `enqueue` on the own stream into *exec (once per turn count).
+template <typename F>
+static int capture(hipStream_t own, hipGraphExec_t *exec, F enqueue) {
+    BLMI_HIP_TRY(hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(own, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess)
+        return blmi_set_error(BLMI_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipError_t ie = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess)
+        return blmi_set_error(BLMI_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    return BLMI_OK;
+}
+
+int Adjoint::forward(int turns, double *profile_out, hipStream_t user) {
+    if (!d_g) return blmi_set_error(BLMI_ERR_ARG, "adjoint: set the wake first");
+    if (turns < 1 || turns > n_turns)
+        return blmi_set_error(BLMI_ERR_ARG, "adjoint: turns %d outside [1, %d]", turns, n_turns);
+    int rc = ensure_stream();
     if (rc) return rc;
+    if (!fwd_exec || fwd_graph_turns != turns) {
+        if (fwd_exec) (void)hipGraphExecDestroy(fwd_exec);
+        fwd_exec = nullptr;
+        rc = capture(own, &fwd_exec, [&] { return enqueue_forward(turns, own); });
+        if (rc) return rc;
+        fwd_graph_turns = turns;
+    }
+    BLMI_HIP_TRY(hipEventRecord(ev_in, user));
+    BLMI_HIP_TRY(hipStreamWaitEvent(own, ev_in, 0));
+    BLMI_HIP_TRY(hipGraphLaunch(fwd_exec, own));
+    BLMI_HIP_TRY(hipMemcpyAsync(profile_out, d_prof + (size_t)turns * n_slices,
+                                sizeof(double) * n_slices, hipMemcpyDeviceToDevice, own));
+    BLMI_HIP_TRY(hipEventRecord(ev_out, own));
+    BLMI_HIP_TRY(hipStreamWaitEvent(user, ev_out, 0));
+    forward_done = true;
+    forward_turns = turns;
+    return BLMI_OK;
+}
+
+int Adjoint::backward(const double *dprofile, double *grad_rf_host, double *corr_host,
+                      hipStream_t user) {
+    if (!forward_done) return blmi_set_error(BLMI_ERR_ARG, "adjoint: run forward first");
+    const int T = forward_turns;
+    int rc = ensure_stream();
+    if (rc) return rc;
+    if (!bwd_exec || bwd_graph_turns != T) {
+        if (bwd_exec) (void)hipGraphExecDestroy(bwd_exec);
+        bwd_exec = nullptr;
+        rc = capture(own, &bwd_exec, [&] { return enqueue_backward(T, own); });
+        if (rc) return rc;
+        bwd_graph_turns = T;
+    }
+    BLMI_HIP_TRY(hipEventRecord(ev_in, user));
+    BLMI_HIP_TRY(hipStreamWaitEvent(own, ev_in, 0));
+    BLMI_HIP_TRY(hipMemcpyAsync(d_lp, dprofile, sizeof(double) * n_slices, hipMemcpyDeviceToDevice,
+                                own));
+    BLMI_HIP_TRY(hipGraphLaunch(bwd_exec, own));
     std::vector<double> gp((size_t)T * grid * 2 * n_rf);
     BLMI_HIP_TRY(hipMemcpyAsync(gp.data(), d_gpart, sizeof(double) * gp.size(),
-                                hipMemcpyDeviceToHost, s));
-    BLMI_HIP_TRY(hipMemcpyAsync(corr_host, d_corr, sizeof(double) * n_fft, hipMemcpyDeviceToHost, s));
-    BLMI_HIP_TRY(hipStreamSynchronize(s));
+                                hipMemcpyDeviceToHost, own));
+    BLMI_HIP_TRY(hipMemcpyAsync(corr_host, d_corr, sizeof(double) * n_fft, hipMemcpyDeviceToHost,
+                                own));
+    BLMI_HIP_TRY(hipStreamSynchronize(own));
     for (int j = 0; j < 2 * n_rf; ++j) {
         double v = 0.0;
         for (int t = 0; t < T; ++t)
