@@ -382,15 +382,25 @@ __global__ void induced_lds_batch_kernel(const InducedArgs *__restrict__ args, i
 // ---------------------------------------------------------------------------
 // one-turn induced voltage by direct convolution over all CUs
 //   V[i] = sum_m q[m] gl[i + m],  q[m] = p[ns-1-m],  gl[k] = g[(k-(ns-1)) mod n_fft]
-// Each workgroup produces C outputs plus the next one (needed by the
-// last table entry).  The profile (reversed) and the workgroup's window of gl
-// are staged in LDS with coalesced loads; thread l then takes samples
-// m = l, l + blockDim, ... so that every LDS read of a wavefront hits consecutive
-// words, and the partial sums are reduced in a fixed order (deterministic).
-// LDS: (2 ns + C) doubles.
+// Workgroup b produces the C outputs i0 = b C .. i0 + C - 1 plus output
+// i0 + C (needed by the last table entry).  The reversed profile q and the
+// workgroup's window w[k] = gl[i0 + k] sit in LDS with one pad word per 8
+// (entry k at k + k/8).  Thread l takes CHUNK consecutive samples
+// m = s + CHUNK l + j of every span s of CHUNK*TH samples: it loads
+// w[m .. m + CHUNK + C) into registers once and reuses each value for up to
+// C + 1 outputs, so a sample costs about two LDS reads instead of C + 2, and
+// with the padding lanes CHUNK samples apart fall on distinct bank pairs.
+// A wavefront's C + 1 sums are reduced by a butterfly that halves the values
+// each lane carries at every exchange, then across wavefronts in LDS, in a
+// fixed order (deterministic).
+// LDS: pad(nq) + pad(nq + C - 1) + 1 doubles, nq = ns rounded up to a span.
 // ---------------------------------------------------------------------------
-template <int C, int TH>
+__device__ __forceinline__ int conv_pad(int k) { return k + (k >> 3); }
+
+template <int C, int TH, int CHUNK>
 __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
+    static_assert(C >= 2 && C <= 32 && (C & (C - 1)) == 0, "C is a power of two up to 32");
+    static_assert(CHUNK % 8 == 0, "a thread's samples start on a pad period");
     if (a.with_fb && blockIdx.x == gridDim.x - 1) {
         // the beam phase loop of this turn, independent of the convolution
         feedback_turn_body(a.fb);
@@ -399,9 +409,18 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     extern __shared__ double conv_lds[];
     __shared__ double red[TH / 64][C + 1];
     __shared__ double vout[C + 1];
+    constexpr int kSpan = CHUNK * TH;
     const int ns = a.n_slices;
     const int i0 = blockIdx.x * C;
     const int lim = 2 * ns - 1;
+    const int nq = (ns + kSpan - 1) / kSpan * kSpan;
+    double *q = conv_lds;                  // q[pad(m)], zero for m >= ns
+    double *wv = conv_lds + conv_pad(nq);  // wv[pad(k)] = gl[i0 + k], k < nq + C
+    // the window does not depend on the turn: its loads are issued first
+    for (int k = threadIdx.x; k < nq + C; k += TH) {
+        const int idx = i0 + k;
+        wv[conv_pad(k)] = (k < ns + C && idx < lim) ? a.gl[idx] : 0.0;
+    }
     const unsigned *cin = nullptr;
     unsigned *cclr = nullptr;
     if (a.counts0) {
@@ -409,67 +428,63 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
         cin = (t & 1) ? a.counts1 : a.counts0;
         cclr = (t & 1) ? a.counts0 : a.counts1;
     }
-    double *q = conv_lds;        // q[m] = p[ns - 1 - m]
-    double *wv = conv_lds + ns;  // wv[k] = gl[i0 + k], k < ns + C
-    // staging: every thread issues all its global loads before its LDS
-    // stores, so the loads of a pass are in flight together
-    constexpr int kStage = 8;
-    const int T = blockDim.x;
-    for (int b0 = 0; b0 < ns; b0 += T * kStage) {
-        double v[kStage];
+    for (int m = threadIdx.x; m < nq; m += TH) {
+        double v = 0.0;
+        if (m < ns) {
+            const int j = ns - 1 - m;
+            if (cin) {
+                unsigned s = 0;
 #pragma unroll
-        for (int u = 0; u < kStage; ++u) {
-            const int m = b0 + u * T + (int)threadIdx.x;
-            v[u] = 0.0;
-            if (m < ns) {
-                const int j = ns - 1 - m;
-                if (cin) {
-                    unsigned s = 0;
-#pragma unroll
-                    for (int g = 0; g < kMaxCountGroups; ++g)
-                        if (g < a.groups) s += cin[(size_t)g * ns + j];
-                    v[u] = (double)s;
-                } else {
-                    v[u] = a.src[j];
-                }
+                for (int g = 0; g < kMaxCountGroups; ++g)
+                    if (g < a.groups) s += cin[(size_t)g * ns + j];
+                v = (double)s;
+            } else {
+                v = a.src[j];
             }
         }
-#pragma unroll
-        for (int u = 0; u < kStage; ++u) {
-            const int m = b0 + u * T + (int)threadIdx.x;
-            if (m < ns) q[m] = v[u];
-        }
-    }
-    for (int b0 = 0; b0 < ns + C; b0 += T * kStage) {
-        double v[kStage];
-#pragma unroll
-        for (int u = 0; u < kStage; ++u) {
-            const int k = b0 + u * T + (int)threadIdx.x;
-            const int idx = i0 + k;
-            v[u] = (k < ns + C && idx < lim) ? a.gl[idx] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kStage; ++u) {
-            const int k = b0 + u * T + (int)threadIdx.x;
-            if (k < ns + C) wv[k] = v[u];
-        }
+        q[conv_pad(m)] = v;
     }
     __syncthreads();
     double acc[C + 1];
 #pragma unroll
     for (int c = 0; c <= C; ++c) acc[c] = 0.0;
-    for (int m = threadIdx.x; m < ns; m += blockDim.x) {
-        const double qm = q[m];
+    for (int s0 = 0; s0 < nq; s0 += kSpan) {
+        const int m0 = s0 + CHUNK * (int)threadIdx.x;
+        if (m0 < ns) {
+            const int pb = conv_pad(m0);  // m0 % 8 == 0: pad(m0 + j) = pb + j + j / 8
+            double w[CHUNK + C];
 #pragma unroll
-        for (int c = 0; c <= C; ++c) acc[c] = fma(qm, wv[m + c], acc[c]);
+            for (int j = 0; j < CHUNK + C; ++j) w[j] = wv[pb + j + (j >> 3)];
+#pragma unroll
+            for (int j = 0; j < CHUNK; ++j) {
+                const double qj = q[pb + j + (j >> 3)];
+#pragma unroll
+                for (int c = 0; c <= C; ++c) acc[c] = fma(qj, w[j + c], acc[c]);
+            }
+        }
     }
+    // butterfly over lane bits 5, 4, ...: at each exchange a lane keeps half
+    // of its sums and adds its partner's copy of them; after log2(C)
+    // exchanges a lane holds one sum (index idx) over 64/C lanes' worth
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int idx = 0, bit = 32;
 #pragma unroll
-    for (int c = 0; c <= C; ++c) {
-        double v = acc[c];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) red[wid][c] = v;
+    for (int half = C / 2; half >= 1; half >>= 1) {
+        const bool hi = lane & bit;
+#pragma unroll
+        for (int k = 0; k < half; ++k) {
+            const double send = hi ? acc[k] : acc[k + half];
+            const double keep = hi ? acc[k + half] : acc[k];
+            acc[k] = keep + __shfl_xor(send, bit);
+        }
+        if (hi) idx += half;
+        bit >>= 1;
     }
+    for (int b = bit; b > 0; b >>= 1) acc[0] += __shfl_xor(acc[0], b);
+    double last = acc[C];
+    for (int off = 32; off > 0; off >>= 1) last += __shfl_xor(last, off);
+    if ((lane & (64 / C - 1)) == 0) red[wid][idx] = acc[0];
+    if (lane == 0) red[wid][C] = last;
     __syncthreads();
     if (threadIdx.x <= C) {
         double v = 0.0;
@@ -481,7 +496,7 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
         const int i = i0 + threadIdx.x;
         if (i < ns) {
             a.volt[i] = vout[threadIdx.x];
-            if (a.profile_out) a.profile_out[i] = q[ns - 1 - i];
+            if (a.profile_out) a.profile_out[i] = q[conv_pad(ns - 1 - i)];
             if (cclr)
                 for (int g = 0; g < a.groups; ++g) cclr[(size_t)g * ns + i] = 0u;
         }
@@ -506,37 +521,34 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
 
 static int set_lds_limit(const void *fn, size_t bytes);
 
-template <int C, int TH>
+template <int C, int TH, int CHUNK>
 static int launch_direct_shape(const InducedArgs &a, hipStream_t s, const KernelEvents &ev) {
     const int blocks = (a.n_slices + C - 1) / C + (a.with_fb ? 1 : 0);
     if (a.with_fb && a.fb.n_rf > TH)
         return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: too many RF systems for the loop");
-    const size_t lds = sizeof(double) * (2 * (size_t)a.n_slices + C);
+    constexpr int kSpan = CHUNK * TH;
+    const int nq = (a.n_slices + kSpan - 1) / kSpan * kSpan;
+    auto pad = [](int k) { return (size_t)(k + (k >> 3)); };
+    const size_t lds = sizeof(double) * (pad(nq) + pad(nq + C - 1) + 1);
     if (lds + sizeof(double) * (TH / 64 + 1) * (C + 1) > (size_t)device_max_lds())
         return blmi_set_error(BLMI_ERR_UNSUPPORTED, "direct induced voltage: %d slices exceed LDS",
                               a.n_slices);
-    int rc = set_lds_limit((const void *)induced_direct_kernel<C, TH>, lds);
+    int rc = set_lds_limit((const void *)induced_direct_kernel<C, TH, CHUNK>, lds);
     if (rc) return rc;
-    launch(induced_direct_kernel<C, TH>, dim3(blocks), dim3(TH), lds, s, ev, a);
+    launch(induced_direct_kernel<C, TH, CHUNK>, dim3(blocks), dim3(TH), lds, s, ev, a);
     return blmi_check_launch("induced_direct");
 }
 
-// Outputs per workgroup and workgroup size of the direct convolution: 8
-// outputs (twice the workgroups) up to 2048 slices, else 16 (measured on a
-// 4e6 bunch: 1024 slices 8.6 against 11.6 us; 4096 slices 13.9 us with 16,
-// 14.6 with 8; 256- and 1024-thread groups and 4 or 32 outputs were slower).
-static void direct_shape(int n_slices, int *c, int *th) {
-    *c = n_slices <= 2048 ? 8 : 16;
-    *th = 512;
-}
-
+// Shape of the direct convolution (outputs per workgroup, workgroup size,
+// samples per thread), measured on one MI355X with a 4096-slice profile and
+// its neighbours: 1024 slices 5.0 us (16, 256, 8); 2048 slices 6.1 us and
+// 4096 slices 6.8 us (16, 512, 8); 8192 slices 11.6 us (32, 512, 16).
 int launch_induced_direct(const InducedArgs &a, hipStream_t s, const KernelEvents &ev) {
     if (!a.gl || a.n_out != a.n_slices || a.n_slices < 1)
         return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: bad arguments");
-    int c = 16, th = 512;
-    direct_shape(a.n_slices, &c, &th);
-    if (c == 8 && th == 512) return launch_direct_shape<8, 512>(a, s, ev);
-    return launch_direct_shape<16, 512>(a, s, ev);
+    if (a.n_slices <= 1024) return launch_direct_shape<16, 256, 8>(a, s, ev);
+    if (a.n_slices <= 4096) return launch_direct_shape<16, 512, 8>(a, s, ev);
+    return launch_direct_shape<32, 512, 16>(a, s, ev);
 }
 
 // ---------------------------------------------------------------------------
