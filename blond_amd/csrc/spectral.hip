@@ -416,11 +416,6 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     const int nq = (ns + kSpan - 1) / kSpan * kSpan;
     double *q = conv_lds;                  // q[pad(m)], zero for m >= ns
     double *wv = conv_lds + conv_pad(nq);  // wv[pad(k)] = gl[i0 + k], k < nq + C
-    // the window does not depend on the turn: its loads are issued first
-    for (int k = threadIdx.x; k < nq + C; k += TH) {
-        const int idx = i0 + k;
-        wv[conv_pad(k)] = (k < ns + C && idx < lim) ? a.gl[idx] : 0.0;
-    }
     const unsigned *cin = nullptr;
     unsigned *cclr = nullptr;
     if (a.counts0) {
@@ -428,21 +423,55 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
         cin = (t & 1) ? a.counts1 : a.counts0;
         cclr = (t & 1) ? a.counts0 : a.counts1;
     }
-    for (int m = threadIdx.x; m < nq; m += TH) {
-        double v = 0.0;
-        if (m < ns) {
-            const int j = ns - 1 - m;
-            if (cin) {
-                unsigned s = 0;
+    // Staging in batches of 8 entries per thread: every load of a batch
+    // (window, profile, and the window's last C entries with the last batch)
+    // is issued before the first LDS store, with clamped in-bounds indexes
+    // and the out-of-range values replaced afterwards, so the batch costs
+    // one memory round trip (plus one per extra count copy).
+    constexpr int kStage = 8;
+    for (int b = 0; b < nq; b += kStage * TH) {
+        double wx[kStage], qx[kStage];
 #pragma unroll
-                for (int g = 0; g < kMaxCountGroups; ++g)
-                    if (g < a.groups) s += cin[(size_t)g * ns + j];
-                v = (double)s;
-            } else {
-                v = a.src[j];
+        for (int u = 0; u < kStage; ++u) {
+            const int k = b + u * TH + (int)threadIdx.x;
+            const bool ok = k < ns + C && i0 + k < lim;
+            const double x = a.gl[ok ? i0 + k : 0];
+            wx[u] = ok ? x : 0.0;
+        }
+        const bool tail = b + kStage * TH >= nq && (int)threadIdx.x < C;
+        const int kt = nq + (int)threadIdx.x;
+        const bool okt = tail && kt < ns + C && i0 + kt < lim;
+        const double xt = a.gl[okt ? i0 + kt : 0];
+        if (cin) {
+            unsigned cs[kStage];
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) cs[u] = 0u;
+            for (int g = 0; g < a.groups; ++g) {
+                const unsigned *row = cin + (size_t)g * ns;
+#pragma unroll
+                for (int u = 0; u < kStage; ++u) {
+                    const int m = b + u * TH + (int)threadIdx.x;
+                    const unsigned x = row[m < ns ? ns - 1 - m : 0];
+                    cs[u] += m < ns ? x : 0u;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) qx[u] = (double)cs[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+                const int m = b + u * TH + (int)threadIdx.x;
+                const double x = a.src[m < ns ? ns - 1 - m : 0];
+                qx[u] = m < ns ? x : 0.0;
             }
         }
-        q[conv_pad(m)] = v;
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int k = b + u * TH + (int)threadIdx.x;  // k < nq: nq is a multiple of 8 TH
+            wv[conv_pad(k)] = wx[u];
+            q[conv_pad(k)] = qx[u];
+        }
+        if (tail) wv[conv_pad(kt)] = okt ? xt : 0.0;
     }
     __syncthreads();
     double acc[C + 1];
