@@ -211,6 +211,15 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
         bc0 = (T)a.tab_params[0];
         ibw = (T)a.tab_params[1];
     }
+    // the first pair of this thread is loaded before the LDS set-up below, so
+    // its memory latency overlaps the clearing and the table staging
+    const int64_t i_first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n2 = n / 2;
+    V2 x_first{}, e_first{};
+    if (i_first < n2) {
+        x_first = reinterpret_cast<const V2 *>(dt)[i_first];
+        e_first = reinterpret_cast<const V2 *>(dE)[i_first];
+    }
     // INTERP == 2: the table is staged in LDS after the histogram words
     const int hwords = (HIST && LDSH) ? hist_words(LDSH, a.n_slices) : 0;
     double2 *ltab = reinterpret_cast<double2 *>(lds_hist + ((hwords + 3) & ~3));
@@ -254,10 +263,15 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
             atomic_add_agent(&counts[bin], 1u);
     };
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t n2 = n / 2;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) {
-        V2 x = reinterpret_cast<const V2 *>(dt)[i];
-        V2 e = reinterpret_cast<const V2 *>(dE)[i];
+    for (int64_t i = i_first; i < n2; i += stride) {
+        V2 x, e;
+        if (i == i_first) {
+            x = x_first;
+            e = e_first;
+        } else {
+            x = reinterpret_cast<const V2 *>(dt)[i];
+            e = reinterpret_cast<const V2 *>(dE)[i];
+        }
         const int b0 = step(x.x, e.x);
         const int b1 = step(x.y, e.y);
         reinterpret_cast<V2 *>(dt)[i] = x;
@@ -714,8 +728,13 @@ int Engine::hist_layout(int block, int *grid_out, size_t *lds_out) const {
         // particles (+1 for an odd tail) within the u16 counters
         const int64_t max_pairs = (kMaxParticlesPerGroup16 - 1) / (2 * (int64_t)block);
         if (max_pairs >= 1) {
-            const int64_t g = (n2 + block * max_pairs - 1) / (block * max_pairs);
-            if (g > grid) grid = (int)g;
+            int64_t g = (n2 + block * max_pairs - 1) / (block * max_pairs);
+            if (g > grid) {
+                // whole waves of workgroups: every CU runs the same number
+                const int64_t wave = (int64_t)device_cu_count() * blocks_per_cu;
+                g = (g + wave - 1) / wave * wave;
+                grid = (int)g;
+            }
             mode = 2;
             lds = (size_t)hist_words(2, n_slices) * sizeof(unsigned);
         }
