@@ -152,7 +152,8 @@ def _form_turns(torch, w, turns, label) -> dict:
 
 def form_wake(torch, turns=200) -> dict:
     """c2 bunch with a 4-turn multi-turn wake and an inductive source; the
-    induced step is the LDS FFT (n_out = 16384) plus the wake step."""
+    induced step is the direct convolution over the memory's 16384 values
+    plus the wake step."""
     from blond_amd.impedances.impedance import InductiveImpedance
     from blond_amd.workloads import build_wake_workload
 

@@ -28,6 +28,7 @@ constexpr int kSortDigitShift = 6;
 // Largest slice count for which the induced voltage uses the direct
 // (Toeplitz) convolution: ns^2 FP64 FMAs spread over all CUs.
 constexpr int kDirectMaxSlices = 8192;
+constexpr double kDirectMaxWork = 67108864.0;  // outputs x slices of the direct method
 
 struct Engine {
     int device = 0;
@@ -62,6 +63,7 @@ struct Engine {
     Fft2Plan plan2{};
     double2 *d_Z = nullptr;
     double *d_gl = nullptr;        // direct method: -scale*irfft(Z), unrolled
+    int64_t gl_len = 0;            // entries of d_gl
     double scale = 0;
     double *volt = nullptr;        // borrowed: induced voltage seen by the caller
     const double *bc = nullptr;    // borrowed: bin centres

@@ -63,6 +63,7 @@ struct InducedArgs {
     double2 *tab;
     double *tab_params;
     const double *gl; // direct method: g[(k - (ns-1)) mod n_fft], g = -scale * irfft(Z)
+    int64_t gl_len;   // entries of gl (at least n_out + n_slices - 1)
     // direct method: one extra workgroup runs this turn's beam phase loop
     // (feedback.h) beside the convolution
     int with_fb;

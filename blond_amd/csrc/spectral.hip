@@ -382,7 +382,8 @@ __global__ void induced_lds_batch_kernel(const InducedArgs *__restrict__ args, i
 // ---------------------------------------------------------------------------
 // one-turn induced voltage by direct convolution over all CUs
 //   V[i] = sum_m q[m] gl[i + m],  q[m] = p[ns-1-m],  gl[k] = g[(k-(ns-1)) mod n_fft]
-// Workgroup b produces the C outputs i0 = b C .. i0 + C - 1 plus output
+// for i < n_out (n_slices for one turn, the multi-turn memory's length for a
+// wake).  Workgroup b produces the C outputs i0 = b C .. i0 + C - 1 plus output
 // i0 + C (needed by the last table entry).  The reversed profile q and the
 // workgroup's window w[k] = gl[i0 + k] sit in LDS with one pad word per 8
 // (entry k at k + k/8).  Thread l takes CHUNK consecutive samples
@@ -412,7 +413,7 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     constexpr int kSpan = CHUNK * TH;
     const int ns = a.n_slices;
     const int i0 = blockIdx.x * C;
-    const int lim = 2 * ns - 1;
+    const int lim = a.n_out + ns - 1;  // gl entries used (host checks gl_len)
     const int nq = (ns + kSpan - 1) / kSpan * kSpan;
     double *q = conv_lds;                  // q[pad(m)], zero for m >= ns
     double *wv = conv_lds + conv_pad(nq);  // wv[pad(k)] = gl[i0 + k], k < nq + C
@@ -523,8 +524,8 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     __syncthreads();
     if (threadIdx.x < C) {
         const int i = i0 + threadIdx.x;
+        if (i < a.n_out) a.volt[i] = vout[threadIdx.x];
         if (i < ns) {
-            a.volt[i] = vout[threadIdx.x];
             if (a.profile_out) a.profile_out[i] = q[conv_pad(ns - 1 - i)];
             if (cclr)
                 for (int g = 0; g < a.groups; ++g) cclr[(size_t)g * ns + i] = 0u;
@@ -552,7 +553,7 @@ static int set_lds_limit(const void *fn, size_t bytes);
 
 template <int C, int TH, int CHUNK>
 static int launch_direct_shape(const InducedArgs &a, hipStream_t s, const KernelEvents &ev) {
-    const int blocks = (a.n_slices + C - 1) / C + (a.with_fb ? 1 : 0);
+    const int blocks = (a.n_out + C - 1) / C + (a.with_fb ? 1 : 0);
     if (a.with_fb && a.fb.n_rf > TH)
         return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: too many RF systems for the loop");
     constexpr int kSpan = CHUNK * TH;
@@ -571,10 +572,14 @@ static int launch_direct_shape(const InducedArgs &a, hipStream_t s, const Kernel
 // Shape of the direct convolution (outputs per workgroup, workgroup size,
 // samples per thread), measured on one MI355X with a 4096-slice profile and
 // its neighbours: 1024 slices 5.0 us (16, 256, 8); 2048 slices 6.1 us and
-// 4096 slices 6.8 us (16, 512, 8); 8192 slices 11.6 us (32, 512, 16).
+// 4096 slices 6.8 us (16, 512, 8); 8192 slices 11.6 us (32, 512, 16).  A
+// multi-turn memory (n_out = 4 x 4096 values of 4096 slices) takes (32, 512,
+// 8): 21.8 against 22.5 us for (16, 512, 8) with its wake step.
 int launch_induced_direct(const InducedArgs &a, hipStream_t s, const KernelEvents &ev) {
-    if (!a.gl || a.n_out != a.n_slices || a.n_slices < 1)
+    if (!a.gl || a.n_slices < 1 || a.n_out < a.n_slices ||
+        (int64_t)a.n_out + a.n_slices - 1 > a.gl_len || (a.tab && a.n_out != a.n_slices))
         return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: bad arguments");
+    if (a.n_out > a.n_slices && a.n_slices > 1024) return launch_direct_shape<32, 512, 8>(a, s, ev);
     if (a.n_slices <= 1024) return launch_direct_shape<16, 256, 8>(a, s, ev);
     if (a.n_slices <= 4096) return launch_direct_shape<16, 512, 8>(a, s, ev);
     return launch_direct_shape<32, 512, 16>(a, s, ev);

@@ -117,11 +117,21 @@ def test_wake_first_turn_fused_exact_profile(kind):
     _check_sources(w, ref)
 
 
+CONV_KINDS = ("mtw", "mtw_front", "mtw_time", "mixed")
+
+
 @pytest.mark.parametrize("kind", WAKE_KINDS)
-def test_wake_turns_fused(kind):
+@pytest.mark.parametrize("method", [None, "fft"])
+def test_wake_turns_fused(kind, method):
+    """The automatic method (the direct convolution forming the whole
+    memory at these sizes) and the LDS FFT."""
+    if method is not None and kind not in CONV_KINDS:
+        pytest.skip("no impedance convolution in this workload")
     w = build_wake_workload(kind, n_particles=40001, n_turns=30)
     ref = orc.track(w.snapshot(), 20)
-    w.fused(20)
+    ft = w.fused(20, method=method)
+    if kind in CONV_KINDS:
+        assert ft.method == (method or "direct")
     _check_intensity(w, ref, 1e-9)
     _check_sources(w, ref)
 
@@ -147,24 +157,28 @@ def test_mtw_fused_graph_vs_eager_and_split_calls():
     assert a.rf.counter[0] == 45
 
 
-def test_mtw_fused_large_memory_fourstep():
-    """4096 slices with a 4-turn memory (n_fft 16384): the direct method
-    cannot write the memory, so the engine takes an FFT method; the
-    four-step transform gives the same turns within the FFT bar."""
+@pytest.mark.parametrize("method", [None, "fourstep", "fft"])
+def test_mtw_fused_large_memory(method):
+    """4096 slices with a 4-turn memory (n_fft 16384, 16384 memory values):
+    the engine's own choice is the direct convolution over all 16384
+    outputs; the four-step and LDS transforms give the same turns within
+    the induced-voltage bar."""
     w = build_wake_workload("mtw", n_particles=200001, n_turns=8, n_slices=4096)
     ref = orc.track(w.snapshot(), 4)
-    ft = w.fused(4)
-    assert ft.method in ("fft", "fourstep")
+    ft = w.fused(0, method=method)
+    ft.track(4)
+    assert ft.method == (method or "direct")
     _check_intensity(w, ref, 1e-9)
     _check_sources(w, ref)
-    w2 = build_wake_workload("mtw", n_particles=200001, n_turns=8, n_slices=4096)
-    ft2 = w2.fused(0, method="fourstep")
-    ft2.track(4)
-    assert ft2.method == "fourstep"
-    _check_intensity(w2, ref, 1e-9)
-    _check_sources(w2, ref)
-    with pytest.raises(_lib.BlmiError, match="direct"):
-        _lib.call("blmi_engine_set_method", ft2._h, 1)
+
+
+def test_mtw_direct_beyond_work_limit_takes_fft():
+    """A memory of 8 x 8192 values over 8192 slices (2^29 products a turn)
+    is past the direct method's work limit: the engine takes a transform."""
+    w = build_wake_workload("mtw", n_particles=20001, n_turns=4, n_slices=8192,
+                            memory_turns=8)
+    ft = w.fused(1)
+    assert ft.method in ("fft", "fourstep")
 
 
 def test_engine_wake_arguments_rejected():
