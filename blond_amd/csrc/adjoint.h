@@ -10,8 +10,8 @@
 
 namespace blmi {
 
-// Largest slice count of the differentiable tracker (its induced-voltage
-// kernels hold the profile, the wake and the adjoints in one workgroup).
+// Largest slice count of the differentiable tracker (each workgroup of its
+// induced-voltage kernels holds the profile, the wake and the adjoints).
 constexpr int kAdjMaxSlices = 1024;
 constexpr int kAdjCopies = 16;  // copies of the global deposit / adjoint accumulators
 
@@ -33,11 +33,11 @@ struct Adjoint {
     double *d_slots = nullptr; // [every][2][n]
     double *d_lam = nullptr;   // [2][n]: adjoints of dt, dE
     // slice axis
-    unsigned long long *d_dep = nullptr;  // [copies][n_slices] fixed-point deposit
+    unsigned long long *d_dep = nullptr;  // [2 parity][copies][n_slices] fixed-point deposit
     double *d_prof = nullptr;  // [n_turns + 1][n_slices] profiles p_t
     double *d_vt = nullptr;    // [n_turns + 1][n_slices] induced voltage V_t
     double *d_g = nullptr;     // wake g = irfft(Z) [n_fft]
-    double *d_lv = nullptr;    // [copies][n_slices] adjoint of V_t (accumulated)
+    double *d_lv = nullptr;    // [2 parity][copies][n_slices] adjoint of V_t (accumulated)
     double *d_lp = nullptr;    // [n_slices] adjoint of the profile
     double *d_corr = nullptr;  // [n_fft] sum_t sum_i lambda_V,t[i] p_t[(i - m) mod n_fft]
     double *d_gpart = nullptr; // [n_turns][grid][2 n_rf] per-workgroup RF gradient sums

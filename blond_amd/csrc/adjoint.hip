@@ -53,7 +53,6 @@ int smooth_scale_bits(int64_t n);
 
 constexpr int kAdjMaxRf = 4;
 constexpr int kAdjBlock = 1024;
-constexpr int kAdjOneBlock = 1024;
 
 struct AdjArgs {
     int n_rf;
@@ -101,9 +100,10 @@ __device__ __forceinline__ void deposit_lds(unsigned long long *h, double x, con
     }
 }
 
-__device__ __forceinline__ void deposit_flush(unsigned long long *h, const AdjArgs &a) {
+// into deposit parity `par` (the profile p_t of turn t is parity t & 1)
+__device__ __forceinline__ void deposit_flush(unsigned long long *h, const AdjArgs &a, int par) {
     __syncthreads();
-    unsigned long long *out = a.dep + (size_t)(blockIdx.x % kAdjCopies) * a.ns;
+    unsigned long long *out = a.dep + ((size_t)par * kAdjCopies + blockIdx.x % kAdjCopies) * a.ns;
     for (int b = threadIdx.x; b < a.ns; b += blockDim.x)
         if (h[b]) atomic_add_agent(&out[b], h[b]);
 }
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(kAdjBlock) adj_deposit_kernel(const double *__
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         deposit_lds(hdep, x[i], a);
-    deposit_flush(hdep, a);
+    deposit_flush(hdep, a, 0);
 }
 
 // q sum_b V[b] w_b(x) and (with D) its u-derivative, V in LDS
@@ -170,38 +170,88 @@ __global__ void __launch_bounds__(kAdjBlock) adj_turn_kernel(const double *xi, c
         eo[i] = e;
         if (DEPOSIT) deposit_lds(hdep, x, a);
     }
-    if (DEPOSIT) deposit_flush(hdep, a);
+    if (DEPOSIT) deposit_flush(hdep, a, (t + 1) & 1);
 }
 
-// One workgroup: p_t from the deposit copies (cleared), stored; with
-// `voltage`, V_t = -scale * (g conv p_t) stored too.
-__global__ void __launch_bounds__(kAdjOneBlock) adj_profile_kernel(AdjArgs a, int t, bool voltage,
-                                                                   double *prof, const double *g,
-                                                                   int n_fft, double scale,
-                                                                   double *vt) {
+// Circular dot product over k = r, r + 4, r + 8, ... < k_end of
+// G[m(k)] X[k], m(k) = (s0 + SGN k) mod n_fft (s0 + SGN k within
+// (-n_fft, 2 n_fft)), G read as zero from g_len on; four accumulators so
+// the LDS reads of consecutive terms overlap.  Lane r of a quad takes
+// residue r: the quad reads consecutive words.
+template <int SGN>
+__device__ __forceinline__ double circ_dot(const double *G, int n_fft, int g_len, const double *X,
+                                           int r, int k_end, int s0) {
+    auto gat = [&](int k) -> double {
+        int m = s0 + SGN * k;
+        m += m < 0 ? n_fft : 0;
+        m -= m >= n_fft ? n_fft : 0;
+        const double g = G[m < g_len ? m : 0];
+        return m < g_len ? g : 0.0;
+    };
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    int k = r;
+    for (; k + 12 < k_end; k += 16) {
+        v0 = fma(gat(k), X[k], v0);
+        v1 = fma(gat(k + 4), X[k + 4], v1);
+        v2 = fma(gat(k + 8), X[k + 8], v2);
+        v3 = fma(gat(k + 12), X[k + 12], v3);
+    }
+    for (; k < k_end; k += 4) v0 = fma(gat(k), X[k], v0);
+    return (v0 + v1) + (v2 + v3);
+}
+
+// An output of a circular product is computed by a quad of lanes (each a
+// residue class of the k range); the quads stay together because every
+// trip count below is a multiple of 4.  The quad's sum, in every lane.
+constexpr int kQuad = 4;
+__device__ __forceinline__ double quad_sum(double v) {
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    return v;
+}
+
+// Outputs per workgroup of the slice-axis kernels (kAdjSliceBlock / kQuad).
+constexpr int kAdjSliceBlock = 256;
+constexpr int kAdjSliceOut = kAdjSliceBlock / kQuad;
+
+// p_t from the deposit copies of parity t & 1, stored; with `voltage`,
+// V_t = -scale * (g conv p_t) stored too.  Workgroup b owns outputs
+// [64 b, 64 b + 64): it stores them and clears the same bins of the other
+// parity's copies (read by the previous launch, written by the next turn).
+__global__ void __launch_bounds__(kAdjSliceBlock) adj_profile_kernel(AdjArgs a, int t, bool voltage,
+                                                                     double *prof, const double *g,
+                                                                     int n_fft, double scale,
+                                                                     double *vt) {
     extern __shared__ double sh[];
     double *p = sh, *gs = sh + a.ns;
+    const int ns = a.ns;
     const double inv = 1.0 / a.fix_scale;
-    for (int j = threadIdx.x; j < a.ns; j += blockDim.x) {
+    const unsigned long long *din = a.dep + (size_t)(t & 1) * kAdjCopies * ns;
+    unsigned long long *dclr = a.dep + (size_t)((t + 1) & 1) * kAdjCopies * ns;
+    const int o0 = blockIdx.x * kAdjSliceOut, o1 = min(ns, o0 + kAdjSliceOut);
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+        // every copy is loaded before the sum: one memory round trip
+        unsigned long long v[kAdjCopies];
+#pragma unroll
+        for (int c = 0; c < kAdjCopies; ++c) v[c] = din[(size_t)c * ns + j];
         unsigned long long s = 0;
-        for (int c = 0; c < kAdjCopies; ++c) {
-            s += a.dep[(size_t)c * a.ns + j];
-            a.dep[(size_t)c * a.ns + j] = 0ull;
-        }
+#pragma unroll
+        for (int c = 0; c < kAdjCopies; ++c) s += v[c];
         p[j] = (double)(long long)s * inv;
-        prof[(size_t)t * a.ns + j] = p[j];
+        if (j >= o0 && j < o1) {
+            prof[(size_t)t * ns + j] = p[j];
+#pragma unroll
+            for (int c = 0; c < kAdjCopies; ++c) dclr[(size_t)c * ns + j] = 0ull;
+        }
     }
     if (!voltage) return;
     for (int m = threadIdx.x; m < n_fft; m += blockDim.x) gs[m] = g[m];
     __syncthreads();
-    for (int i = threadIdx.x; i < a.ns; i += blockDim.x) {
-        double v = 0.0;
-        for (int j = 0; j < a.ns; ++j) {
-            int m = i - j;
-            if (m < 0) m += n_fft;
-            v = fma(gs[m], p[j], v);
-        }
-        vt[(size_t)t * a.ns + i] = -(scale * v);
+    // V_t[i] = -scale sum_j g[(i - j) mod n_fft] p[j]
+    for (int q = threadIdx.x; q < (o1 - o0) * kQuad; q += blockDim.x) {
+        const int i = o0 + q / kQuad, r = q % kQuad;
+        const double v = quad_sum(circ_dot<-1>(gs, n_fft, n_fft, p, r, ns, i));
+        if (r == 0) vt[(size_t)t * ns + i] = -(scale * v);
     }
 }
 
@@ -235,12 +285,17 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
         double wk[3], dkick;
         int k;
         const double e1 = e + induced_kick(vl, x, a, &dkick, wk, k);
+        // constant trip counts with a guard keep sn, cs and gsum in registers
         double sn[kAdjMaxRf], cs[kAdjMaxRf];
         double s = 0.0, ds = 0.0;
-        for (int j = 0; j < a.n_rf && j < kAdjMaxRf; ++j) {
-            cephes_sincos(w[j] * x + phi[j], sn[j], cs[j]);
-            s = s + V[j] * sn[j];
-            ds = ds + V[j] * w[j] * cs[j];
+#pragma unroll
+        for (int j = 0; j < kAdjMaxRf; ++j) {
+            sn[j] = cs[j] = 0.0;
+            if (j < a.n_rf) {
+                cephes_sincos(w[j] * x + phi[j], sn[j], cs[j]);
+                s = s + V[j] * sn[j];
+                ds = ds + V[j] * w[j] * cs[j];
+            }
         }
         const double e2 = e1 + (a.charge * s + acc);
         const double x1 = x + coeff * e2;
@@ -259,9 +314,12 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
         }
         const double le2 = lam_e[i] + coeff * lx;
         lx = lx + le2 * (a.charge * ds);
-        for (int j = 0; j < a.n_rf && j < kAdjMaxRf; ++j) {
-            gsum[2 * j] += le2 * a.charge * sn[j];
-            gsum[2 * j + 1] += le2 * a.charge * V[j] * cs[j];
+#pragma unroll
+        for (int j = 0; j < kAdjMaxRf; ++j) {
+            if (j < a.n_rf) {
+                gsum[2 * j] += le2 * a.charge * sn[j];
+                gsum[2 * j + 1] += le2 * a.charge * V[j] * cs[j];
+            }
         }
         lx = lx + le2 * dkick;
         if (k != kNoBin) {
@@ -276,10 +334,13 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
     }
     // per-workgroup RF gradient sums (fixed order: wavefront tree, then waves)
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int j = 0; j < 2 * a.n_rf && j < 2 * kAdjMaxRf; ++j) {
-        double v = gsum[j];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        if (lane == 0) red[wid][j] = v;
+#pragma unroll
+    for (int j = 0; j < 2 * kAdjMaxRf; ++j) {
+        if (j < 2 * a.n_rf) {
+            double v = gsum[j];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (lane == 0) red[wid][j] = v;
+        }
     }
     __syncthreads();
     if ((int)threadIdx.x < 2 * a.n_rf) {
@@ -287,48 +348,71 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
         for (int q = 0; q < kAdjBlock / 64; ++q) v += red[q][threadIdx.x];
         a.gpart[((size_t)t * gridDim.x + blockIdx.x) * 2 * a.n_rf + threadIdx.x] = v;
     }
-    double *out = a.lv + (size_t)(blockIdx.x % kAdjCopies) * a.ns;
+    double *out = a.lv + ((size_t)(t & 1) * kAdjCopies + blockIdx.x % kAdjCopies) * a.ns;
     for (int b = threadIdx.x; b < a.ns; b += blockDim.x)
         if (lvs[b] != 0.0) atomic_add_agent(&out[b], lvs[b]);
 }
 
-// One workgroup: lV_t (copies summed, cleared) -> lp_t = -scale (g corr lV_t),
-// corr[m] += sum_i lV_t[i] p_t[(i - m) mod n_fft].
-__global__ void __launch_bounds__(kAdjOneBlock) adj_iv_kernel(AdjArgs a, int t, const double *prof,
-                                                              const double *g, int n_fft,
-                                                              double scale, double *lp_out,
-                                                              double *corr) {
+// lV_t (copies of parity t & 1 summed) -> lp_t = -scale (g corr lV_t) and
+// corr[m] += sum_i lV_t[i] p_t[(i - m) mod n_fft].  The outputs (ns values
+// of lp_t, then the corr entries that have a term: m < ns and
+// m > n_fft - ns) are split 64 per workgroup; each workgroup also clears
+// its share of the other parity's copies (read by the previous launch,
+// written by the next reverse turn).
+__global__ void __launch_bounds__(kAdjSliceBlock) adj_iv_kernel(AdjArgs a, int t, const double *prof,
+                                                                const double *g, int n_fft,
+                                                                double scale, double *lp_out,
+                                                                double *corr) {
     extern __shared__ double sh[];
-    double *lvs = sh, *p = sh + a.ns, *gs = sh + 2 * a.ns;
-    for (int i = threadIdx.x; i < a.ns; i += blockDim.x) {
+    const int ns = a.ns;
+    double *lvs = sh, *p = sh + ns, *gs = sh + 2 * ns;
+    const double *lin = a.lv + (size_t)(t & 1) * kAdjCopies * ns;
+    double *lclr = a.lv + (size_t)((t + 1) & 1) * kAdjCopies * ns;
+    const int per = (ns + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int c0 = blockIdx.x * per, c1 = min(ns, c0 + per);
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+        // every copy is loaded before the sum: one memory round trip
+        double v[kAdjCopies];
+#pragma unroll
+        for (int c = 0; c < kAdjCopies; ++c) v[c] = lin[(size_t)c * ns + i];
         double s = 0.0;
-        for (int c = 0; c < kAdjCopies; ++c) {
-            s += a.lv[(size_t)c * a.ns + i];
-            a.lv[(size_t)c * a.ns + i] = 0.0;
-        }
+#pragma unroll
+        for (int c = 0; c < kAdjCopies; ++c) s += v[c];
         lvs[i] = s;
-        p[i] = prof[(size_t)t * a.ns + i];
+        p[i] = prof[(size_t)t * ns + i];
+        if (i >= c0 && i < c1)
+#pragma unroll
+            for (int c = 0; c < kAdjCopies; ++c) lclr[(size_t)c * ns + i] = 0.0;
     }
     for (int m = threadIdx.x; m < n_fft; m += blockDim.x) gs[m] = g[m];
     __syncthreads();
-    for (int j = threadIdx.x; j < a.ns; j += blockDim.x) {
-        double v = 0.0;
-        for (int i = 0; i < a.ns; ++i) {
-            int m = i - j;
-            if (m < 0) m += n_fft;
-            v = fma(gs[m], lvs[i], v);
+    const int hi0 = max(ns, n_fft - ns + 1);
+    const int n_out = ns + ns + (n_fft - hi0);
+    const int o0 = blockIdx.x * kAdjSliceOut, o1 = min(n_out, o0 + kAdjSliceOut);
+    for (int q = threadIdx.x; q < (o1 - o0) * kQuad; q += blockDim.x) {
+        const int o = o0 + q / kQuad, r = q % kQuad;
+        int m = 0;
+        double v;
+        if (o < ns) {
+            v = circ_dot<1>(gs, n_fft, n_fft, lvs, r, ns, -o);  // lp_t[o]
+        } else {
+            m = o - ns < ns ? o - ns : hi0 + (o - 2 * ns);
+            v = circ_dot<1>(p, n_fft, ns, lvs, r, ns, -m);  // corr[m]
         }
-        lp_out[j] = -(scale * v);
-    }
-    for (int m = threadIdx.x; m < n_fft; m += blockDim.x) {
-        double v = 0.0;
-        for (int i = 0; i < a.ns; ++i) {
-            int j = i - m;
-            if (j < 0) j += n_fft;
-            if (j < a.ns) v = fma(lvs[i], p[j], v);
+        v = quad_sum(v);
+        if (r == 0) {
+            if (o < ns) lp_out[o] = -(scale * v);
+            else corr[m] += v;
         }
-        corr[m] += v;
     }
+}
+
+// Workgroups of the slice-axis kernels: 64 outputs each.
+static int adj_profile_grid(int ns) { return (ns + kAdjSliceOut - 1) / kAdjSliceOut; }
+static int adj_iv_grid(int ns, int64_t n_fft) {
+    const int64_t hi0 = ns > n_fft - ns + 1 ? ns : n_fft - ns + 1;
+    const int64_t n_out = 2 * (int64_t)ns + (n_fft - hi0);
+    return (int)((n_out + kAdjSliceOut - 1) / kAdjSliceOut);
 }
 
 // ---------------------------------------------------------------------------
@@ -396,13 +480,13 @@ int Adjoint::init(int64_t n_, int n_rf_, int n_turns_, int n_slices_, int every_
     BLMI_HIP_TRY(hipMalloc(&d_ckpt, sizeof(double) * pn * n_ckpt));
     BLMI_HIP_TRY(hipMalloc(&d_slots, sizeof(double) * pn * every));
     BLMI_HIP_TRY(hipMalloc(&d_lam, sizeof(double) * pn));
-    BLMI_HIP_TRY(hipMalloc(&d_dep, sizeof(unsigned long long) * kAdjCopies * n_slices));
-    BLMI_HIP_TRY(hipMemset(d_dep, 0, sizeof(unsigned long long) * kAdjCopies * n_slices));
+    BLMI_HIP_TRY(hipMalloc(&d_dep, sizeof(unsigned long long) * 2 * kAdjCopies * n_slices));
+    BLMI_HIP_TRY(hipMemset(d_dep, 0, sizeof(unsigned long long) * 2 * kAdjCopies * n_slices));
     BLMI_HIP_TRY(hipMalloc(&d_prof, sizeof(double) * nt * n_slices));
     BLMI_HIP_TRY(hipMalloc(&d_vt, sizeof(double) * nt * n_slices));
     BLMI_HIP_TRY(hipMemset(d_vt, 0, sizeof(double) * nt * n_slices));
-    BLMI_HIP_TRY(hipMalloc(&d_lv, sizeof(double) * kAdjCopies * n_slices));
-    BLMI_HIP_TRY(hipMemset(d_lv, 0, sizeof(double) * kAdjCopies * n_slices));
+    BLMI_HIP_TRY(hipMalloc(&d_lv, sizeof(double) * 2 * kAdjCopies * n_slices));
+    BLMI_HIP_TRY(hipMemset(d_lv, 0, sizeof(double) * 2 * kAdjCopies * n_slices));
     BLMI_HIP_TRY(hipMalloc(&d_lp, sizeof(double) * n_slices));
     BLMI_HIP_TRY(hipMalloc(&d_gpart, sizeof(double) * (size_t)n_turns * grid * 2 * n_rf));
     fixed_bits = smooth_scale_bits(n);
@@ -462,6 +546,13 @@ int Adjoint::set_wake(int64_t n_fft_, const double *g_host, double scale_) {
     n_fft = n_fft_;
     scale = scale_;
     BLMI_HIP_TRY(hipMemcpy(d_g, g_host, sizeof(double) * n_fft, hipMemcpyHostToDevice));
+    // slice-axis kernels: profile + wake (+ adjoint) in LDS, up to 80 KiB
+    const int lds_p = (int)(sizeof(double) * ((size_t)n_slices + n_fft));
+    const int lds_i = (int)(sizeof(double) * (2 * (size_t)n_slices + n_fft));
+    BLMI_HIP_TRY(hipFuncSetAttribute((const void *)adj_profile_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds_p));
+    BLMI_HIP_TRY(hipFuncSetAttribute((const void *)adj_iv_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds_i));
     forward_done = false;
     return BLMI_OK;
 }
@@ -490,17 +581,23 @@ int Adjoint::enqueue_forward(int turns, hipStream_t s) {
     const size_t pn = 2 * (size_t)n;
     const size_t lds_dep = 2 * sizeof(double) * n_slices;  // deposit + V_t
     const size_t lds_one = sizeof(double) * ((size_t)n_slices + n_fft);
+    const int pgrid = adj_profile_grid(n_slices);
     double *x = d_work, *e = d_work + n;
+    // both deposit parities start cleared (a previous pass leaves its last
+    // profile's copies behind)
+    int rc = launch_fill<double>(reinterpret_cast<double *>(d_dep),
+                                 2 * (int64_t)kAdjCopies * n_slices, 0.0, s);
+    if (rc) return rc;
     adj_copy(d_work, d_x0, (int64_t)pn, grid, s);
     hipLaunchKernelGGL(adj_deposit_kernel, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, n, a);
-    hipLaunchKernelGGL(adj_profile_kernel, dim3(1), dim3(kAdjOneBlock), lds_one, s, a, 0, true,
-                       d_prof, d_g, (int)n_fft, scale, d_vt);
+    hipLaunchKernelGGL(adj_profile_kernel, dim3(pgrid), dim3(kAdjSliceBlock), lds_one, s, a, 0,
+                       true, d_prof, d_g, (int)n_fft, scale, d_vt);
     for (int t = 0; t < turns; ++t) {
         if (t % every == 0) adj_copy(d_ckpt + (size_t)(t / every) * pn, d_work, (int64_t)pn, grid, s);
         hipLaunchKernelGGL(adj_turn_kernel<true>, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, e, x,
                            e, n, t, a);
-        hipLaunchKernelGGL(adj_profile_kernel, dim3(1), dim3(kAdjOneBlock), lds_one, s, a, t + 1,
-                           t + 1 < turns, d_prof, d_g, (int)n_fft, scale, d_vt);
+        hipLaunchKernelGGL(adj_profile_kernel, dim3(pgrid), dim3(kAdjSliceBlock), lds_one, s, a,
+                           t + 1, t + 1 < turns, d_prof, d_g, (int)n_fft, scale, d_vt);
     }
     return blmi_check_launch("adjoint_forward");
 }
@@ -511,8 +608,10 @@ int Adjoint::enqueue_backward(int T, hipStream_t s) {
     const size_t lds_rev = sizeof(double) * 3 * (size_t)n_slices;
     const size_t lds_one = sizeof(double) * (2 * (size_t)n_slices + n_fft);
     AdjArgs a = adj_args(*this);
+    const int igrid = adj_iv_grid(n_slices, n_fft);
     int rc = launch_fill<double>(d_lam, (int64_t)pn, 0.0, s);
     if (!rc) rc = launch_fill<double>(d_corr, n_fft, 0.0, s);
+    if (!rc) rc = launch_fill<double>(d_lv, 2 * (int64_t)kAdjCopies * n_slices, 0.0, s);
     if (rc) return rc;
     for (int seg = (T - 1) / every; seg >= 0; --seg) {
         const int t0 = seg * every, t1 = (t0 + every < T) ? t0 + every : T;
@@ -527,8 +626,8 @@ int Adjoint::enqueue_backward(int T, hipStream_t s) {
             const double *st = d_slots + (size_t)(t - t0) * pn;
             hipLaunchKernelGGL(adj_reverse_kernel, dim3(grid), dim3(kAdjBlock), lds_rev, s, st,
                                st + n, d_lam, d_lam + n, n, t, a);
-            hipLaunchKernelGGL(adj_iv_kernel, dim3(1), dim3(kAdjOneBlock), lds_one, s, a, t, d_prof,
-                               d_g, (int)n_fft, scale, d_lp, d_corr);
+            hipLaunchKernelGGL(adj_iv_kernel, dim3(igrid), dim3(kAdjSliceBlock), lds_one, s, a, t,
+                               d_prof, d_g, (int)n_fft, scale, d_lp, d_corr);
         }
     }
     return blmi_check_launch("adjoint_backward");
