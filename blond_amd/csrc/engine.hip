@@ -231,15 +231,19 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
     }
     auto step = [&](T &x, T &e) -> int {
         if (INTERP) {
-            // linear_interp_kick.cpp:44-64 (acceleration kick 0)
+            // linear_interp_kick.cpp:44-64 (acceleration kick 0).  The table
+            // entry is loaded unconditionally (bin 0 stands in outside the
+            // table) before the RF kick is evaluated, so the gather's latency
+            // overlaps the sines; the sums keep the order e + interp + rf.
             const T f = floor((x - bc0) * ibw);
-            if (f >= T(0) && f < (T)a.n_tab) {
-                const int k = (int)f;
-                const double2 tk = INTERP == 2 ? ltab[k] : a.tab[k];
-                e = e + (x * (T)tk.x + (T)tk.y);
-            }
+            const bool in = f >= T(0) && f < (T)a.n_tab;
+            const double2 tk = INTERP == 2 ? ltab[in ? (int)f : 0] : a.tab[in ? (int)f : 0];
+            const T rf = e_rf_kick(x, a.n_rf, a.charge, V, w, phi, acc);
+            if (in) e = e + (x * (T)tk.x + (T)tk.y);
+            e = e + rf;
+        } else {
+            e = e + e_rf_kick(x, a.n_rf, a.charge, V, w, phi, acc);
         }
-        e = e + e_rf_kick(x, a.n_rf, a.charge, V, w, phi, acc);
         x = x + e_drift(dc, e, SOLVER);
         if (HIST) {
             const T f = floor((x - a.cut_left) * a.inv_bw);
