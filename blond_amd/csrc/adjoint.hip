@@ -588,6 +588,24 @@ int Adjoint::enqueue_forward(int turns, hipStream_t s) {
     int rc = launch_fill<double>(reinterpret_cast<double *>(d_dep),
                                  2 * (int64_t)kAdjCopies * n_slices, 0.0, s);
     if (rc) return rc;
+    if (every == 1) {
+        // every state is kept (n_turns x 16 B per macroparticle of HBM): turn t
+        // reads checkpoint t and writes checkpoint t + 1 (the last state goes
+        // to the work array), and the reverse pass recomputes nothing
+        adj_copy(d_ckpt, d_x0, (int64_t)pn, grid, s);
+        hipLaunchKernelGGL(adj_deposit_kernel, dim3(grid), dim3(kAdjBlock), lds_dep, s, d_ckpt, n, a);
+        hipLaunchKernelGGL(adj_profile_kernel, dim3(pgrid), dim3(kAdjSliceBlock), lds_one, s, a, 0,
+                           true, d_prof, d_g, (int)n_fft, scale, d_vt);
+        for (int t = 0; t < turns; ++t) {
+            const double *si = d_ckpt + (size_t)t * pn;
+            double *so = t + 1 < turns ? d_ckpt + (size_t)(t + 1) * pn : d_work;
+            hipLaunchKernelGGL(adj_turn_kernel<true>, dim3(grid), dim3(kAdjBlock), lds_dep, s, si,
+                               si + n, so, so + n, n, t, a);
+            hipLaunchKernelGGL(adj_profile_kernel, dim3(pgrid), dim3(kAdjSliceBlock), lds_one, s, a,
+                               t + 1, t + 1 < turns, d_prof, d_g, (int)n_fft, scale, d_vt);
+        }
+        return blmi_check_launch("adjoint_forward");
+    }
     adj_copy(d_work, d_x0, (int64_t)pn, grid, s);
     hipLaunchKernelGGL(adj_deposit_kernel, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, n, a);
     hipLaunchKernelGGL(adj_profile_kernel, dim3(pgrid), dim3(kAdjSliceBlock), lds_one, s, a, 0,
@@ -613,6 +631,16 @@ int Adjoint::enqueue_backward(int T, hipStream_t s) {
     if (!rc) rc = launch_fill<double>(d_corr, n_fft, 0.0, s);
     if (!rc) rc = launch_fill<double>(d_lv, 2 * (int64_t)kAdjCopies * n_slices, 0.0, s);
     if (rc) return rc;
+    if (every == 1) {
+        for (int t = T - 1; t >= 0; --t) {
+            const double *st = d_ckpt + (size_t)t * pn;
+            hipLaunchKernelGGL(adj_reverse_kernel, dim3(grid), dim3(kAdjBlock), lds_rev, s, st,
+                               st + n, d_lam, d_lam + n, n, t, a);
+            hipLaunchKernelGGL(adj_iv_kernel, dim3(igrid), dim3(kAdjSliceBlock), lds_one, s, a, t,
+                               d_prof, d_g, (int)n_fft, scale, d_lp, d_corr);
+        }
+        return blmi_check_launch("adjoint_backward");
+    }
     for (int seg = (T - 1) / every; seg >= 0; --seg) {
         const int t0 = seg * every, t1 = (t0 + every < T) ? t0 + every : T;
         // states t0 .. t1-1 into slots 0 .. t1-t0-1

@@ -142,3 +142,26 @@ def test_backward_after_another_forward_recomputes():
     for x, y in zip(g1, b):
         assert torch.allclose(x, y.grad, rtol=1e-10, atol=0)
     dtr.close()
+
+
+@pytest.mark.parametrize("turns", [1, 23])
+def test_every_state_kept_equals_checkpoint_segments(turns):
+    """checkpoint_every=1 keeps every state and recomputes nothing; the
+    profile is the same bits as with segments of 7 turns recomputed from
+    their checkpoints, and the gradients agree to 1e-13 relative (the
+    recompute launch, compiled without the deposit, may round an operation
+    of a state differently: one gradient of 23 turns differs by 1 ulp)."""
+    out = []
+    for every in (1, 7):
+        c, dtr = setup(6001, 30, 48, every)
+        th, ts = c.theta0, c.theta_star
+        with torch.no_grad():
+            target = dtr.profile_after(*params(ts), turns=turns).clone()
+        ps = params(th)
+        p = dtr.profile_after(*ps, turns=turns)
+        torch.sum((p - target) ** 2).backward()
+        out.append((p.detach().cpu().numpy(), [t.grad.cpu().numpy() for t in ps]))
+        dtr.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        np.testing.assert_allclose(a, b, rtol=1e-13, atol=0)
