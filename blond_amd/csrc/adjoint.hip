@@ -146,6 +146,15 @@ __global__ void __launch_bounds__(kAdjBlock) adj_turn_kernel(const double *xi, c
                                                              int t, AdjArgs a) {
     extern __shared__ unsigned long long hdep[];
     double *vl = reinterpret_cast<double *>(hdep + a.ns);  // V_t
+    // software pipeline: a thread's next particle is loaded while the
+    // current one is computed (the first before the LDS set-up)
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t i_first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double xn = 0.0, en = 0.0;
+    if (i_first < n) {
+        xn = xi[i_first];
+        en = ei[i_first];
+    }
     for (int b = threadIdx.x; b < a.ns; b += blockDim.x) {
         if (DEPOSIT) hdep[b] = 0ull;
         vl[b] = a.vt[(size_t)t * a.ns + b];
@@ -156,9 +165,12 @@ __global__ void __launch_bounds__(kAdjBlock) adj_turn_kernel(const double *xi, c
     const double *phi = w + a.n_rf;
     const double acc = a.acc[t];
     const double coeff = a.drift[t + 1].coeff;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        double x = xi[i], e = ei[i];
+    for (int64_t i = i_first; i < n; i += stride) {
+        double x = xn, e = en;
+        if (i + stride < n) {
+            xn = xi[i + stride];
+            en = ei[i + stride];
+        }
         double wk[3];
         int k;
         e = e + induced_kick(vl, x, a, nullptr, wk, k);
@@ -264,6 +276,16 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
     extern __shared__ double shr[];
     double *lp = shr, *lvs = shr + a.ns, *vl = shr + 2 * a.ns;
     __shared__ double red[kAdjBlock / 64][2 * kAdjMaxRf];
+    // software pipeline as in adj_turn_kernel
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t i_first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double xn = 0.0, en = 0.0, lxn = 0.0, len = 0.0;
+    if (i_first < n) {
+        xn = xs[i_first];
+        en = es[i_first];
+        lxn = lam_x[i_first];
+        len = lam_e[i_first];
+    }
     for (int b = threadIdx.x; b < a.ns; b += blockDim.x) {
         lp[b] = a.lp[b];
         lvs[b] = 0.0;
@@ -278,9 +300,14 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
     double gsum[2 * kAdjMaxRf];
 #pragma unroll
     for (int j = 0; j < 2 * kAdjMaxRf; ++j) gsum[j] = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const double x = xs[i], e = es[i];
+    for (int64_t i = i_first; i < n; i += stride) {
+        const double x = xn, e = en, lam_x_i = lxn, lam_e_i = len;
+        if (i + stride < n) {
+            xn = xs[i + stride];
+            en = es[i + stride];
+            lxn = lam_x[i + stride];
+            len = lam_e[i + stride];
+        }
         // forward step t, as adj_turn_kernel
         double wk[3], dkick;
         int k;
@@ -302,7 +329,7 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
         // deposit adjoint of p_{t+1} at x1
         double sw[3], sdw[3];
         const int kk = spline(x1, a, sw, sdw);
-        double lx = lam_x[i];
+        double lx = lam_x_i;
         if (kk != kNoBin) {
             double s3 = 0.0;
 #pragma unroll
@@ -312,7 +339,7 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
             }
             lx = lx + s3 * a.inv_bw_cut;
         }
-        const double le2 = lam_e[i] + coeff * lx;
+        const double le2 = lam_e_i + coeff * lx;
         lx = lx + le2 * (a.charge * ds);
 #pragma unroll
         for (int j = 0; j < kAdjMaxRf; ++j) {
