@@ -941,6 +941,18 @@ int Engine::launch_induced(hipStream_t s, int off, const KernelEvents &ev) {
     if (have_conv) {
         InducedArgs ia = induced_args(off);
         KernelEvents e1 = ev;
+        if (w && method == 1) {
+            // the direct convolution runs the wake step in its own epilogue
+            ia.n_out = (int)(n_mem > n_slices ? n_mem : n_slices);
+            ia.with_wake = 1;
+            ia.wk = wake_args(off);
+            if (fb_in_induced()) {
+                ia.with_fb = 1;
+                ia.fb = feedback_args(off);
+                ia.fb.from_counts = 1;
+            }
+            return launch_induced_direct(ia, s, e1);
+        }
         if (w) {
             // the convolution of this turn goes to d_v1; the wake step
             // forms the caller's voltage and the table

@@ -6,6 +6,7 @@
 
 #include "blmi_common.h"
 #include "feedback.h"
+#include "wake.h"
 
 namespace blmi {
 
@@ -68,6 +69,11 @@ struct InducedArgs {
     // (feedback.h) beside the convolution
     int with_fb;
     FeedbackArgs fb;
+    // direct method: the wake step of the turn in the same launch (the
+    // multi-turn memory shift, the inductive term and the table of the
+    // total, from this launch's V); volt and tab above are then the total's
+    int with_wake;
+    WakeArgs wk;
 };
 // Single workgroup: LDS Stockham rfft, impedance product, irfft.
 int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream_t s,

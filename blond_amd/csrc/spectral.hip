@@ -398,6 +398,64 @@ __global__ void induced_lds_batch_kernel(const InducedArgs *__restrict__ args, i
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int conv_pad(int k) { return k + (k >> 3); }
 
+// The wake step of the turn on this workgroup's outputs, as the engine's
+// separate wake step computes it (wake.hip): the memory after the turn
+// mem(i) = interp(old memory one revolution later) + V(i) for i < n_add
+// (impedance.py:491-512), written for i < n_mem; the total induced voltage
+// 0 + mem(i) (or 0 + V(i) without a memory) - coef * d(profile)/dt (i)
+// (impedance.py:117-127, 943-1000) and its linear_interp_kick table
+// (linear_interp_kick.cpp:33-48).  q holds the reversed profile in LDS.
+template <int C>
+__device__ __forceinline__ void direct_wake_epilogue(const InducedArgs &a, const double *q,
+                                                     const double *vout, double *memv,
+                                                     double mprev, int t, int i0,
+                                                     unsigned *cclr) {
+    const WakeArgs &w = a.wk;
+    const int ns = a.n_slices;
+    if ((int)threadIdx.x <= C) {
+        const int64_t i = (int64_t)i0 + threadIdx.x;
+        double v = vout[threadIdx.x];
+        if (w.n_mem > 0) {
+            v = i < w.n_add ? mprev + vout[threadIdx.x] : mprev;
+            if ((int)threadIdx.x < C && i < w.n_mem) ((t & 1) ? w.mem0 : w.mem1)[i] = v;
+        }
+        memv[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x >= C) return;
+    const int i = i0 + threadIdx.x;
+    if (i >= ns) return;
+    const double coef = w.ind_coef ? w.ind_coef[t] : 0.0;
+    auto prof = [&](int k) -> double { return q[conv_pad(ns - 1 - k)]; };
+    auto total = [&](int c, int j) -> double {
+        double v = 0.0;
+        v = v + memv[c];
+        if (w.ind_coef) {
+            const double d = profile_derivative_at(prof, w.bc, ns, w.bc[1] - w.bc[0],
+                                                   w.deriv_mode, j);
+            v = v + (-(coef * d));
+        }
+        return v;
+    };
+    const double vi = total(threadIdx.x, i);
+    a.volt[i] = vi;
+    if (a.profile_out) a.profile_out[i] = prof(i);
+    if (cclr)
+        for (int g = 0; g < a.groups; ++g) cclr[(size_t)g * ns + i] = 0u;
+    if (a.tab) {
+        const double inv_bw = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
+        if (i == 0) {
+            a.tab_params[0] = a.bc[0];
+            a.tab_params[1] = inv_bw;
+        }
+        if (i < ns - 1) {
+            const double vn = total(threadIdx.x + 1, i + 1);
+            const double k = a.charge * (vn - vi) * inv_bw;
+            a.tab[i] = make_double2(k, (a.charge * vi - a.bc[i] * k) + 0.0);
+        }
+    }
+}
+
 template <int C, int TH, int CHUNK>
 __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     static_assert(C >= 2 && C <= 32 && (C & (C - 1)) == 0, "C is a power of two up to 32");
@@ -410,9 +468,22 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     extern __shared__ double conv_lds[];
     __shared__ double red[TH / 64][C + 1];
     __shared__ double vout[C + 1];
+    __shared__ double memv[C + 1];
     constexpr int kSpan = CHUNK * TH;
     const int ns = a.n_slices;
     const int i0 = blockIdx.x * C;
+    // wake step: the previous memory's value one revolution later at this
+    // workgroup's outputs (impedance.py:536-550), read first so that its
+    // dependent loads overlap the staging and the products below
+    double mprev = 0.0;
+    int wt = 0;
+    if (a.with_wake) {
+        wt = *a.wk.turn_base + a.wk.turn_off;
+        const int64_t i = (int64_t)i0 + threadIdx.x;
+        if ((int)threadIdx.x <= C && i < a.wk.n_mem)
+            mprev = mtw_value((wt & 1) ? a.wk.mem1 : a.wk.mem0, a.wk.time, a.wk.n_mem,
+                              a.wk.t_rev[wt], i);
+    }
     const int lim = a.n_out + ns - 1;  // gl entries used (host checks gl_len)
     const int nq = (ns + kSpan - 1) / kSpan * kSpan;
     double *q = conv_lds;                  // q[pad(m)], zero for m >= ns
@@ -522,6 +593,10 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
         vout[threadIdx.x] = v;
     }
     __syncthreads();
+    if (a.with_wake) {
+        direct_wake_epilogue<C>(a, q, vout, memv, mprev, wt, i0, cclr);
+        return;
+    }
     if (threadIdx.x < C) {
         const int i = i0 + threadIdx.x;
         if (i < a.n_out) a.volt[i] = vout[threadIdx.x];
@@ -577,7 +652,9 @@ static int launch_direct_shape(const InducedArgs &a, hipStream_t s, const Kernel
 // 8): 21.8 against 22.5 us for (16, 512, 8) with its wake step.
 int launch_induced_direct(const InducedArgs &a, hipStream_t s, const KernelEvents &ev) {
     if (!a.gl || a.n_slices < 1 || a.n_out < a.n_slices ||
-        (int64_t)a.n_out + a.n_slices - 1 > a.gl_len || (a.tab && a.n_out != a.n_slices))
+        (int64_t)a.n_out + a.n_slices - 1 > a.gl_len ||
+        (a.tab && a.n_out != a.n_slices && !a.with_wake) ||
+        (a.with_wake && (a.wk.n_mem > a.n_out || (a.wk.ind_coef && a.n_slices < 2))))
         return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: bad arguments");
     if (a.n_out > a.n_slices && a.n_slices > 1024) return launch_direct_shape<32, 512, 8>(a, s, ev);
     if (a.n_slices <= 1024) return launch_direct_shape<16, 256, 8>(a, s, ev);

@@ -109,6 +109,24 @@ __device__ __forceinline__ double profile_derivative_at(P p, const double *__res
     return r;
 }
 
+// shift_trev_time (impedance.py:536-550): the memory seen one revolution
+// later, interp(time + t_rev, time, memory, left=0, right=0), then the
+// current turn's voltage added to its first n_add points (impedance.py:510-512).
+__device__ __forceinline__ double mtw_value(const double *__restrict__ mem,
+                                            const double *__restrict__ time, int64_t n_mem,
+                                            double t_rev, int64_t i) {
+    const double x = time[i] + t_rev;
+    int64_t g = -1;
+    if (n_mem > 1) {
+        const double step = time[1] - time[0];
+        if (step > 0.0) {
+            const double sh = floor(t_rev / step);
+            if (sh >= 0.0 && sh < (double)n_mem) g = i + (int64_t)sh;
+        }
+    }
+    return np_interp_at(x, time, mem, n_mem, 0.0, 0.0, g);
+}
+
 int launch_interp(const double *x, int64_t n, const double *xp, const double *fp, int64_t m,
                   double left, double right, double *out, hipStream_t s);
 int launch_gradient(const double *f, int64_t n, double h, double *out, hipStream_t s);
