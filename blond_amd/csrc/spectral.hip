@@ -460,7 +460,7 @@ template <int C, int TH, int CHUNK>
 __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     static_assert(C >= 2 && C <= 32 && (C & (C - 1)) == 0, "C is a power of two up to 32");
     static_assert(CHUNK % 8 == 0, "a thread's samples start on a pad period");
-    if (a.with_fb && blockIdx.x == gridDim.x - 1) {
+    if (a.with_fb && blockIdx.x == 0) {
         // the beam phase loop of this turn, independent of the convolution
         feedback_turn_body(a.fb);
         return;
@@ -471,7 +471,9 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     __shared__ double memv[C + 1];
     constexpr int kSpan = CHUNK * TH;
     const int ns = a.n_slices;
-    const int i0 = blockIdx.x * C;
+    // with the loop, workgroup 0 runs it (dispatched first: it is the
+    // longest) and workgroup b > 0 takes outputs of b - 1
+    const int i0 = (blockIdx.x - (a.with_fb ? 1 : 0)) * C;
     // wake step: the previous memory's value one revolution later at this
     // workgroup's outputs (impedance.py:536-550), read first so that its
     // dependent loads overlap the staging and the products below
@@ -609,7 +611,7 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     if (a.tab) {
         // linear_interp_kick.cpp:33-48 on V (acc = 0 for the induced kick)
         const double inv_bw = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (i0 == 0 && threadIdx.x == 0) {
             a.tab_params[0] = a.bc[0];
             a.tab_params[1] = inv_bw;
         }
