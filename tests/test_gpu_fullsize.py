@@ -1,7 +1,8 @@
 """Parity of the fused engine at the measured workloads' full sizes: c2
-(4e6 macroparticles, 4096 slices, n_fft 16384, direct convolution) and c3
+(4e6 macroparticles, 4096 slices, n_fft 16384, direct convolution), c3
 (1e7 macroparticles, two RF systems, 65536 slices, n_fft 131072, four-step
-FFT, u16-pair histograms over 473 workgroups).
+FFT, u16-pair histograms over 512 workgroups) and the c4 ensemble (256
+bunches, 5.5e7 macroparticles).
 
 From identical coordinates, one turn against the oracle's turn: the profile
 bit-identical, the induced voltage within 1e-10 of its maximum, dE within
@@ -48,3 +49,26 @@ def test_full_size_first_turn_and_histogram_of_own_coordinates(name):
     # every macroparticle whose bin (histogram.cpp:40-56) is in range, once
     f = np.floor((dt - st["cut_left"]) * (st["n_slices"] / (st["cut_right"] - st["cut_left"])))
     assert prof.sum() == np.count_nonzero((f >= 0) & (f < st["n_slices"]))
+
+
+def test_full_size_batched_ensemble_first_turn():
+    """c4 at its size: 256 bunches of 1e4..1e6 macroparticles (5.5e7 in
+    all), 1024 slices each, one launch pair per turn; every bunch's first
+    turn against the oracle's turn of that bunch alone."""
+    from blond_amd.workloads import build_ensemble
+
+    ens = build_ensemble(n_bunches=256, n_turns=10)
+    snaps = [ens.bunch(k).snapshot() for k in range(256)]
+    print(f"\nc4: {int(ens.counts.sum())} macroparticles in 256 bunches", flush=True)
+    et = ens.tracker()
+    et.track(1)
+    for k in range(256):
+        ref = orc.track(snaps[k], 1)
+        np.testing.assert_array_equal(host(ens.profiles[k].n_macroparticles), ref["profile"])
+        v = host(ens.tivs[k].induced_voltage)
+        vmax = np.abs(ref["induced_voltage"]).max()
+        assert np.max(np.abs(v - ref["induced_voltage"])) <= 1e-10 * vmax, f"bunch {k}"
+        sc = np.abs(ref["dE"]).max()
+        np.testing.assert_allclose(host(ens.trackers[k].beam.dE), ref["dE"], rtol=0,
+                                   atol=1e-12 * sc)
+    et.close()
