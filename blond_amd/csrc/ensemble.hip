@@ -108,9 +108,19 @@ __global__ void __launch_bounds__(1024) ensemble_turn_kernel(double *__restrict_
             }
             return -1;
         };
+        // software pipeline: a thread's next pair is loaded while the
+        // current one is computed
+        double2 xn{}, en{};
+        if (g.p0 + threadIdx.x < g.p1) {
+            xn = reinterpret_cast<const double2 *>(dt)[g.p0 + threadIdx.x];
+            en = reinterpret_cast<const double2 *>(dE)[g.p0 + threadIdx.x];
+        }
         for (int64_t p = g.p0 + threadIdx.x; p < g.p1; p += blockDim.x) {
-            double2 x = reinterpret_cast<const double2 *>(dt)[p];
-            double2 e = reinterpret_cast<const double2 *>(dE)[p];
+            double2 x = xn, e = en;
+            if (p + blockDim.x < g.p1) {
+                xn = reinterpret_cast<const double2 *>(dt)[p + blockDim.x];
+                en = reinterpret_cast<const double2 *>(dE)[p + blockDim.x];
+            }
             int b0 = step(x.x, e.x);
             int b1 = step(x.y, e.y);
             reinterpret_cast<double2 *>(dt)[p] = x;
