@@ -22,8 +22,9 @@ Prints one JSON line with the contract keys plus
                           turn for all bunches, autotuned, against 256
                           sequential fused runs of the same bunches
                   adjoint c5: 1e6 macroparticles x 500 turns, forward +
-                          reverse pass with every state kept in HBM,
-                          80 B per macroparticle-turn
+                          reverse pass with checkpoints every 50 turns and
+                          recomputation (112 B per macroparticle-turn); as
+                          a variant every state kept in HBM (80 B)
                   wake    c2 bunch (4e6, 4096 slices) with a multi-turn
                           wake over 4 revolutions (memory 16384 points,
                           n_fft 16384) plus an inductive Z/n source
@@ -217,14 +218,12 @@ def form_batched(torch, turns=100, seq_turns=20) -> dict:
             "speedup_vs_sequential": batched / sequential}
 
 
-def form_adjoint(torch, turns=500) -> dict:
-    """c5: forward + checkpointed adjoint, 1e6 macroparticles x 500 turns."""
+def _adjoint(torch, turns, every) -> dict:
     from blond_amd.trackers.differentiable import DifferentiableTracker
     from blond_amd.workloads import build_adjoint_case
 
     c = build_adjoint_case(n_particles=1_000_000, n_turns=turns)
-    # every state kept (500 x 16 MB of HBM): no recompute in the reverse pass
-    dtr = DifferentiableTracker(c.tracker, c.profile, checkpoint_every=1)
+    dtr = DifferentiableTracker(c.tracker, c.profile, checkpoint_every=every)
     ts = c.theta_star
     mk = lambda th, g: [torch.tensor(th[k], dtype=torch.float64, device="cuda", requires_grad=g)
                         for k in ("V", "phi", "R", "fr", "Q")]
@@ -238,13 +237,24 @@ def form_adjoint(torch, turns=500) -> dict:
     step()
     el = _timed(step, torch.cuda.synchronize)
     n = c.beam.n_macroparticles
-    bytes_ = 80 * n * turns
+    bpp = 80 if every == 1 else 112  # forward 32 + reverse 48 (+ recompute 32)
     dtr.close()
-    return {"workload": "c5: 1e6 macroparticles x 500 turns, 256 slices, 1 broadband resonator, "
-                        "every turn's state kept (checkpoint_every=1)",
+    how = ("every turn's state kept in HBM, no recompute" if every == 1 else
+           f"checkpoints every {every} turns with recomputation")
+    return {"workload": f"c5: 1e6 macroparticles x {turns} turns, 256 slices, 1 broadband "
+                        f"resonator, {how}",
             "value": n * turns / el, "unit": "macroparticle-turns/s (forward + reverse)",
-            "seconds": el, "bytes_per_macroparticle_turn": 80,
-            "whole_frac": bytes_ / el / 1e9 / HBM_PEAK_GBS}
+            "seconds": el, "bytes_per_macroparticle_turn": bpp,
+            "whole_frac": bpp * n * turns / el / 1e9 / HBM_PEAK_GBS}
+
+
+def form_adjoint(torch, turns=500) -> dict:
+    """c5 as configured: forward + adjoint with checkpoints every 50 turns
+    and recomputation, 1e6 macroparticles x 500 turns; the variant keeps
+    every state in HBM (8 GB) and recomputes nothing."""
+    out = _adjoint(torch, turns, 50)
+    out["variants"] = {"every_state_kept": _adjoint(torch, turns, 1)}
+    return out
 
 
 def measured_traffic(workload: str, n: int):
