@@ -15,7 +15,8 @@ namespace blmi {
 // LDS budget of the turn kernel's private histogram: u32 per bin when it
 // fits, else two u16 counters per u32 word (a workgroup then takes at most
 // 65535 particles, so no counter can carry into its neighbour).
-constexpr int kLdsHistBytes = 131072;
+constexpr int kLdsHistBytes = 131072;    // histogram words in LDS at most
+constexpr int kLdsHistU32Bytes = 65536;  // u32 counts up to this, u16 pairs above
 constexpr int kMaxParticlesPerGroup16 = 65535;
 // LDS the particle kernel may use for its histogram plus a staged copy of the
 // interpolation table (random 16-byte gathers served by LDS instead of L2)

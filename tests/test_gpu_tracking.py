@@ -136,6 +136,18 @@ def test_plain_turns_2e16_slices_bit_exact(n_particles):
     np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
 
 
+
+@pytest.mark.parametrize("n_slices", [16384, 16385, 24576, 32768])
+def test_plain_turns_histogram_mode_boundary_bit_exact(n_slices):
+    """u32 LDS counts up to 16384 slices, u16 pairs with partial rows above
+    (where u32 would still fit in LDS): both sides of the switch."""
+    w = build_workload("c1", n_particles=300001, n_turns=4, n_slices=n_slices)
+    ref = orc.track(w.snapshot(), 3)
+    w.fused(3)
+    np.testing.assert_array_equal(host(w.beam.dt), ref["dt"])
+    np.testing.assert_array_equal(host(w.beam.dE), ref["dE"])
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
+
 def test_intensity_turns_object_by_object():
     w = build_workload("c2", n_particles=40001, n_turns=12, n_slices=512, n_fft=2048)
     ref = orc.track(w.snapshot(), 8)
