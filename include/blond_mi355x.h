@@ -287,10 +287,11 @@ int blmi_engine_get_statistics(void *engine, int turn0, int turns, double *momen
  * and the number of turns per captured graph. */
 int blmi_engine_set_launch(void *engine, int block, int blocks_per_cu, int graph_chunk);
 /* Induced-voltage method: 0 = one-workgroup LDS FFT (rfft, x Z, irfft),
- * 1 = direct convolution over all CUs (n_slices <= 8192), 2 = four-step FFT
- * over many workgroups (any 5-smooth n_fft with a factor 16).  set_induced
- * picks 1, else 0, else 2; a multi-turn memory longer than n_slices moves 1
- * to 0 (or 2). */
+ * 1 = direct convolution over all CUs (n_slices <= 8192; with a multi-turn
+ * memory it forms all n_mem values and runs the wake step in the same
+ * launch), 2 = four-step FFT over many workgroups (any 5-smooth n_fft with a
+ * factor 16).  set_induced picks 1, else 0 below 8192 points, else 2; a
+ * multi-turn memory with n_mem * n_slices > 2^26 moves 1 to the FFT. */
 int blmi_engine_set_method(void *engine, int method);
 int blmi_engine_get_method(void *engine, int *method);
 /* Histogram: `groups` copies of the global counts (1..16, or 0 = automatic:
@@ -375,7 +376,8 @@ int blmi_ensemble_autotune(void *ensemble, int turns, int *block, int *wg_per_cu
  * and linear_interp_kick.cpp:21-71's slope jumps there) and V = -scale * (g circular-convolved with the
  * profile), g = irfft(Z, n_fft) (impedance.py:457-489).  forward() tracks
  * `turns` turns from the particles given to set_particles (copied), keeping
- * (dt, dE) every `checkpoint_every` turns; backward() takes dL/d(final
+ * (dt, dE) every `checkpoint_every` turns (1: every state, and backward()
+ * recomputes nothing; n_turns x 16 B per macroparticle of HBM); backward() takes dL/d(final
  * profile) and returns dL/dV_j, dL/dphi_j summed over turns
  * (grad_rf[2j], grad_rf[2j+1], host) and corr[m] = sum_t sum_i
  * dL/dV_t[i] p_t[(i - m) mod n_fft] (host), from which dL/dtheta of any
