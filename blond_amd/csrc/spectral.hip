@@ -871,7 +871,7 @@ int launch_fast_resonator(double *re, double *im, const double *freq, const doub
 // contiguous run (coalesced); LDS rows are padded by one element.
 // ---------------------------------------------------------------------------
 constexpr int kMaxCols = 8;
-constexpr int kColThreads = 256;
+constexpr int kColThreads = 512;
 
 __device__ __forceinline__ void count_buffers(const InducedArgs &a, const unsigned *&cin,
                                               unsigned *&cclr) {
@@ -1127,7 +1127,9 @@ int fft2_plan_create(int64_t n_real, Fft2Plan *plan) {
     // pass B: at most 4 elements a thread (registers hold their operands)
     int tb = (2 * M2 + 3) / 4;
     tb = ((tb + 63) / 64) * 64;
-    p.threads_b = tb < 256 ? 256 : tb;  // <= kPassBThreads for M2 <= 1024
+    // at least 512 threads (c3's 2^17 transform: passes at 512 threads take
+    // the three launches from 33.1 to 31.6 us, with kColThreads = 512)
+    p.threads_b = tb < 512 ? 512 : tb;  // <= kPassBThreads for M2 <= 1024
     auto cis = [](long double num, long double den) {
         const long double two_pi = 6.283185307179586476925286766559L;
         long double x = two_pi * num / den;
