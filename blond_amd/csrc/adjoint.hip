@@ -185,11 +185,16 @@ __global__ void __launch_bounds__(kAdjBlock) adj_turn_kernel(const double *xi, c
     if (DEPOSIT) deposit_flush(hdep, a, (t + 1) & 1);
 }
 
-// Circular dot product over k = r, r + 4, r + 8, ... < k_end of
-// G[m(k)] X[k], m(k) = (s0 + SGN k) mod n_fft (s0 + SGN k within
-// (-n_fft, 2 n_fft)), G read as zero from g_len on; four accumulators so
-// the LDS reads of consecutive terms overlap.  Lane r of a quad takes
-// residue r: the quad reads consecutive words.
+// Lanes that share one output of a circular product (each a residue class
+// of the k range); 16 with 1024-thread workgroups (c5: 2.51e10 against
+// 2.42e10 macroparticle-turns/s with 4 lanes and 256 threads).
+constexpr int kLanes = 16;
+
+// Circular dot product over k = r, r + kLanes, ... < k_end of G[m(k)] X[k],
+// m(k) = (s0 + SGN k) mod n_fft (s0 + SGN k within (-n_fft, 2 n_fft)), G
+// read as zero from g_len on; four accumulators so the LDS reads of
+// consecutive terms overlap.  Lane r of a group takes residue r: the group
+// reads consecutive words.
 template <int SGN>
 __device__ __forceinline__ double circ_dot(const double *G, int n_fft, int g_len, const double *X,
                                            int r, int k_end, int s0) {
@@ -200,31 +205,31 @@ __device__ __forceinline__ double circ_dot(const double *G, int n_fft, int g_len
         const double g = G[m < g_len ? m : 0];
         return m < g_len ? g : 0.0;
     };
+    constexpr int L = kLanes;
     double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
     int k = r;
-    for (; k + 12 < k_end; k += 16) {
+    for (; k + 3 * L < k_end; k += 4 * L) {
         v0 = fma(gat(k), X[k], v0);
-        v1 = fma(gat(k + 4), X[k + 4], v1);
-        v2 = fma(gat(k + 8), X[k + 8], v2);
-        v3 = fma(gat(k + 12), X[k + 12], v3);
+        v1 = fma(gat(k + L), X[k + L], v1);
+        v2 = fma(gat(k + 2 * L), X[k + 2 * L], v2);
+        v3 = fma(gat(k + 3 * L), X[k + 3 * L], v3);
     }
-    for (; k < k_end; k += 4) v0 = fma(gat(k), X[k], v0);
+    for (; k < k_end; k += L) v0 = fma(gat(k), X[k], v0);
     return (v0 + v1) + (v2 + v3);
 }
 
-// An output of a circular product is computed by a quad of lanes (each a
-// residue class of the k range); the quads stay together because every
-// trip count below is a multiple of 4.  The quad's sum, in every lane.
-constexpr int kQuad = 4;
-__device__ __forceinline__ double quad_sum(double v) {
-    v += __shfl_xor(v, 1);
-    v += __shfl_xor(v, 2);
+// An output of a circular product is computed by a group of kLanes lanes;
+// the groups stay together because every trip count below is a multiple of
+// kLanes.  The group's sum, in every lane.
+__device__ __forceinline__ double lanes_sum(double v) {
+#pragma unroll
+    for (int o = 1; o < kLanes; o <<= 1) v += __shfl_xor(v, o);
     return v;
 }
 
-// Outputs per workgroup of the slice-axis kernels (kAdjSliceBlock / kQuad).
-constexpr int kAdjSliceBlock = 256;
-constexpr int kAdjSliceOut = kAdjSliceBlock / kQuad;
+// Outputs per workgroup of the slice-axis kernels (kAdjSliceBlock / kLanes).
+constexpr int kAdjSliceBlock = 1024;
+constexpr int kAdjSliceOut = kAdjSliceBlock / kLanes;
 
 // p_t from the deposit copies of parity t & 1, stored; with `voltage`,
 // V_t = -scale * (g conv p_t) stored too.  Workgroup b owns outputs
@@ -260,9 +265,9 @@ __global__ void __launch_bounds__(kAdjSliceBlock) adj_profile_kernel(AdjArgs a, 
     for (int m = threadIdx.x; m < n_fft; m += blockDim.x) gs[m] = g[m];
     __syncthreads();
     // V_t[i] = -scale sum_j g[(i - j) mod n_fft] p[j]
-    for (int q = threadIdx.x; q < (o1 - o0) * kQuad; q += blockDim.x) {
-        const int i = o0 + q / kQuad, r = q % kQuad;
-        const double v = quad_sum(circ_dot<-1>(gs, n_fft, n_fft, p, r, ns, i));
+    for (int q = threadIdx.x; q < (o1 - o0) * kLanes; q += blockDim.x) {
+        const int i = o0 + q / kLanes, r = q % kLanes;
+        const double v = lanes_sum(circ_dot<-1>(gs, n_fft, n_fft, p, r, ns, i));
         if (r == 0) vt[(size_t)t * ns + i] = -(scale * v);
     }
 }
@@ -416,8 +421,8 @@ __global__ void __launch_bounds__(kAdjSliceBlock) adj_iv_kernel(AdjArgs a, int t
     const int hi0 = max(ns, n_fft - ns + 1);
     const int n_out = ns + ns + (n_fft - hi0);
     const int o0 = blockIdx.x * kAdjSliceOut, o1 = min(n_out, o0 + kAdjSliceOut);
-    for (int q = threadIdx.x; q < (o1 - o0) * kQuad; q += blockDim.x) {
-        const int o = o0 + q / kQuad, r = q % kQuad;
+    for (int q = threadIdx.x; q < (o1 - o0) * kLanes; q += blockDim.x) {
+        const int o = o0 + q / kLanes, r = q % kLanes;
         int m = 0;
         double v;
         if (o < ns) {
@@ -426,7 +431,7 @@ __global__ void __launch_bounds__(kAdjSliceBlock) adj_iv_kernel(AdjArgs a, int t
             m = o - ns < ns ? o - ns : hi0 + (o - 2 * ns);
             v = circ_dot<1>(p, n_fft, ns, lvs, r, ns, -m);  // corr[m]
         }
-        v = quad_sum(v);
+        v = lanes_sum(v);
         if (r == 0) {
             if (o < ns) lp_out[o] = -(scale * v);
             else corr[m] += v;
