@@ -100,7 +100,8 @@ struct Engine {
     double *d_fb = nullptr;         // harmonic, omega_rf_d [nt][n_rf]; prog [nt][kFbCols];
                                     // on_time [n_on]; state [kFbState + n_rf]
     double *fb_h = nullptr, *fb_wd = nullptr, *fb_prog = nullptr, *fb_on_time = nullptr,
-           *fb_state = nullptr;
+           *fb_state = nullptr, *fb_parts = nullptr;
+    unsigned *fb_arrive = nullptr;  // the split loop's arrival counter (zero between launches)
     // FP32 mode: float particles and programs; the slice axis (profile,
     // induced voltage, table) stays FP64 in engine-owned buffers and is
     // converted to the caller's float arrays at the end of every chunk

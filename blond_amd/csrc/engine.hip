@@ -1016,13 +1016,17 @@ int Engine::set_feedback(int machine, int delay, int first_bin, double alpha, do
         return blmi_set_error(BLMI_ERR_ARG, "feedback: null program");
     const size_t nt = (size_t)n_turns + 1;
     const size_t nh = nt * (size_t)n_rf, np = nt * kFbCols, ns = (size_t)kFbState + n_rf;
-    const size_t total = 2 * nh + np + (size_t)n_on_time + ns;
+    // after the state: the split loop's partial sums and its arrival counter
+    const size_t total = 2 * nh + np + (size_t)n_on_time + ns + 2 * kFbMaxParts + 1;
     BLMI_HIP_TRY(hipMalloc(&d_fb, sizeof(double) * total));
     fb_h = d_fb;
     fb_wd = fb_h + nh;
     fb_prog = fb_wd + nh;
     fb_on_time = fb_prog + np;
     fb_state = fb_on_time + n_on_time;
+    fb_parts = fb_state + ns;
+    fb_arrive = reinterpret_cast<unsigned *>(fb_parts + 2 * kFbMaxParts);
+    BLMI_HIP_TRY(hipMemset(fb_arrive, 0, sizeof(double)));
     BLMI_HIP_TRY(hipMemcpy(fb_h, harmonic, sizeof(double) * nh, hipMemcpyHostToDevice));
     BLMI_HIP_TRY(hipMemcpy(fb_wd, omega_d, sizeof(double) * nh, hipMemcpyHostToDevice));
     BLMI_HIP_TRY(hipMemcpy(fb_prog, prog, sizeof(double) * np, hipMemcpyHostToDevice));
@@ -1129,6 +1133,9 @@ FeedbackArgs Engine::feedback_args(int off) const {
     a.n_on_time = fb_n_on;
     a.state = fb_state;
     a.from_counts = 0;
+    a.n_parts = 1;  // the direct convolution's launch splits the sums
+    a.parts = fb_parts;
+    a.arrive = fb_arrive;
     return a;
 }
 

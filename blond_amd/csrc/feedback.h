@@ -64,7 +64,15 @@ struct FeedbackArgs {
     const double *on_time;        // [n_on_time] PSB loop turns
     int n_on_time;
     double *state;                // [kFbState + n_rf]
+    // n_parts > 1: the trapezoid sums are split over that many workgroups;
+    // each leaves its interior sums in parts[2 part], parts[2 part + 1] and
+    // the last to arrive (arrive counts them, zero between launches) adds
+    // them in part order and runs the control law
+    int n_parts;
+    double *parts;                // [2 kFbMaxParts]
+    unsigned *arrive;
 };
+constexpr int kFbMaxParts = 16;
 int launch_feedback(const FeedbackArgs &a, hipStream_t s);
 
 // beam_phase.cpp:6-39 (alpha_mode 1) and 42-65 (beam_phase_fast, no window

@@ -73,13 +73,16 @@ __device__ __forceinline__ void phase_terms(double bc, double p, double alpha, d
 
 // ---------------------------------------------------------------------------
 // one turn of the loop (any workgroup size of whole wavefronts, at least n_rf
-// threads)
+// threads), as workgroup `part` of a.n_parts (one when n_parts <= 1)
 // ---------------------------------------------------------------------------
-__device__ inline void feedback_turn_body(const FeedbackArgs &a) {
+__device__ inline void feedback_turn_body(const FeedbackArgs &a, int part = 0) {
     __shared__ double domega_sh;
+    __shared__ int last_sh;
     const int t = *a.turn_base + a.turn_off;
     const int nr = a.n_rf;
     const bool active = t >= a.delay;
+    const int n_parts = a.n_parts > 1 ? a.n_parts : 1;
+    if (!active && part != 0) return;  // the RF update below is part 0's
     double *dphi_rf = a.state + kFbState;
     if (active) {
         const unsigned *cin = nullptr;
@@ -98,7 +101,49 @@ __device__ inline void feedback_turn_body(const FeedbackArgs &a) {
         auto f = [&](int i, double &s, double &c) {
             phase_terms(a.bc[b0 + i], prof(b0 + i), a.alpha, omega, phi, false, s, c);
         };
-        const double coeff = trapz_ratio(f, ns - b0, a.bin_size);
+        double coeff;
+        if (n_parts == 1) {
+            coeff = trapz_ratio(f, ns - b0, a.bin_size);
+        } else {
+            // trapz_const_delta split by interior index over the parts
+            const int n = ns - b0;
+            double ps = 0.0, pc = 0.0;
+            for (int i = 1 + part * (int)blockDim.x + (int)threadIdx.x; i < n - 1;
+                 i += n_parts * (int)blockDim.x) {
+                double s, c;
+                f(i, s, c);
+                ps += s;
+                pc += c;
+            }
+            block_sum2(ps, pc);
+            if (threadIdx.x == 0) {
+                __hip_atomic_store(&a.parts[2 * part], ps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&a.parts[2 * part + 1], pc, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                // release of this part's sums, acquire of the earlier parts'
+                const unsigned prev = __hip_atomic_fetch_add(a.arrive, 1u, __ATOMIC_ACQ_REL,
+                                                             __HIP_MEMORY_SCOPE_AGENT);
+                last_sh = prev == (unsigned)(n_parts - 1);
+            }
+            __syncthreads();
+            if (!last_sh) return;  // uniform over the workgroup
+            coeff = 0.0;
+            if (threadIdx.x == 0) {
+                double ss = 0.0, sc = 0.0;
+                for (int k = 0; k < n_parts; ++k) {
+                    ss += __hip_atomic_load(&a.parts[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    sc += __hip_atomic_load(&a.parts[2 * k + 1], __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+                }
+                __hip_atomic_store(a.arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                double s0, c0, s1, c1;
+                f(0, s0, c0);
+                f(n - 1, s1, c1);
+                const double psum_s = (s0 + s1) / 2. + ss;
+                const double psum_c = (c0 + c1) / 2. + sc;
+                coeff = (a.bin_size * psum_s) / (a.bin_size * psum_c);
+            }
+        }
         if (threadIdx.x == 0) {
             const double *pr = a.prog + (size_t)t * kFbCols;
             double *st = a.state;

@@ -9,6 +9,7 @@
 // exact table computed on the host in extended precision.
 #include <math.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -460,9 +461,10 @@ template <int C, int TH, int CHUNK>
 __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     static_assert(C >= 2 && C <= 32 && (C & (C - 1)) == 0, "C is a power of two up to 32");
     static_assert(CHUNK % 8 == 0, "a thread's samples start on a pad period");
-    if (a.with_fb && blockIdx.x == 0) {
+    const int nfb = a.with_fb ? a.fb.n_parts : 0;
+    if ((int)blockIdx.x < nfb) {
         // the beam phase loop of this turn, independent of the convolution
-        feedback_turn_body(a.fb);
+        feedback_turn_body(a.fb, blockIdx.x);
         return;
     }
     extern __shared__ double conv_lds[];
@@ -471,9 +473,9 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     __shared__ double memv[C + 1];
     constexpr int kSpan = CHUNK * TH;
     const int ns = a.n_slices;
-    // with the loop, workgroup 0 runs it (dispatched first: it is the
-    // longest) and workgroup b > 0 takes outputs of b - 1
-    const int i0 = (blockIdx.x - (a.with_fb ? 1 : 0)) * C;
+    // with the loop, workgroups 0 .. nfb - 1 run it (dispatched first: they
+    // are the longest) and workgroup b >= nfb takes outputs of b - nfb
+    const int i0 = (blockIdx.x - nfb) * C;
     // wake step: the previous memory's value one revolution later at this
     // workgroup's outputs (impedance.py:536-550), read first so that its
     // dependent loads overlap the staging and the products below
@@ -629,8 +631,20 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
 static int set_lds_limit(const void *fn, size_t bytes);
 
 template <int C, int TH, int CHUNK>
-static int launch_direct_shape(const InducedArgs &a, hipStream_t s, const KernelEvents &ev) {
-    const int blocks = (a.n_out + C - 1) / C + (a.with_fb ? 1 : 0);
+static int launch_direct_shape(const InducedArgs &a0, hipStream_t s, const KernelEvents &ev) {
+    InducedArgs a = a0;
+    int nfb = 0;
+    if (a.with_fb) {
+        // the loop's sums over ceil(bins / TH) workgroups (one bin a thread),
+        // when the caller provides the partial-sum buffer and counter
+        nfb = 1;
+        if (a.fb.parts && a.fb.arrive) {
+            const int nb = a.fb.n_slices - a.fb.first_bin;
+            nfb = std::min(kFbMaxParts, std::max(1, (nb + TH - 1) / TH));
+        }
+        a.fb.n_parts = nfb;
+    }
+    const int blocks = (a.n_out + C - 1) / C + nfb;
     if (a.with_fb && a.fb.n_rf > TH)
         return blmi_set_error(BLMI_ERR_ARG, "direct induced voltage: too many RF systems for the loop");
     constexpr int kSpan = CHUNK * TH;
