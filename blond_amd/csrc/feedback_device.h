@@ -84,6 +84,25 @@ __device__ inline void feedback_turn_body(const FeedbackArgs &a, int part = 0) {
     const int n_parts = a.n_parts > 1 ? a.n_parts : 1;
     if (!active && part != 0) return;  // the RF update below is part 0's
     double *dphi_rf = a.state + kFbState;
+    // inputs of the serial tail, read before the sums so that their latency
+    // overlaps them: nothing in this launch writes them before their use
+    const int j = threadIdx.x;
+    double nx_w = 0.0, nx_p = 0.0, hj = 0.0, h0 = 0.0, wdj = 0.0, dpj = 0.0;
+    if (j < nr) {
+        const double *nxt = a.rf + (size_t)(t + 1) * 3 * nr;
+        const double *h = a.harmonic + (size_t)(t + 1) * nr;
+        nx_w = nxt[nr + j];
+        nx_p = nxt[2 * nr + j];
+        hj = h[j];
+        h0 = h[0];
+        wdj = a.omega_d[(size_t)(t + 1) * nr + j];
+        dpj = dphi_rf[j];
+    }
+    double on_time_pc = 0.0;  // PSB: the loop turn the counter points at
+    if (active && a.machine == kFbPSB && threadIdx.x == 0) {
+        const int pc = (int)a.state[kFbPLCounter];
+        if (pc < a.n_on_time) on_time_pc = a.on_time[pc];
+    }
     if (active) {
         const unsigned *cin = nullptr;
         if (a.from_counts || (a.pending && a.pending[t & 1] == t))
@@ -164,7 +183,7 @@ __device__ inline void feedback_turn_body(const FeedbackArgs &a, int part = 0) {
                 // beam_feedback.py:563-622
                 st[kFbDphiSum] += dphi;
                 const int pc = (int)st[kFbPLCounter];
-                if (pc < a.n_on_time && (double)t == a.on_time[pc] && t >= a.delay) {
+                if (pc < a.n_on_time && (double)t == on_time_pc && t >= a.delay) {
                     const int prev = pc - 1 < 0 ? a.n_on_time - 1 : pc - 1;
                     st[kFbDphiAv] = st[kFbDphiSum] / (a.on_time[pc] - a.on_time[prev]);
                     st[kFbDomegaPL] = 0.99803799 * st[kFbDomegaPL] +
@@ -191,14 +210,13 @@ __device__ inline void feedback_turn_body(const FeedbackArgs &a, int part = 0) {
     __syncthreads();
     // beam_feedback.py:246-251: the frequency of the next turn; then
     // tracker.py:551-565: the accumulated phase offset and the next phase
-    const int j = threadIdx.x;
     if (j < nr) {
         double *nxt = a.rf + (size_t)(t + 1) * 3 * nr;
-        const double *h = a.harmonic + (size_t)(t + 1) * nr;
-        const double *wd = a.omega_d + (size_t)(t + 1) * nr;
-        if (active) nxt[nr + j] += domega_sh * h[j] / h[0];
-        dphi_rf[j] += 2.0 * M_PI * h[j] * (nxt[nr + j] - wd[j]) / wd[j];
-        nxt[2 * nr + j] += dphi_rf[j];
+        if (active) nx_w += domega_sh * hj / h0;
+        nxt[nr + j] = nx_w;
+        dpj += 2.0 * M_PI * hj * (nx_w - wdj) / wdj;
+        dphi_rf[j] = dpj;
+        nxt[2 * nr + j] = nx_p + dpj;
     }
 }
 
