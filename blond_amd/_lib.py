@@ -71,6 +71,7 @@ SIGNATURES = {
     "blmi_engine_track": [_p, _i, _i, _p],
     "blmi_engine_time_kernels": [_p, _i, ct.POINTER(_f), ct.POINTER(_f), _p],
     "blmi_engine_set_method": [_p, _i],
+    "blmi_engine_set_interpolation": [_p, _p],
     "blmi_engine_get_method": [_p, ct.POINTER(_i)],
     "blmi_engine_set_hist": [_p, _i, _i],
     "blmi_engine_set_order": [_p, _i],
@@ -95,6 +96,7 @@ SIGNATURES = {
     "blmi_adjoint_set_particles": [_p, _p, _p],
     "blmi_adjoint_set_programs": [_p, _p, _p, _p, _d, _d, _p, _p, _p, _p, _p, _p, _p, _p, _d],
     "blmi_adjoint_set_slices": [_p, _d, _d],
+    "blmi_adjoint_set_map": [_p, _i, _p],
     "blmi_adjoint_set_wake": [_p, _i64, _p, _d],
     "blmi_adjoint_forward": [_p, _i, _p, _p],
     "blmi_adjoint_backward": [_p, _p, _p, _p, _p],

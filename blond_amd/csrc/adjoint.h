@@ -1,5 +1,6 @@
-// Differentiable tracking: forward turns with a smooth (quadratic B-spline)
-// profile deposit, turn checkpoints, and the hand-written reverse pass.
+// Differentiable tracking: forward turns with a smooth profile (the
+// reference's smooth_histogram and linear_interp_kick, or a quadratic
+// B-spline variant), turn checkpoints, and the hand-written reverse pass.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,6 +23,10 @@ struct Adjoint {
     int64_t n_fft = 0;
     double charge = 1.0, scale = 0.0;
     double cut_left = 0, inv_bw_cut = 0;  // deposit: bin = (dt - cut_left) * inv_bw_cut - 1/2
+    double lo = 0, hi = 0;                // smooth_histogram's particle range
+    int map = 0;                          // 0: spline variant, 1: the reference's functions
+    double *d_bc = nullptr;               // [n_slices] bin centres (reference map)
+    double bc0 = 0, inv_bw_bc = 0;        // linear_interp_kick's table origin and scale
     int fixed_bits = 40;                  // deposit fixed point 2^-S (exact, order-free sums)
     // programs
     double *d_rf = nullptr, *d_acc = nullptr;
@@ -61,6 +66,7 @@ struct Adjoint {
                      const double *alpha1, const double *alpha2, const double *beta,
                      const double *energy, double charge);
     int set_slices(double cut_left, double cut_right);
+    int set_map(int map, const double *bin_centers_host);
     int set_wake(int64_t n_fft, const double *g_host, double scale);
     void invalidate_graphs();
     int ensure_stream();

@@ -1,39 +1,50 @@
 // Differentiable tracking on the MI355X: forward turns, checkpoints and the
 // hand-written reverse (adjoint) pass.
 //
-// The map per turn t (the intensity map of the fused engine, with the
-// profile deposited continuously so the loss is differentiable):
+// The map per turn t (the intensity map of the fused engine with a smooth
+// profile), in one of two forms chosen at compile time (MAP):
+//
+// MAP = kMapReference, the reference's own functions:
+//   p_t     = smooth_histogram(dt_t)   histogram.cpp:75-132 with its quirk:
+//                            f = (dt - cut_left) inv_bw, k = (int) f,
+//                            d = f - k; bin k gets 0.5 - d, bin
+//                            k2 = (int)(f + 1) (d > 1/2) or (int)(f - 1)
+//                            gets 0.5 + d; particles outside
+//                            [cut_left + bw/2, cut_right - bw/2] dropped
+//   e1      = linear_interp_kick(dt, dE, V_t, bin_centers, q, 0)
+//                            (linear_interp_kick.cpp:21-71: slope and
+//                            intercept of each bin, formed per workgroup in
+//                            LDS from V_t in the reference's order)
+//   The weights are piecewise linear in dt, with jumps where a particle
+//   crosses a bin centre or edge; the reverse pass uses their derivatives on
+//   the pieces: d p[k]/d dt = -inv_bw, d p[k2]/d dt = +inv_bw, d kick/d dt =
+//   the bin's slope, d kick/d V[k], V[k+1] = q (1 - s), q s with
+//   s = (dt - bc[k]) inv_bw_bc.
+//
+// MAP = kMapSpline, a variant with continuous first derivatives:
 //   p_t     = D(dt_t)        quadratic B-spline deposit on bin centres: a
 //                            particle at bin-centre coordinate
 //                            u = (dt - cut_left) inv_bw - 1/2 = k + d,
 //                            k = round(u), adds (1/2 - d)^2 / 2, 3/4 - d^2,
 //                            (1/2 + d)^2 / 2 to bins k-1, k, k+1
+//   e1      = dE + q sum_b V_t[b] w_b(dt)   the same spline weights
+//
+// Both:
 //   V_t     = -scale * (g conv p_t)[:ns],  g = irfft(Z, n_fft) (circular,
 //                            as the FFT route of impedance.py:457-489)
-//   e1      = dE + q sum_b V_t[b] w_b(dt)   the induced kick with the same
-//                            spline weights (linear_interp_kick.cpp:21-71
-//                            interpolates linearly: its slope jumps at every
-//                            bin centre)
 //   e2      = e1 + q sum_j V_j sin(w_j dt + phi_j) + acc   (kick.cpp:16-35)
 //   dt_{t+1} = dt + coeff_{t+1} e2,  dE_{t+1} = e2          (drift.cpp "simple")
-// and the loss depends on p_T.  The reference's smooth_histogram
-// (histogram.cpp:75-132) moves a particle's whole weight between bins k-1
-// and k+1 when it crosses the centre of bin k, so it has no derivative
-// there.  Deposit and kick here have continuous first derivatives, so
-// central differences of the loss converge at second order (linear
-// weights' kinks at bin centres leave first-order errors of the order of
-// the fraction of particles within one step of a bin centre).
+// and the loss depends on p_T.
 //
 // Reverse turn t, per particle (state at t recomputed from a checkpoint):
-//   lx <- lx + inv_bw sum_b lp_{t+1}[b] dw_b/du              deposit of p_{t+1}
+//   lx <- lx + sum_b lp_{t+1}[b] dp_b/d dt                    deposit of p_{t+1}
 //   le2 = le + coeff lx
-//   lx <- lx + le2 (q sum_j V_j w_j cos(.)
-//                   + q inv_bw sum_b V_t[b] dw_b/du)         RF + induced kick
-//   lV_t[b] += le2 q w_b                                      induced kick
+//   lx <- lx + le2 (q sum_j V_j w_j cos(.) + d kick / d dt)   RF + induced kick
+//   lV_t[b] += le2 d kick / d V_t[b]                           induced kick
 //   dL/dV_j += le2 q sin(.),  dL/dphi_j += le2 q V_j cos(.)
 //   le <- le2
-// then, once per turn on one workgroup: lp_t = -scale * (g corr lV_t) and
-// corr[m] += sum_i lV_t[i] p_t[(i - m) mod n_fft], from which the host
+// then, once per turn over a few workgroups: lp_t = -scale * (g corr lV_t)
+// and corr[m] += sum_i lV_t[i] p_t[(i - m) mod n_fft], from which the host
 // forms dL/dZ-parameters as -scale * sum_m irfft(dZ/dtheta)[m] corr[m].
 //
 // Traffic per macroparticle-turn: 32 B forward, 32 B recompute (slot to
@@ -67,7 +78,15 @@ struct AdjArgs {
     double *lv;               // [copies][ns]
     const double *lp;         // [ns]
     double *gpart;            // [turn][grid][2 n_rf]
+    // the reference map: smooth_histogram's range and the interpolation's
+    // bin centres (histogram.cpp:82-85, linear_interp_kick.cpp:33)
+    double lo, hi;
+    const double *bc;  // [ns]
+    double bc0, inv_bw_bc;
 };
+
+constexpr int kMapSpline = 0;
+constexpr int kMapReference = 1;
 
 // Quadratic B-spline: u = (x - cut_left) inv_bw - 1/2 = k + d, k = round(u),
 // d in [-1/2, 1/2); weights of bins k-1, k, k+1 and their u-derivatives.
@@ -100,6 +119,57 @@ __device__ __forceinline__ void deposit_lds(unsigned long long *h, double x, con
     }
 }
 
+// smooth_histogram's two bins of a particle (histogram.cpp:100-113): bin k
+// gets 0.5 - d, bin k2 gets 0.5 + d.  False outside [lo, hi] (NaN
+// included); a bin outside [0, ns) (a rounding corner at hi) is skipped by
+// the callers.
+__device__ __forceinline__ bool smooth_bins(double x, const AdjArgs &a, int &k, int &k2,
+                                            double &d) {
+    if (!(x >= a.lo && x <= a.hi)) return false;
+    const double f = (x - a.cut_left) * a.inv_bw_cut;
+    k = (int)f;
+    d = f - (double)k;
+    k2 = d > 0.5 ? (int)(f + 1.0) : (int)(f - 1.0);
+    return true;
+}
+
+__device__ __forceinline__ void deposit_smooth(unsigned long long *h, double x, const AdjArgs &a) {
+    int k, k2;
+    double d;
+    if (!smooth_bins(x, a, k, k2, d)) return;
+    if (k >= 0 && k < a.ns)
+        atomicAdd(&h[k], (unsigned long long)__double2ll_rn((0.5 - d) * a.fix_scale));
+    if (k2 >= 0 && k2 < a.ns)
+        atomicAdd(&h[k2], (unsigned long long)__double2ll_rn((0.5 + d) * a.fix_scale));
+}
+
+template <int MAP>
+__device__ __forceinline__ void deposit(unsigned long long *h, double x, const AdjArgs &a) {
+    if (MAP == kMapReference) deposit_smooth(h, x, a);
+    else deposit_lds(h, x, a);
+}
+
+// linear_interp_kick.cpp:44-48 on V_t (acceleration 0, as
+// TotalInducedVoltage.track passes it): slope and intercept of bin b, into
+// LDS, in the reference's order of operations.
+__device__ __forceinline__ void interp_table_lds(double *vk, double *fac, const double *vt,
+                                                 const AdjArgs &a) {
+    for (int b = threadIdx.x; b < a.ns - 1; b += blockDim.x) {
+        const double v0 = vt[b], v1 = vt[b + 1];
+        const double k = a.charge * (v1 - v0) * a.inv_bw_bc;
+        vk[b] = k;
+        fac[b] = (a.charge * v0 - a.bc[b] * k) + 0.0;
+    }
+}
+
+// linear_interp_kick.cpp:56-63: the bin a particle is kicked by, or -1 (a
+// signed range test in place of the reference's unsigned cast: negative,
+// NaN and last-bin positions get no kick).
+__device__ __forceinline__ int interp_bin(double x, const AdjArgs &a) {
+    const double f = floor((x - a.bc0) * a.inv_bw_bc);
+    return (f >= 0.0 && f < (double)(a.ns - 1)) ? (int)f : -1;
+}
+
 // into deposit parity `par` (the profile p_t of turn t is parity t & 1)
 __device__ __forceinline__ void deposit_flush(unsigned long long *h, const AdjArgs &a, int par) {
     __syncthreads();
@@ -108,6 +178,7 @@ __device__ __forceinline__ void deposit_flush(unsigned long long *h, const AdjAr
         if (h[b]) atomic_add_agent(&out[b], h[b]);
 }
 
+template <int MAP>
 __global__ void __launch_bounds__(kAdjBlock) adj_deposit_kernel(const double *__restrict__ x,
                                                                 int64_t n, AdjArgs a) {
     extern __shared__ unsigned long long hdep[];
@@ -115,7 +186,7 @@ __global__ void __launch_bounds__(kAdjBlock) adj_deposit_kernel(const double *__
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        deposit_lds(hdep, x[i], a);
+        deposit<MAP>(hdep, x[i], a);
     deposit_flush(hdep, a, 0);
 }
 
@@ -139,13 +210,20 @@ __device__ __forceinline__ double induced_kick(const double *V, double x, const 
     return a.charge * s;
 }
 
+// LDS of the forward turn kernel: the deposit (u64) plus V_t (spline) or
+// the interpolation table's slopes and intercepts (reference).
+static size_t adj_turn_lds(int map, int ns) {
+    return sizeof(unsigned long long) * ns + sizeof(double) * (map == kMapReference ? 2 : 1) * ns;
+}
+
 // Forward turn t: (xi, ei) -> (xo, eo) (may alias); DEPOSIT adds p_{t+1}.
-template <bool DEPOSIT>
+template <bool DEPOSIT, int MAP>
 __global__ void __launch_bounds__(kAdjBlock) adj_turn_kernel(const double *xi, const double *ei,
                                                              double *xo, double *eo, int64_t n,
                                                              int t, AdjArgs a) {
     extern __shared__ unsigned long long hdep[];
-    double *vl = reinterpret_cast<double *>(hdep + a.ns);  // V_t
+    double *vl = reinterpret_cast<double *>(hdep + a.ns);  // V_t, or the table's slopes
+    double *fac = vl + a.ns;                               // the table's intercepts
     // software pipeline: a thread's next particle is loaded while the
     // current one is computed (the first before the LDS set-up)
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -155,10 +233,12 @@ __global__ void __launch_bounds__(kAdjBlock) adj_turn_kernel(const double *xi, c
         xn = xi[i_first];
         en = ei[i_first];
     }
-    for (int b = threadIdx.x; b < a.ns; b += blockDim.x) {
-        if (DEPOSIT) hdep[b] = 0ull;
-        vl[b] = a.vt[(size_t)t * a.ns + b];
-    }
+    if (DEPOSIT)
+        for (int b = threadIdx.x; b < a.ns; b += blockDim.x) hdep[b] = 0ull;
+    if (MAP == kMapReference)
+        interp_table_lds(vl, fac, a.vt + (size_t)t * a.ns, a);
+    else
+        for (int b = threadIdx.x; b < a.ns; b += blockDim.x) vl[b] = a.vt[(size_t)t * a.ns + b];
     __syncthreads();
     const double *V = a.rf + (size_t)t * 3 * a.n_rf;
     const double *w = V + a.n_rf;
@@ -171,16 +251,21 @@ __global__ void __launch_bounds__(kAdjBlock) adj_turn_kernel(const double *xi, c
             xn = xi[i + stride];
             en = ei[i + stride];
         }
-        double wk[3];
-        int k;
-        e = e + induced_kick(vl, x, a, nullptr, wk, k);
+        if (MAP == kMapReference) {
+            const int k = interp_bin(x, a);
+            if (k >= 0) e = e + (x * vl[k] + fac[k]);  // linear_interp_kick.cpp:60
+        } else {
+            double wk[3];
+            int k;
+            e = e + induced_kick(vl, x, a, nullptr, wk, k);
+        }
         double s = 0.0;
         for (int j = 0; j < a.n_rf; ++j) s = s + V[j] * cephes_sin(w[j] * x + phi[j]);
         e = e + (a.charge * s + acc);
         x = x + coeff * e;
         xo[i] = x;
         eo[i] = e;
-        if (DEPOSIT) deposit_lds(hdep, x, a);
+        if (DEPOSIT) deposit<MAP>(hdep, x, a);
     }
     if (DEPOSIT) deposit_flush(hdep, a, (t + 1) & 1);
 }
@@ -272,7 +357,14 @@ __global__ void __launch_bounds__(kAdjSliceBlock) adj_profile_kernel(AdjArgs a, 
     }
 }
 
+// LDS of the reverse kernel: lp_{t+1} and lV_t, plus V_t (spline) or the
+// interpolation table's slopes, intercepts and bin centres (reference).
+static size_t adj_reverse_lds(int map, int ns) {
+    return sizeof(double) * (map == kMapReference ? 5 : 3) * (size_t)ns;
+}
+
 // Reverse turn t (state (x, e) at turn t): see the file comment.
+template <int MAP>
 __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__restrict__ xs,
                                                                 const double *__restrict__ es,
                                                                 double *__restrict__ lam_x,
@@ -280,6 +372,7 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
                                                                 int64_t n, int t, AdjArgs a) {
     extern __shared__ double shr[];
     double *lp = shr, *lvs = shr + a.ns, *vl = shr + 2 * a.ns;
+    double *fac = shr + 3 * a.ns, *bcs = shr + 4 * a.ns;  // reference map only
     __shared__ double red[kAdjBlock / 64][2 * kAdjMaxRf];
     // software pipeline as in adj_turn_kernel
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -294,8 +387,10 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
     for (int b = threadIdx.x; b < a.ns; b += blockDim.x) {
         lp[b] = a.lp[b];
         lvs[b] = 0.0;
-        vl[b] = a.vt[(size_t)t * a.ns + b];
+        if (MAP == kMapReference) bcs[b] = a.bc[b];
+        else vl[b] = a.vt[(size_t)t * a.ns + b];
     }
+    if (MAP == kMapReference) interp_table_lds(vl, fac, a.vt + (size_t)t * a.ns, a);
     __syncthreads();
     const double *V = a.rf + (size_t)t * 3 * a.n_rf;
     const double *w = V + a.n_rf;
@@ -316,7 +411,14 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
         // forward step t, as adj_turn_kernel
         double wk[3], dkick;
         int k;
-        const double e1 = e + induced_kick(vl, x, a, &dkick, wk, k);
+        double e1;
+        if (MAP == kMapReference) {
+            k = interp_bin(x, a);
+            e1 = k >= 0 ? e + (x * vl[k] + fac[k]) : e;
+            dkick = k >= 0 ? vl[k] : 0.0;
+        } else {
+            e1 = e + induced_kick(vl, x, a, &dkick, wk, k);
+        }
         // constant trip counts with a guard keep sn, cs and gsum in registers
         double sn[kAdjMaxRf], cs[kAdjMaxRf];
         double s = 0.0, ds = 0.0;
@@ -332,17 +434,27 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
         const double e2 = e1 + (a.charge * s + acc);
         const double x1 = x + coeff * e2;
         // deposit adjoint of p_{t+1} at x1
-        double sw[3], sdw[3];
-        const int kk = spline(x1, a, sw, sdw);
         double lx = lam_x_i;
-        if (kk != kNoBin) {
-            double s3 = 0.0;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const int b = kk - 1 + q;
-                if (b >= 0 && b < a.ns) s3 += lp[b] * sdw[q];
+        if (MAP == kMapReference) {
+            int k1, k2;
+            double d;
+            if (smooth_bins(x1, a, k1, k2, d)) {
+                const double l1 = (k1 >= 0 && k1 < a.ns) ? lp[k1] : 0.0;
+                const double l2 = (k2 >= 0 && k2 < a.ns) ? lp[k2] : 0.0;
+                lx = lx + (l2 - l1) * a.inv_bw_cut;
             }
-            lx = lx + s3 * a.inv_bw_cut;
+        } else {
+            double sw[3], sdw[3];
+            const int kk = spline(x1, a, sw, sdw);
+            if (kk != kNoBin) {
+                double s3 = 0.0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const int b = kk - 1 + q;
+                    if (b >= 0 && b < a.ns) s3 += lp[b] * sdw[q];
+                }
+                lx = lx + s3 * a.inv_bw_cut;
+            }
         }
         const double le2 = lam_e_i + coeff * lx;
         lx = lx + le2 * (a.charge * ds);
@@ -354,7 +466,13 @@ __global__ void __launch_bounds__(kAdjBlock) adj_reverse_kernel(const double *__
             }
         }
         lx = lx + le2 * dkick;
-        if (k != kNoBin) {
+        if (MAP == kMapReference) {
+            if (k >= 0) {
+                const double sk = (x - bcs[k]) * a.inv_bw_bc;
+                atomicAdd(&lvs[k], le2 * a.charge * (1.0 - sk));
+                atomicAdd(&lvs[k + 1], le2 * a.charge * sk);
+            }
+        } else if (k != kNoBin) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 const int b = k - 1 + q;
@@ -485,10 +603,10 @@ void Adjoint::release() {
     own = nullptr;
     ev_in = ev_out = nullptr;
     void *bufs[] = {d_rf, d_acc, d_drift, d_x0, d_work, d_ckpt, d_slots, d_lam, d_dep,
-                    d_prof, d_vt, d_g, d_lv, d_lp, d_corr, d_gpart};
+                    d_prof, d_vt, d_g, d_lv, d_lp, d_corr, d_gpart, d_bc};
     for (void *p : bufs) (void)hipFree(p);
     d_rf = d_acc = d_x0 = d_work = d_ckpt = d_slots = d_lam = nullptr;
-    d_prof = d_vt = d_g = d_lv = d_lp = d_corr = d_gpart = nullptr;
+    d_prof = d_vt = d_g = d_lv = d_lp = d_corr = d_gpart = d_bc = nullptr;
     d_drift = nullptr;
     d_dep = nullptr;
 }
@@ -555,10 +673,31 @@ int Adjoint::set_programs(const double *rf_host, const double *acc_host, const d
 }
 
 int Adjoint::set_slices(double cl, double cr) {
-    const double ibw = (double)n_slices / (cr - cl);
+    const double ibw = (double)n_slices / (cr - cl);  // histogram.cpp:82
     if (cl != cut_left || ibw != inv_bw_cut) invalidate_graphs();
     cut_left = cl;
     inv_bw_cut = ibw;
+    const double bw = (cr - cl) / (double)n_slices;  // histogram.cpp:83-85
+    lo = cl + bw * 0.5;
+    hi = cr - bw * 0.5;
+    forward_done = false;
+    return BLMI_OK;
+}
+
+int Adjoint::set_map(int map_, const double *bc_host) {
+    if (map_ != kMapSpline && map_ != kMapReference)
+        return blmi_set_error(BLMI_ERR_ARG, "adjoint: unknown map %d", map_);
+    if (map_ == kMapReference) {
+        if (!bc_host || n_slices < 2)
+            return blmi_set_error(BLMI_ERR_ARG, "adjoint: the reference map needs >= 2 bin centres");
+        if (!d_bc) BLMI_HIP_TRY(hipMalloc(&d_bc, sizeof(double) * n_slices));
+        BLMI_HIP_TRY(hipMemcpy(d_bc, bc_host, sizeof(double) * n_slices, hipMemcpyHostToDevice));
+        bc0 = bc_host[0];
+        // linear_interp_kick.cpp:33-35
+        inv_bw_bc = (double)(n_slices - 1) / (bc_host[n_slices - 1] - bc_host[0]);
+    }
+    map = map_;
+    invalidate_graphs();
     forward_done = false;
     return BLMI_OK;
 }
@@ -605,59 +744,84 @@ static AdjArgs adj_args(const Adjoint &j) {
     a.lv = j.d_lv;
     a.lp = j.d_lp;
     a.gpart = j.d_gpart;
+    a.lo = j.lo;
+    a.hi = j.hi;
+    a.bc = j.d_bc;
+    a.bc0 = j.bc0;
+    a.inv_bw_bc = j.inv_bw_bc;
     return a;
 }
 
-int Adjoint::enqueue_forward(int turns, hipStream_t s) {
-    const AdjArgs a = adj_args(*this);
+template <int MAP>
+static int enqueue_forward_map(Adjoint &j, int turns, hipStream_t s) {
+    const int64_t n = j.n;
+    const int n_slices = j.n_slices, every = j.every, grid = j.grid;
+    const int64_t n_fft = j.n_fft;
+    const double scale = j.scale;
+    double *d_dep_ = reinterpret_cast<double *>(j.d_dep);
+    double *d_ckpt = j.d_ckpt, *d_work = j.d_work, *d_x0 = j.d_x0, *d_prof = j.d_prof;
+    double *d_g = j.d_g, *d_vt = j.d_vt;
+    const AdjArgs a = adj_args(j);
     const size_t pn = 2 * (size_t)n;
-    const size_t lds_dep = 2 * sizeof(double) * n_slices;  // deposit + V_t
+    const size_t lds_dep = adj_turn_lds(MAP, n_slices);
     const size_t lds_one = sizeof(double) * ((size_t)n_slices + n_fft);
     const int pgrid = adj_profile_grid(n_slices);
     double *x = d_work, *e = d_work + n;
     // both deposit parities start cleared (a previous pass leaves its last
     // profile's copies behind)
-    int rc = launch_fill<double>(reinterpret_cast<double *>(d_dep),
-                                 2 * (int64_t)kAdjCopies * n_slices, 0.0, s);
+    int rc = launch_fill<double>(d_dep_, 2 * (int64_t)kAdjCopies * n_slices, 0.0, s);
     if (rc) return rc;
     if (every == 1) {
         // every state is kept (n_turns x 16 B per macroparticle of HBM): turn t
         // reads checkpoint t and writes checkpoint t + 1 (the last state goes
         // to the work array), and the reverse pass recomputes nothing
         adj_copy(d_ckpt, d_x0, (int64_t)pn, grid, s);
-        hipLaunchKernelGGL(adj_deposit_kernel, dim3(grid), dim3(kAdjBlock), lds_dep, s, d_ckpt, n, a);
+        hipLaunchKernelGGL(adj_deposit_kernel<MAP>, dim3(grid), dim3(kAdjBlock), lds_dep, s, d_ckpt,
+                           n, a);
         hipLaunchKernelGGL(adj_profile_kernel, dim3(pgrid), dim3(kAdjSliceBlock), lds_one, s, a, 0,
                            true, d_prof, d_g, (int)n_fft, scale, d_vt);
         for (int t = 0; t < turns; ++t) {
             const double *si = d_ckpt + (size_t)t * pn;
             double *so = t + 1 < turns ? d_ckpt + (size_t)(t + 1) * pn : d_work;
-            hipLaunchKernelGGL(adj_turn_kernel<true>, dim3(grid), dim3(kAdjBlock), lds_dep, s, si,
-                               si + n, so, so + n, n, t, a);
+            hipLaunchKernelGGL((adj_turn_kernel<true, MAP>), dim3(grid), dim3(kAdjBlock), lds_dep, s,
+                               si, si + n, so, so + n, n, t, a);
             hipLaunchKernelGGL(adj_profile_kernel, dim3(pgrid), dim3(kAdjSliceBlock), lds_one, s, a,
                                t + 1, t + 1 < turns, d_prof, d_g, (int)n_fft, scale, d_vt);
         }
         return blmi_check_launch("adjoint_forward");
     }
     adj_copy(d_work, d_x0, (int64_t)pn, grid, s);
-    hipLaunchKernelGGL(adj_deposit_kernel, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, n, a);
+    hipLaunchKernelGGL(adj_deposit_kernel<MAP>, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, n, a);
     hipLaunchKernelGGL(adj_profile_kernel, dim3(pgrid), dim3(kAdjSliceBlock), lds_one, s, a, 0,
                        true, d_prof, d_g, (int)n_fft, scale, d_vt);
     for (int t = 0; t < turns; ++t) {
         if (t % every == 0) adj_copy(d_ckpt + (size_t)(t / every) * pn, d_work, (int64_t)pn, grid, s);
-        hipLaunchKernelGGL(adj_turn_kernel<true>, dim3(grid), dim3(kAdjBlock), lds_dep, s, x, e, x,
-                           e, n, t, a);
+        hipLaunchKernelGGL((adj_turn_kernel<true, MAP>), dim3(grid), dim3(kAdjBlock), lds_dep, s, x,
+                           e, x, e, n, t, a);
         hipLaunchKernelGGL(adj_profile_kernel, dim3(pgrid), dim3(kAdjSliceBlock), lds_one, s, a,
                            t + 1, t + 1 < turns, d_prof, d_g, (int)n_fft, scale, d_vt);
     }
     return blmi_check_launch("adjoint_forward");
 }
 
-int Adjoint::enqueue_backward(int T, hipStream_t s) {
+int Adjoint::enqueue_forward(int turns, hipStream_t s) {
+    return map == kMapReference ? enqueue_forward_map<kMapReference>(*this, turns, s)
+                                : enqueue_forward_map<kMapSpline>(*this, turns, s);
+}
+
+template <int MAP>
+static int enqueue_backward_map(Adjoint &j, int T, hipStream_t s) {
+    const int64_t n = j.n;
+    const int n_slices = j.n_slices, every = j.every, grid = j.grid;
+    const int64_t n_fft = j.n_fft;
+    const double scale = j.scale;
+    double *d_lam = j.d_lam, *d_corr = j.d_corr, *d_lv = j.d_lv, *d_ckpt = j.d_ckpt;
+    double *d_slots = j.d_slots, *d_prof = j.d_prof, *d_g = j.d_g, *d_lp = j.d_lp;
     const size_t pn = 2 * (size_t)n;
-    const size_t lds_dep = 2 * sizeof(double) * n_slices;
-    const size_t lds_rev = sizeof(double) * 3 * (size_t)n_slices;
+    const size_t lds_dep = adj_turn_lds(MAP, n_slices);
+    const size_t lds_rev = adj_reverse_lds(MAP, n_slices);
     const size_t lds_one = sizeof(double) * (2 * (size_t)n_slices + n_fft);
-    AdjArgs a = adj_args(*this);
+    AdjArgs a = adj_args(j);
     const int igrid = adj_iv_grid(n_slices, n_fft);
     int rc = launch_fill<double>(d_lam, (int64_t)pn, 0.0, s);
     if (!rc) rc = launch_fill<double>(d_corr, n_fft, 0.0, s);
@@ -666,7 +830,7 @@ int Adjoint::enqueue_backward(int T, hipStream_t s) {
     if (every == 1) {
         for (int t = T - 1; t >= 0; --t) {
             const double *st = d_ckpt + (size_t)t * pn;
-            hipLaunchKernelGGL(adj_reverse_kernel, dim3(grid), dim3(kAdjBlock), lds_rev, s, st,
+            hipLaunchKernelGGL(adj_reverse_kernel<MAP>, dim3(grid), dim3(kAdjBlock), lds_rev, s, st,
                                st + n, d_lam, d_lam + n, n, t, a);
             hipLaunchKernelGGL(adj_iv_kernel, dim3(igrid), dim3(kAdjSliceBlock), lds_one, s, a, t,
                                d_prof, d_g, (int)n_fft, scale, d_lp, d_corr);
@@ -679,18 +843,23 @@ int Adjoint::enqueue_backward(int T, hipStream_t s) {
         adj_copy(d_slots, d_ckpt + (size_t)seg * pn, (int64_t)pn, grid, s);
         for (int t = t0; t < t1 - 1; ++t) {
             double *si = d_slots + (size_t)(t - t0) * pn, *so = si + pn;
-            hipLaunchKernelGGL(adj_turn_kernel<false>, dim3(grid), dim3(kAdjBlock), lds_dep, s, si,
-                               si + n, so, so + n, n, t, a);
+            hipLaunchKernelGGL((adj_turn_kernel<false, MAP>), dim3(grid), dim3(kAdjBlock), lds_dep, s,
+                               si, si + n, so, so + n, n, t, a);
         }
         for (int t = t1 - 1; t >= t0; --t) {
             const double *st = d_slots + (size_t)(t - t0) * pn;
-            hipLaunchKernelGGL(adj_reverse_kernel, dim3(grid), dim3(kAdjBlock), lds_rev, s, st,
+            hipLaunchKernelGGL(adj_reverse_kernel<MAP>, dim3(grid), dim3(kAdjBlock), lds_rev, s, st,
                                st + n, d_lam, d_lam + n, n, t, a);
             hipLaunchKernelGGL(adj_iv_kernel, dim3(igrid), dim3(kAdjSliceBlock), lds_one, s, a, t,
                                d_prof, d_g, (int)n_fft, scale, d_lp, d_corr);
         }
     }
     return blmi_check_launch("adjoint_backward");
+}
+
+int Adjoint::enqueue_backward(int T, hipStream_t s) {
+    return map == kMapReference ? enqueue_backward_map<kMapReference>(*this, T, s)
+                                : enqueue_backward_map<kMapSpline>(*this, T, s);
 }
 
 // Capture `enqueue` on the own stream into *exec (once per turn count).

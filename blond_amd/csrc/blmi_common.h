@@ -195,6 +195,28 @@ template <> struct Trig<float> {
     __device__ __forceinline__ static float sin(float x) { return cephes_sinf(x); }
 };
 
+// linear_interp_kick.cpp:33-48: the interpolation table of a voltage on
+// the bin centres (acceleration kick 0), its scale and one bin's (slope,
+// intercept).  With f32 the reference's single-precision build (real_t =
+// float): the voltage and the bin centres are float arrays there, so V is
+// rounded to float and the scale and each entry are formed in float, in the
+// same order, and stored exactly as doubles.
+__device__ __forceinline__ double interp_inv_bw(const double *bc, int ns, bool f32) {
+    if (f32) return (double)((float)(ns - 1) / ((float)bc[ns - 1] - (float)bc[0]));
+    return (double)(ns - 1) / (bc[ns - 1] - bc[0]);
+}
+
+__device__ __forceinline__ double2 interp_entry(double v0, double v1, double bcb, double charge,
+                                                double inv_bw, bool f32) {
+    if (f32) {
+        const float a = (float)v0, b = (float)v1, q = (float)charge, ib = (float)inv_bw;
+        const float k = q * (b - a) * ib;
+        return make_double2((double)k, (double)((q * a - (float)bcb * k) + 0.0f));
+    }
+    const double k = charge * (v1 - v0) * inv_bw;
+    return make_double2(k, (charge * v0 - bcb * k) + 0.0);
+}
+
 // Agent-scope atomics (all XCDs of the device see one coherent value).
 __device__ __forceinline__ void atomic_add_agent(unsigned *p, unsigned v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

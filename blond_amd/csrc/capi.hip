@@ -600,6 +600,12 @@ int blmi_engine_track(void *h, int turns, int use_graph, void *stream) {
     return e->track(turns, S(stream), use_graph != 0);
 }
 
+int blmi_engine_set_interpolation(void *h, const double *bin_centers) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    return e->set_interpolation(bin_centers);
+}
+
 int blmi_engine_set_method(void *h, int method) {
     Engine *e = reinterpret_cast<Engine *>(h);
     BLMI_REQUIRE(e, "engine: null handle");
@@ -781,6 +787,12 @@ int blmi_adjoint_set_slices(void *h, double cut_left, double cut_right) {
     BLMI_REQUIRE(a, "adjoint: null handle");
     BLMI_REQUIRE(cut_right > cut_left, "adjoint: cut_right must exceed cut_left");
     return a->set_slices(cut_left, cut_right);
+}
+
+int blmi_adjoint_set_map(void *h, int map, const double *bin_centers) {
+    Adjoint *a = reinterpret_cast<Adjoint *>(h);
+    BLMI_REQUIRE(a, "adjoint: null handle");
+    return a->set_map(map, bin_centers);
 }
 
 int blmi_adjoint_set_wake(void *h, int64_t n_fft, const double *g, double scale) {

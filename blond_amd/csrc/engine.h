@@ -63,6 +63,7 @@ struct Engine {
     FftPlan plan{};
     Fft2Plan plan2{};
     double2 *d_Z = nullptr;
+    double2 *d_Zp = nullptr;       // four-step method: Z in its second pass's order [2][M]
     double *d_gl = nullptr;        // direct method: -scale*irfft(Z), unrolled
     int64_t gl_len = 0;            // entries of d_gl
     double scale = 0;
@@ -121,6 +122,13 @@ struct Engine {
     unsigned *d_digcnt = nullptr;  // [ndig][grid] column offsets, then [ndig] digit starts
     int64_t ord_n = 0;
     int ord_grid = 0, ord_ndig = 0, ord_cur = 0, ord_since = 0;
+    // interpolation=True (tracker.py:585-601): every turn one launch forms
+    // the table from the RF voltage on the bin centres (plus the induced
+    // voltage) with the acceleration kick in its intercepts; the particle
+    // kernel then kicks by the table only (no RF systems, no acceleration)
+    bool interp_rf = false;
+    const double *interp_bc = nullptr;  // borrowed: bin centres
+    double *d_zero_acc = nullptr;       // [n_turns + 1] zeros
     // launch shape (tunable)
     int block_size = 1024;
     int blocks_per_cu = 1;
@@ -148,6 +156,11 @@ struct Engine {
     int set_induced_f32(int64_t n_fft, const double *Z_host, double scale, float *volt,
                         const float *bc, double charge);
     int set_method(int method);
+    int set_interpolation(const double *bc);
+    int launch_interp_table(hipStream_t s, int off);
+    // one turn's launches in map order (all but the wake/feedback-free case
+    // of the caller's events)
+    int launch_turn_seq(hipStream_t s, int k, const KernelEvents &ei, const KernelEvents &et);
     int preferred_fft() const;
     int set_wake(int64_t n_mem, int64_t n_add, const double *time, double *memory,
                  const double *ind_coef_host, int deriv_mode);
@@ -189,6 +202,10 @@ struct Engine {
     int order_out(hipStream_t s);
     int launch_window_turn(hipStream_t s, int off, bool interp, const KernelEvents &ev);
     int ensure_partials();
+    // whether the u16-pair rows are stored transposed for the four-step
+    // transform's first pass, which then sums them itself (no reduction
+    // launch); shift = log2 M2 and lshift = log2 of the words per column
+    bool part_perm(int *shift, int *lshift) const;
     int launch_reduce(hipStream_t s, int off);
     int launch_induced(hipStream_t s, int off, const KernelEvents &ev);
     InducedArgs induced_args(int off) const;

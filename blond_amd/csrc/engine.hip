@@ -75,9 +75,12 @@ struct TurnArgs {
     // plain mode: the previous turn's counts are published by all workgroups
     double *profile;
     int *pending;                      // [2]: turn whose histogram counts[p] hold, or -1
-    // u16-pair histograms: workgroup b stores its LDS words at partials + b * part_words
+    // u16-pair histograms: workgroup b stores its LDS words at partials + b * part_words,
+    // or (part_shift < 31) padded in LDS and stored transposed for the
+    // four-step transform's first pass (store_partial_perm)
     unsigned *partials;
     int part_words;
+    int part_shift, part_lshift;
     // bunch moments of the coordinates after the drift: workgroup b writes
     // sum(dt - shift_t), sum((dt - shift_t)^2), sum(dE - shift_e),
     // sum((dE - shift_e)^2) of its particles to stats[(t + 1) * grid + b]
@@ -207,6 +210,7 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) a.pending[(t + 1) & 1] = t + 1;
     }
+    const RfProg<T> rfp(V, w, phi, a.n_rf);
     T bc0 = T(0), ibw = T(0);
     if (INTERP) {
         bc0 = (T)a.tab_params[0];
@@ -224,8 +228,10 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
     // INTERP == 2: the table is staged in LDS after the histogram words
     const int hwords = (HIST && LDSH) ? hist_words(LDSH, a.n_slices) : 0;
     double2 *ltab = reinterpret_cast<double2 *>(lds_hist + ((hwords + 3) & ~3));
+    // the padded layout adds one word per 2^part_shift (none when 31)
+    const int lwords = LDSH == 2 ? lds_word(hwords, a.part_shift) : hwords;
     if ((HIST && LDSH) || INTERP == 2) {
-        for (int b = threadIdx.x; b < hwords; b += blockDim.x) lds_hist[b] = 0u;
+        for (int b = threadIdx.x; b < lwords; b += blockDim.x) lds_hist[b] = 0u;
         if (INTERP == 2)
             for (int k = threadIdx.x; k < a.n_tab; k += blockDim.x) ltab[k] = a.tab[k];
         __syncthreads();
@@ -239,11 +245,11 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
             const T f = floor((x - bc0) * ibw);
             const bool in = f >= T(0) && f < (T)a.n_tab;
             const double2 tk = INTERP == 2 ? ltab[in ? (int)f : 0] : a.tab[in ? (int)f : 0];
-            const T rf = e_rf_kick(x, a.n_rf, a.charge, V, w, phi, acc);
+            const T rf = rfp.kick(x, a.charge, acc);
             if (in) e = e + (x * (T)tk.x + (T)tk.y);
             e = e + rf;
         } else {
-            e = e + e_rf_kick(x, a.n_rf, a.charge, V, w, phi, acc);
+            e = e + rfp.kick(x, a.charge, acc);
         }
         x = x + e_drift(dc, e, SOLVER);
         if (HIST) {
@@ -263,7 +269,7 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
     auto count = [&](int bin) {
         if (!HIST) return;
         if (LDSH)
-            lds_count<LDSH == 2>(lds_hist, bin, a.max_peel);
+            lds_count<LDSH == 2>(lds_hist, bin, a.max_peel, a.part_shift);
         else if (bin >= 0)
             atomic_add_agent(&counts[bin], 1u);
     };
@@ -317,7 +323,10 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
     if (a.stats) stats_flush(st, a.stats + ((size_t)(t + 1) * gridDim.x + blockIdx.x) * 4);
     if (HIST && LDSH) {
         __syncthreads();
-        if (LDSH == 2)
+        if (LDSH == 2 && a.part_shift < 31)
+            store_partial_perm(lds_hist, a.partials, blockIdx.x, gridDim.x, a.part_shift,
+                               a.part_lshift);
+        else if (LDSH == 2)
             store_partial(lds_hist, a.partials + (size_t)blockIdx.x * a.part_words, a.part_words);
         else
             hist_flush<LDSH>(lds_hist, counts, a.n_slices);
@@ -356,13 +365,15 @@ __global__ void __launch_bounds__(1024) prime_hist_kernel(const T *__restrict__ 
                                                           const int *base, unsigned *counts0,
                                                           unsigned *counts1, int n_slices,
                                                           T cut_left, T inv_bw,
-                                                          unsigned *partials, int part_words) {
+                                                          unsigned *partials, int part_words,
+                                                          int part_shift, int part_lshift) {
     using V2 = typename Vec2<T>::type;
     extern __shared__ unsigned lds_hist[];
     const int t = *base;
     unsigned *counts = (t & 1) ? counts1 : counts0;
     if (LDSH) {
-        const int words = hist_words(LDSH, n_slices);
+        const int hw = hist_words(LDSH, n_slices);
+        const int words = LDSH == 2 ? lds_word(hw, part_shift) : hw;
         for (int b = threadIdx.x; b < words; b += blockDim.x) lds_hist[b] = 0u;
         __syncthreads();
     }
@@ -372,7 +383,7 @@ __global__ void __launch_bounds__(1024) prime_hist_kernel(const T *__restrict__ 
     };
     auto count = [&](int bin) {
         if (LDSH)
-            lds_count<LDSH == 2>(lds_hist, bin, 0);
+            lds_count<LDSH == 2>(lds_hist, bin, 0, part_shift);
         else if (bin >= 0)
             atomic_add_agent(&counts[bin], 1u);
     };
@@ -387,7 +398,9 @@ __global__ void __launch_bounds__(1024) prime_hist_kernel(const T *__restrict__ 
         count(threadIdx.x == 0 ? bin_of(dt[n - 1]) : -1);
     if (LDSH) {
         __syncthreads();
-        if (LDSH == 2)
+        if (LDSH == 2 && part_shift < 31)
+            store_partial_perm(lds_hist, partials, blockIdx.x, gridDim.x, part_shift, part_lshift);
+        else if (LDSH == 2)
             store_partial(lds_hist, partials + (size_t)blockIdx.x * part_words, part_words);
         else
             hist_flush<LDSH>(lds_hist, counts, n_slices);
@@ -408,6 +421,8 @@ void Engine::release() {
     (void)hipFree(d_counts);
     (void)hipFree(d_table);
     (void)hipFree(d_Z);
+    (void)hipFree(d_Zp);
+    d_Zp = nullptr;
     (void)hipFree(d_gl);
     (void)hipFree(d_partials);
     d_partials = nullptr;
@@ -435,6 +450,9 @@ void Engine::release() {
     (void)hipFree(d_fb);
     d_fb = nullptr;
     fb_on = false;
+    (void)hipFree(d_zero_acc);
+    d_zero_acc = nullptr;
+    interp_rf = false;
     (void)hipFree(d_stats);
     (void)hipFree(d_moments);
     d_stats = d_moments = nullptr;
@@ -589,9 +607,10 @@ int Engine::set_induced(int64_t n_fft_, const double *Z_host, double scale_, dou
     charge = charge_;
     const int64_t nz = n_fft / 2 + 1;
     (void)hipFree(d_Z);
+    (void)hipFree(d_Zp);
     (void)hipFree(d_table);
     (void)hipFree(d_gl);
-    d_Z = nullptr;
+    d_Z = d_Zp = nullptr;
     d_table = d_gl = nullptr;
     gl_len = 0;
     have_fft = have_direct = have_fourstep = false;
@@ -622,6 +641,13 @@ int Engine::set_induced(int64_t n_fft_, const double *Z_host, double scale_, dou
     have_fft = fft_plan_get(n_fft, &plan) == BLMI_OK;
     fft2_plan_destroy(&plan2);
     have_fourstep = n_slices <= n_fft && fft2_plan_create(n_fft, &plan2) == BLMI_OK;
+    if (have_fourstep) {
+        std::vector<double2> zp(2 * (size_t)plan2.M);
+        fft2_permute_z(plan2, reinterpret_cast<const double2 *>(Z_host), zp.data());
+        BLMI_HIP_TRY(hipMalloc(&d_Zp, sizeof(double2) * zp.size()));
+        BLMI_HIP_TRY(hipMemcpy(d_Zp, zp.data(), sizeof(double2) * zp.size(),
+                               hipMemcpyHostToDevice));
+    }
     // the direct method's wake kernel g = -scale * irfft(Z) needs one
     // transform of length n_fft at set-up
     have_direct = false;
@@ -793,8 +819,31 @@ int Engine::ensure_partials() {
     return BLMI_OK;
 }
 
+bool Engine::part_perm(int *shift, int *lshift) const {
+    *shift = 31;
+    *lshift = 0;
+    if (!intensity || !have_conv || method != 2 || !have_fourstep || fb_on || ordered())
+        return false;
+    int grid = 0;
+    size_t lds = 0;
+    if (hist_layout(block_size, &grid, &lds) != 2) return false;
+    const int W = hist_words(2, n_slices);
+    const int M2 = plan2.M2;
+    if (M2 < 2 || (M2 & (M2 - 1)) || W % M2) return false;
+    const int L = W / M2;
+    if (L < 4 || (L & (L - 1)) || L > plan2.M1 || kColThreads % (plan2.cols * (L / 4)))
+        return false;
+    // the padded LDS rows must still fit
+    if ((size_t)(W + W / M2) * sizeof(unsigned) > (size_t)device_max_lds()) return false;
+    *shift = __builtin_ctz(M2);
+    *lshift = __builtin_ctz(L);
+    return true;
+}
+
 int Engine::launch_reduce(hipStream_t s, int off) {
     if (ordered()) return BLMI_OK;
+    int ps = 31, pl = 0;
+    if (part_perm(&ps, &pl)) return BLMI_OK;  // the four-step's first pass sums the rows
     int grid = 0;
     size_t lds = 0;
     if (hist_layout(block_size, &grid, &lds) != 2) return BLMI_OK;
@@ -808,6 +857,8 @@ int Engine::launch_reduce(hipStream_t s, int off) {
 template <typename T>
 static int launch_prime(Engine *e, const T *x, T cl, T ibw, int mode, int grid, size_t lds,
                         unsigned *c1, int W, hipStream_t s) {
+    int ps = 31, pl = 0;
+    if (mode == 2 && e->part_perm(&ps, &pl)) lds = sizeof(unsigned) * (size_t)lds_word(W, ps);
     const void *fn = mode == 2   ? (const void *)prime_hist_kernel<T, 2>
                      : mode == 1 ? (const void *)prime_hist_kernel<T, 1>
                                  : (const void *)prime_hist_kernel<T, 0>;
@@ -816,13 +867,16 @@ static int launch_prime(Engine *e, const T *x, T cl, T ibw, int mode, int grid, 
     const int blk = e->block_size;
     if (mode == 2)
         hipLaunchKernelGGL((prime_hist_kernel<T, 2>), dim3(grid), dim3(blk), lds, s, x, e->n,
-                           e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W);
+                           e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W,
+                           ps, pl);
     else if (mode == 1)
         hipLaunchKernelGGL((prime_hist_kernel<T, 1>), dim3(grid), dim3(blk), lds, s, x, e->n,
-                           e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W);
+                           e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W,
+                           ps, pl);
     else
         hipLaunchKernelGGL((prime_hist_kernel<T, 0>), dim3(grid), dim3(blk), 0, s, x, e->n,
-                           e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W);
+                           e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W,
+                           ps, pl);
     return BLMI_OK;
 }
 
@@ -885,6 +939,11 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
     a.charge = (T)e.charge;
     a.rf = rf;
     a.acc = acc;
+    if (e.interp_rf) {
+        // the table carries the RF voltage and the acceleration kick
+        a.n_rf = 0;
+        a.acc = reinterpret_cast<const T *>(e.d_zero_acc);
+    }
     a.drift = drift;
     a.base = e.d_base;
     a.off = off;
@@ -903,6 +962,7 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
     a.pending = e.d_base + 1;
     a.partials = e.d_partials;
     a.part_words = hist_words(2, e.n_slices);
+    const bool perm = e.part_perm(&a.part_shift, &a.part_lshift);
     a.stats = e.stats_on ? e.d_stats : nullptr;
     a.shift_t = e.stat_shift_t;
     a.shift_e = e.stat_shift_e;
@@ -913,6 +973,7 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
         ldsh = 0;
         lds = 0;
     }
+    if (ldsh == 2 && perm) lds = sizeof(unsigned) * (size_t)lds_word(a.part_words, a.part_shift);
     int imode = interp ? 1 : 0;
     // stage the interpolation table in LDS when it fits beside a u32 histogram
     const size_t tab_bytes = sizeof(double2) * (size_t)(e.n_slices > 1 ? e.n_slices - 1 : 0);
@@ -941,6 +1002,7 @@ InducedArgs Engine::induced_args(int off) const {
     ia.profile_out = profile;
     ia.n_slices = n_slices;
     ia.Z = reinterpret_cast<const double2 *>(d_Z);
+    ia.Zp = d_Zp;
     ia.scale = scale;
     ia.volt = volt;
     ia.n_out = n_slices;
@@ -948,6 +1010,17 @@ InducedArgs Engine::induced_args(int off) const {
     ia.charge = charge;
     ia.tab = reinterpret_cast<double2 *>(d_table);
     ia.tab_params = d_table + 2 * n_slices;
+    ia.f32 = f32 ? 1 : 0;
+    int ps = 31, pl = 0;
+    ia.part = nullptr;
+    if (part_perm(&ps, &pl)) {
+        int grid = 0;
+        size_t lds = 0;
+        (void)hist_layout(block_size, &grid, &lds);
+        ia.part = d_partials;
+        ia.part_rows = grid;
+        ia.part_lshift = pl;
+    }
     ia.gl = d_gl;
     ia.gl_len = gl_len;
     return ia;
@@ -1221,6 +1294,85 @@ int Engine::set_wake(int64_t n_mem_, int64_t n_add_, const double *time, double 
     return BLMI_OK;
 }
 
+// tracker.py:585-601, interpolation=True: the table of one turn from the
+// RF voltage on the bin centres (rf_volt_comp, kick.cpp:37-51, summed from
+// zero over the systems), plus the induced voltage when there is one, with
+// the acceleration kick in the intercepts (linear_interp_kick.cpp:30-48).
+__global__ void interp_rf_table_kernel(const double *__restrict__ rf, int n_rf,
+                                       const int *__restrict__ base, int off, int last_turn,
+                                       const double *__restrict__ acc,
+                                       const double *__restrict__ volt,
+                                       const double *__restrict__ bc, int ns, double charge,
+                                       double2 *__restrict__ tab, double *__restrict__ params) {
+    int t = *base + off;
+    if (t < 0) t = 0;
+    if (t > last_turn) t = last_turn;
+    const double *V = rf + (size_t)t * 3 * n_rf;
+    const double *w = V + n_rf;
+    const double *phi = w + n_rf;
+    const double a = acc[t];
+    const double ibw = (double)(ns - 1) / (bc[ns - 1] - bc[0]);
+    auto total = [&](int b) {
+        double r = 0.0;
+        for (int j = 0; j < n_rf; ++j) r = r + V[j] * cephes_sin(w[j] * bc[b] + phi[j]);
+        return volt ? r + volt[b] : r;
+    };
+    const int stride = gridDim.x * blockDim.x;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < ns - 1; b += stride) {
+        const double v0 = total(b), v1 = total(b + 1);
+        const double k = charge * (v1 - v0) * ibw;
+        tab[b] = make_double2(k, (charge * v0 - bc[b] * k) + a);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        params[0] = bc[0];
+        params[1] = ibw;
+    }
+}
+
+int Engine::set_interpolation(const double *bc_) {
+    if (f32) return blmi_set_error(BLMI_ERR_UNSUPPORTED, "interpolation: FP64 particles only");
+    if (n_slices < 2 || !bc_)
+        return blmi_set_error(BLMI_ERR_ARG, "interpolation: set the slices (>= 2) and bin centres");
+    if (ordered())
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "interpolation with the bin-ordered working set");
+    if (!d_table) {
+        BLMI_HIP_TRY(hipMalloc(&d_table, sizeof(double) * (2 * (size_t)n_slices + 2)));
+        BLMI_HIP_TRY(hipMemset(d_table, 0, sizeof(double) * (2 * (size_t)n_slices + 2)));
+    }
+    if (!d_zero_acc) {
+        BLMI_HIP_TRY(hipMalloc(&d_zero_acc, sizeof(double) * ((size_t)n_turns + 1)));
+        BLMI_HIP_TRY(hipMemset(d_zero_acc, 0, sizeof(double) * ((size_t)n_turns + 1)));
+    }
+    interp_bc = bc_;
+    interp_rf = true;
+    invalidate_graph();
+    return BLMI_OK;
+}
+
+int Engine::launch_interp_table(hipStream_t s, int off) {
+    const int threads = 256;
+    const int blocks = (n_slices - 1 + threads - 1) / threads;
+    hipLaunchKernelGGL(interp_rf_table_kernel, dim3(blocks), dim3(threads), 0, s, d_rf, n_rf,
+                       d_base, off, n_turns - 1, d_acc, intensity ? volt : nullptr, interp_bc,
+                       n_slices, charge, reinterpret_cast<double2 *>(d_table),
+                       d_table + 2 * n_slices);
+    return blmi_check_launch("interp_rf_table");
+}
+
+int Engine::launch_turn_seq(hipStream_t s, int k, const KernelEvents &ei, const KernelEvents &et) {
+    int rc = BLMI_OK;
+    if (intensity) rc = launch_induced(s, k, ei);
+    if (!rc && fb_on && !fb_in_induced()) rc = launch_feedback_turn(s, k);
+    if (!rc && interp_rf) rc = launch_interp_table(s, k);
+    if (!rc) rc = launch_turn_kernel(s, k, intensity || interp_rf, et);
+    if (!rc) rc = launch_reduce(s, k + 1);
+    if (!rc && ordered() && ++ord_since == sort_every) {
+        ord_since = 0;
+        rc = order_sort(s);
+    }
+    return rc;
+}
+
 int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const KernelEvents &ev) {
     if (ordered()) return launch_window_turn(s, off, interp, ev);
     if (f32)
@@ -1235,20 +1387,7 @@ int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const Kernel
 int Engine::launch_turns(hipStream_t s, int count, bool advance) {
     const KernelEvents none;
     for (int k = 0; k < count; ++k) {
-        if (intensity) {
-            int rc = launch_induced(s, k, none);
-            if (rc) return rc;
-        }
-        if (fb_on && !fb_in_induced()) {
-            int rc = launch_feedback_turn(s, k);
-            if (rc) return rc;
-        }
-        int rc = launch_turn_kernel(s, k, intensity, none);
-        if (!rc) rc = launch_reduce(s, k + 1);
-        if (!rc && ordered() && ++ord_since == sort_every) {
-            ord_since = 0;
-            rc = order_sort(s);
-        }
+        int rc = launch_turn_seq(s, k, none, none);
         if (rc) return rc;
     }
     if (advance) return finish_chunk(s, count);
@@ -1319,14 +1458,7 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
     int rc = BLMI_OK;
     for (int k = 0; k < turns && rc == BLMI_OK; ++k) {
         KernelEvents ei{ev[4 * k], ev[4 * k + 1]}, et{ev[4 * k + 2], ev[4 * k + 3]};
-        if (intensity) rc = launch_induced(own_stream, k, ei);
-        if (rc == BLMI_OK && fb_on && !fb_in_induced()) rc = launch_feedback_turn(own_stream, k);
-        if (rc == BLMI_OK) rc = launch_turn_kernel(own_stream, k, intensity, et);
-        if (rc == BLMI_OK) rc = launch_reduce(own_stream, k + 1);
-        if (rc == BLMI_OK && ordered() && ++ord_since == sort_every) {
-            ord_since = 0;
-            rc = order_sort(own_stream);
-        }
+        rc = launch_turn_seq(own_stream, k, ei, et);
     }
     if (rc == BLMI_OK) rc = finish_chunk(own_stream, turns);
     if (rc == BLMI_OK && ord) rc = order_out(own_stream);

@@ -76,6 +76,7 @@ __global__ void __launch_bounds__(1024) turn_window_kernel(double *__restrict__ 
     const double *V = a.rf + (size_t)t * 3 * a.n_rf;
     const double *w = V + a.n_rf;
     const double *phi = w + a.n_rf;
+    const RfProg<double> rfp(V, w, phi, a.n_rf);
     const double acc = a.acc[t];
     const DriftCoeffs<double> dc = a.drift[t + 1];
     const int ns = a.n_slices;
@@ -113,7 +114,7 @@ __global__ void __launch_bounds__(1024) turn_window_kernel(double *__restrict__ 
                 e = e + (x * tk.x + tk.y);
             }
         }
-        e = e + e_rf_kick(x, a.n_rf, a.charge, V, w, phi, acc);
+        e = e + rfp.kick(x, a.charge, acc);
         x = x + e_drift(dc, e, SOLVER);
         const double f = floor((x - a.cut_left) * a.inv_bw);
         return (f >= 0.0 && f < (double)ns) ? (int)f : -1;

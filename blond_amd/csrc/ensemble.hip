@@ -74,6 +74,7 @@ __global__ void __launch_bounds__(1024) ensemble_turn_kernel(double *__restrict_
         const double *V = a.rf + (size_t)b * a.rf_bunch_stride + (size_t)t * 3 * a.n_rf;
         const double *w = V + a.n_rf;
         const double *phi = w + a.n_rf;
+        const RfProg<double> rfp(V, w, phi, a.n_rf);
         const double2 *tab = nullptr;
         double bc0 = 0.0, ibw = 0.0;
         if (INTERP) {
@@ -100,7 +101,7 @@ __global__ void __launch_bounds__(1024) ensemble_turn_kernel(double *__restrict_
                     e = e + (x * tk.x + tk.y);
                 }
             }
-            e = e + e_rf_kick(x, a.n_rf, a.charge, V, w, phi, acc);
+            e = e + rfp.kick(x, a.charge, acc);
             x = x + e_drift(dc, e, SOLVER);
             if (HIST) {
                 const double f = floor((x - cl) * hbw);

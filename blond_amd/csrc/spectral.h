@@ -63,6 +63,15 @@ struct InducedArgs {
     // tab_params = (bin_centers[0], inv_bw)
     double2 *tab;
     double *tab_params;
+    int f32;          // table in the reference's single-precision arithmetic
+    // four-step method: the profile as the sum of part_rows u16-pair rows
+    // stored transposed (column m2's word m1 of row r at
+    // part[(m2 part_rows + r) 2^part_lshift + m1]); nullptr: the counts
+    const unsigned *part;
+    int part_rows, part_lshift;
+    // four-step method: Z in pass B's order, Zp[r M2 + k2] = Z[k] and
+    // Zp[M + r M2 + k2] = Z[M - k], k = r + M1 k2 (nullptr: read Z)
+    const double2 *Zp;
     const double *gl; // direct method: g[(k - (ns-1)) mod n_fft], g = -scale * irfft(Z)
     int64_t gl_len;   // entries of gl (at least n_out + n_slices - 1)
     // direct method: one extra workgroup runs this turn's beam phase loop
@@ -93,6 +102,9 @@ int launch_induced_direct(const InducedArgs &a, hipStream_t s,
 int launch_fast_resonator(double *re, double *im, const double *freq, const double *R,
                           const double *Q, const double *fr, int n_res, int n_f, hipStream_t s);
 
+// Threads of the four-step transform's column passes.
+constexpr int kColThreads = 512;
+
 // Multi-workgroup real-FFT convolution for n_fft beyond one workgroup's LDS:
 // complex length M = n_fft/2 = M1 * M2 as a four-step transform in three
 // launches -- (A) M1-point column transforms with twiddles, (B) per pair of
@@ -108,10 +120,17 @@ struct Fft2Plan {
     int r1[kMaxStages], r2[kMaxStages];
     double2 *twM = nullptr;       // exp(-2 pi i j / M), j < M (all sub-transforms)
     double2 *twN = nullptr;       // exp(-2 pi i k / 2M), k <= M (real split / merge)
+    // the same twiddles in the order the passes read them, so that every
+    // lane's load is contiguous: twA[m2 M1 + k1] = twM[m2 k1] (pass A's
+    // output twiddle), twB[3][r M2 + k2] = twN[k], twN[M - k], twM[r k2]
+    // with k = r + M1 k2 (pass B's split/merge and output twiddles)
+    double2 *twA = nullptr, *twB = nullptr;
     double2 *T = nullptr, *S = nullptr;  // M-complex intermediates
 };
 int fft2_plan_create(int64_t n_real, Fft2Plan *plan);
 void fft2_plan_destroy(Fft2Plan *plan);
+// Z[n_fft/2 + 1] (host) in pass B's order (InducedArgs::Zp), 2 M values.
+void fft2_permute_z(const Fft2Plan &p, const double2 *Z_host, double2 *Zp_host);
 // Same contract as launch_induced_voltage_lds (InducedArgs, including the
 // interpolation table).  `ev`, when given, spans the three
 // launches (start at pass A, stop at pass C).

@@ -347,15 +347,13 @@ __device__ void induced_lds_body(const InducedArgs &a, const unsigned *cin, unsi
     if (a.tab) {
         __syncthreads();
         // linear_interp_kick.cpp:33-48 on V (acc = 0 for the induced kick)
-        const double inv_bw = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
+        const double inv_bw = interp_inv_bw(a.bc, ns, a.f32);
         if (threadIdx.x == 0) {
             a.tab_params[0] = a.bc[0];
             a.tab_params[1] = inv_bw;
         }
-        for (int i = threadIdx.x; i < ns - 1; i += blockDim.x) {
-            const double k = a.charge * (vl[i + 1] - vl[i]) * inv_bw;
-            a.tab[i] = make_double2(k, (a.charge * vl[i] - a.bc[i] * k) + 0.0);
-        }
+        for (int i = threadIdx.x; i < ns - 1; i += blockDim.x)
+            a.tab[i] = interp_entry(vl[i], vl[i + 1], a.bc[i], a.charge, inv_bw, a.f32);
     }
 }
 
@@ -612,18 +610,16 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
     }
     if (a.tab) {
         // linear_interp_kick.cpp:33-48 on V (acc = 0 for the induced kick)
-        const double inv_bw = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
+        const double inv_bw = interp_inv_bw(a.bc, ns, a.f32);
         if (i0 == 0 && threadIdx.x == 0) {
             a.tab_params[0] = a.bc[0];
             a.tab_params[1] = inv_bw;
         }
         if (threadIdx.x < C) {
             const int i = i0 + threadIdx.x;
-            if (i < ns - 1) {
-                const double vi = vout[threadIdx.x], vi1 = vout[threadIdx.x + 1];
-                const double k = a.charge * (vi1 - vi) * inv_bw;
-                a.tab[i] = make_double2(k, (a.charge * vi - a.bc[i] * k) + 0.0);
-            }
+            if (i < ns - 1)
+                a.tab[i] = interp_entry(vout[threadIdx.x], vout[threadIdx.x + 1], a.bc[i],
+                                        a.charge, inv_bw, a.f32);
         }
     }
 }
@@ -888,7 +884,13 @@ int launch_fast_resonator(double *re, double *im, const double *freq, const doub
 // contiguous run (coalesced); LDS rows are padded by one element.
 // ---------------------------------------------------------------------------
 constexpr int kMaxCols = 8;
-constexpr int kColThreads = 512;
+
+// Workgroups are dispatched to the 8 XCDs round-robin; the column passes
+// give each XCD a contiguous range of columns, so that the columns sharing a
+// 128-byte line of T or S (adjacent m2) meet in one L2.
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    return (n % 8) ? b : (b % 8) * (n / 8) + b / 8;
+}
 
 __device__ __forceinline__ void count_buffers(const InducedArgs &a, const unsigned *&cin,
                                               unsigned *&cclr) {
@@ -904,22 +906,73 @@ __device__ __forceinline__ void count_buffers(const InducedArgs &a, const unsign
 __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int M1, int M2,
                                                              int cols, StageList st1,
                                                              const double2 *__restrict__ twM,
+                                                             const double2 *__restrict__ twA,
                                                              double2 *__restrict__ T) {
     extern __shared__ double2 buf[];
     const int ld = M1 + 1;
     double2 *twl = buf + cols * ld;  // exp(-2 pi i j / M1), j < M1
     const int ns = a.n_slices;
-    const int c0 = blockIdx.x * cols;
+    const int c0 = xcd_block(blockIdx.x, gridDim.x) * cols;
     const unsigned *cin;
     unsigned *cclr;
     count_buffers(a, cin, cclr);
     for (int j = threadIdx.x; j < M1; j += blockDim.x) twl[j] = twM[(size_t)j * M2];
+    // transposed partial rows: thread (g, q) sums words 4q'..4q'+3 (q' the
+    // quad q within the batch's columns) of rows g, g + G, ... with 16-byte
+    // loads (consecutive lanes, consecutive words); the G row groups are
+    // then summed per element below
+    unsigned *red = reinterpret_cast<unsigned *>(twl + M1);
+    const int L = a.part ? 1 << a.part_lshift : 0;
+    const int QC = cols * (L / 4);  // quads of the batch (host: QC divides blockDim)
+    if (a.part) {
+        const int G = blockDim.x / QC;
+        const int q = threadIdx.x % QC, g = threadIdx.x / QC;
+        const int c = q / (L / 4), w = 4 * (q % (L / 4));
+        unsigned lo[4] = {0u, 0u, 0u, 0u}, hi[4] = {0u, 0u, 0u, 0u};
+        const unsigned *src = a.part + (size_t)(c0 + c) * a.part_rows * L + w;
+        for (int r = g; r < a.part_rows; r += G) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(src + (size_t)r * L);
+            lo[0] += v.x & 0xffffu;
+            hi[0] += v.x >> 16;
+            lo[1] += v.y & 0xffffu;
+            hi[1] += v.y >> 16;
+            lo[2] += v.z & 0xffffu;
+            hi[2] += v.z >> 16;
+            lo[3] += v.w & 0xffffu;
+            hi[3] += v.w >> 16;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            red[(g * QC + q) * 8 + 2 * i] = lo[i];
+            red[(g * QC + q) * 8 + 2 * i + 1] = hi[i];
+        }
+        __syncthreads();
+    }
     for (int j = threadIdx.x; j < cols * M1; j += blockDim.x) {
         const int c = j % cols, m1 = j / cols;
         const int m = M2 * m1 + c0 + c;
         const int j0 = 2 * m, j1 = j0 + 1;
         double p[2] = {0.0, 0.0};
         const int jj[2] = {j0, j1};
+        if (a.part) {
+            // bins 2m (low halves) and 2m + 1 (high halves) of column c's word m1
+            if (m1 < L) {
+                const int qm = c * (L / 4) + m1 / 4, i = m1 % 4;
+                unsigned s0 = 0u, s1 = 0u;
+                for (int g = 0; g < (int)blockDim.x / QC; ++g) {
+                    s0 += red[(g * QC + qm) * 8 + 2 * i];
+                    s1 += red[(g * QC + qm) * 8 + 2 * i + 1];
+                }
+                p[0] = (double)s0;
+                p[1] = (double)s1;
+            }
+            if (a.profile_out) {
+                if (j0 < ns) a.profile_out[j0] = p[0];
+                if (j1 < ns) a.profile_out[j1] = p[1];
+            }
+            buf[c * ld + m1] = make_double2(p[0], p[1]);
+            continue;
+        }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (jj[h] >= ns) continue;
@@ -940,7 +993,7 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
     for (int j = threadIdx.x; j < cols * M1; j += blockDim.x) {
         const int c = j % cols, k1 = j / cols;
         const int m2 = c0 + c;
-        T[(size_t)k1 * M2 + m2] = cmul(buf[c * ld + k1], twM[(size_t)m2 * k1]);
+        T[(size_t)k1 * M2 + m2] = cmul(buf[c * ld + k1], twA[(size_t)m2 * M1 + k1]);
     }
 }
 
@@ -948,7 +1001,7 @@ constexpr int kPassBThreads = 512;
 
 __global__ void __launch_bounds__(kPassBThreads) fft2_b_kernel(InducedArgs a, int M1, int M2,
                                                                StageList st2,
-                              const double2 *__restrict__ twM, const double2 *__restrict__ twN,
+                              const double2 *__restrict__ twM, const double2 *__restrict__ twB,
                               const double2 *__restrict__ T, double2 *__restrict__ S) {
     extern __shared__ double2 buf[];
     double2 *twl = buf + 2 * M2;  // exp(-2 pi i j / M2), j < M2
@@ -967,12 +1020,18 @@ __global__ void __launch_bounds__(kPassBThreads) fft2_b_kernel(InducedArgs a, in
         if (j < ne) {
             const int row = j >= M2, k2 = j - row * M2;
             const int r = row ? r1 : r0;
-            const int k = r + M1 * k2;
-            zk[e] = a.Z[k];
-            zmk[e] = a.Z[M - k];
-            wk[e] = twN[k];
-            wmk[e] = twN[M - k];
-            wo[e] = twM[(size_t)r * k2];  // output twiddle of element (row, m2 = k2)
+            const size_t rk = (size_t)r * M2 + k2;  // the tables in this pass's order
+            if (a.Zp) {
+                zk[e] = a.Zp[rk];
+                zmk[e] = a.Zp[M + rk];
+            } else {
+                const int k = r + M1 * k2;
+                zk[e] = a.Z[k];
+                zmk[e] = a.Z[M - k];
+            }
+            wk[e] = twB[rk];
+            wmk[e] = twB[M + rk];
+            wo[e] = twB[2 * (size_t)M + rk];  // output twiddle of element (row, m2 = k2)
         }
     }
     for (int j = threadIdx.x; j < M2; j += blockDim.x) twl[j] = twM[(size_t)j * M1];
@@ -1040,7 +1099,7 @@ __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int 
     const int ld = M1 + 1;
     double2 *twl = buf + NC * ld;
     const int M = M1 * M2;
-    const int c0 = blockIdx.x * cols;
+    const int c0 = xcd_block(blockIdx.x, gridDim.x) * cols;
     const bool last = c0 + cols == M2;
     for (int j = threadIdx.x; j < M1; j += blockDim.x) twl[j] = twM[(size_t)j * M2];
     for (int j = threadIdx.x; j < NC * M1; j += blockDim.x) {
@@ -1070,15 +1129,14 @@ __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int 
         if (2 * m < a.n_out) a.volt[2 * m] = v0;
         if (2 * m + 1 < a.n_out) a.volt[2 * m + 1] = v1;
         if (a.tab && 2 * m < ns - 1) {
-            const double k0 = a.charge * (v1 - v0) * inv_bw;
-            a.tab[2 * m] = make_double2(k0, (a.charge * v0 - a.bc[2 * m] * k0) + 0.0);
+            a.tab[2 * m] = interp_entry(v0, v1, a.bc[2 * m], a.charge, inv_bw, a.f32);
             if (2 * m + 1 < ns - 1) {
                 // V[2m+2]: next column, same row (column 0, next row after the last)
                 const double2 un = c + 1 < cols ? buf[(c + 1) * ld + m1]
                                    : buf[cols * ld + (last ? m1 + 1 : m1)];
                 const double v2 = -(a.scale * (un.x * fct));
-                const double k1 = a.charge * (v2 - v1) * inv_bw;
-                a.tab[2 * m + 1] = make_double2(k1, (a.charge * v1 - a.bc[2 * m + 1] * k1) + 0.0);
+                a.tab[2 * m + 1] = interp_entry(v1, v2, a.bc[2 * m + 1], a.charge, inv_bw,
+                                                a.f32);
             }
         }
     }
@@ -1155,8 +1213,25 @@ int fft2_plan_create(int64_t n_real, Fft2Plan *plan) {
     std::vector<double2> twM(M), twN(M + 1);
     for (int j = 0; j < M; ++j) twM[j] = cis(j, M);
     for (int k = 0; k <= M; ++k) twN[k] = cis(k, 2.0L * M);
+    std::vector<double2> twA((size_t)M), twB(3 * (size_t)M);
+    for (int m2 = 0; m2 < M2; ++m2)
+        for (int k1 = 0; k1 < M1; ++k1) twA[(size_t)m2 * M1 + k1] = twM[(size_t)m2 * k1];
+    for (int r = 0; r < M1; ++r)
+        for (int k2 = 0; k2 < M2; ++k2) {
+            const size_t rk = (size_t)r * M2 + k2;
+            const int k = r + M1 * k2;
+            twB[rk] = twN[k];
+            twB[M + rk] = twN[M - k];
+            twB[2 * (size_t)M + rk] = twM[(size_t)r * k2];
+        }
     hipError_t e = hipMalloc(&p.twM, sizeof(double2) * M);
     if (e == hipSuccess) e = hipMalloc(&p.twN, sizeof(double2) * (M + 1));
+    if (e == hipSuccess) e = hipMalloc(&p.twA, sizeof(double2) * M);
+    if (e == hipSuccess) e = hipMalloc(&p.twB, sizeof(double2) * 3 * (size_t)M);
+    if (e == hipSuccess)
+        e = hipMemcpy(p.twA, twA.data(), sizeof(double2) * M, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p.twB, twB.data(), sizeof(double2) * 3 * (size_t)M, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&p.T, sizeof(double2) * M);
     if (e == hipSuccess) e = hipMalloc(&p.S, sizeof(double2) * M);
     if (e == hipSuccess)
@@ -1171,9 +1246,22 @@ int fft2_plan_create(int64_t n_real, Fft2Plan *plan) {
     return BLMI_OK;
 }
 
+void fft2_permute_z(const Fft2Plan &p, const double2 *Z, double2 *Zp) {
+    const size_t M = (size_t)p.M;
+    for (int r = 0; r < p.M1; ++r)
+        for (int k2 = 0; k2 < p.M2; ++k2) {
+            const size_t rk = (size_t)r * p.M2 + k2;
+            const size_t k = (size_t)r + (size_t)p.M1 * k2;
+            Zp[rk] = Z[k];
+            Zp[M + rk] = Z[M - k];
+        }
+}
+
 void fft2_plan_destroy(Fft2Plan *p) {
     (void)hipFree(p->twM);
     (void)hipFree(p->twN);
+    (void)hipFree(p->twA);
+    (void)hipFree(p->twB);
     (void)hipFree(p->T);
     (void)hipFree(p->S);
     *p = Fft2Plan();
@@ -1187,7 +1275,15 @@ int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t
                               a.n_out, p.n_real);
     if (a.counts0 && a.groups > kMaxCountGroups)
         return blmi_set_error(BLMI_ERR_ARG, "too many count groups");
-    const size_t lds_a = sizeof(double2) * (p.cols * (size_t)(p.M1 + 1) + p.M1);
+    if (a.part) {
+        const int L = 1 << a.part_lshift;
+        if (L < 4 || kColThreads % (p.cols * (L / 4)) || (int64_t)L * p.M2 * 2 < a.n_slices ||
+            L > p.M1)
+            return blmi_set_error(BLMI_ERR_ARG, "four-step: partial-row layout does not fit");
+    }
+    // the transposed partial rows add the row groups' sums (8 words a thread)
+    const size_t lds_a = sizeof(double2) * (p.cols * (size_t)(p.M1 + 1) + p.M1) +
+                         (a.part ? sizeof(unsigned) * 8 * kColThreads : 0);
     const size_t lds_c = sizeof(double2) * ((p.cols + 1) * (size_t)(p.M1 + 1) + p.M1);
     const size_t lds_b = sizeof(double2) * 3 * (size_t)p.M2;
     int rc = set_lds_limit((const void *)fft2_a_kernel, lds_a);
@@ -1198,9 +1294,9 @@ int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t
     const int cols = p.M2 / p.cols;
     const KernelEvents first{ev.start, nullptr}, last{nullptr, ev.stop};
     launch(fft2_a_kernel, dim3(cols), dim3(kColThreads), lds_a, s, first, a, p.M1, p.M2, p.cols,
-           st1, p.twM, p.T);
+           st1, p.twM, p.twA, p.T);
     hipLaunchKernelGGL(fft2_b_kernel, dim3(p.M1 / 2 + 1), dim3(p.threads_b), lds_b, s, a, p.M1,
-                       p.M2, st2, p.twM, p.twN, p.T, p.S);
+                       p.M2, st2, p.twM, p.twB, p.T, p.S);
     launch(fft2_c_kernel, dim3(cols), dim3(kColThreads), lds_c, s, last, a, p.M1, p.M2, p.cols,
            st1, p.twM, p.S);
     return blmi_check_launch("induced_fourstep");

@@ -19,6 +19,40 @@ __device__ __forceinline__ T e_rf_kick(T t, int n_rf, T charge, const T *__restr
     return charge * s + acc;
 }
 
+// The RF programs of one turn, held in registers for a whole launch: the
+// first kRfRegs systems are loaded once (the values are wave-uniform, so
+// they sit in scalar registers) and the loop over systems is unrolled under
+// a uniform guard, so no load or wait precedes a particle's sines; further
+// systems are read from the table.  The same sum in the same order as
+// e_rf_kick (kick.cpp:27-33).
+constexpr int kRfRegs = 4;
+template <typename T>
+struct RfProg {
+    T V[kRfRegs], w[kRfRegs], phi[kRfRegs];
+    const T *Vt, *wt, *pt;
+    int n;
+    __device__ __forceinline__ RfProg(const T *V_, const T *w_, const T *phi_, int n_)
+        : Vt(V_), wt(w_), pt(phi_), n(n_) {
+#pragma unroll
+        for (int j = 0; j < kRfRegs; ++j) {
+            V[j] = w[j] = phi[j] = T(0);
+            if (j < n) {
+                V[j] = V_[j];
+                w[j] = w_[j];
+                phi[j] = phi_[j];
+            }
+        }
+    }
+    __device__ __forceinline__ T kick(T t, T charge, T acc) const {
+        T s = T(0);
+#pragma unroll
+        for (int j = 0; j < kRfRegs; ++j)
+            if (j < n) s = s + V[j] * Trig<T>::sin(w[j] * t + phi[j]);
+        for (int j = kRfRegs; j < n; ++j) s = s + Vt[j] * Trig<T>::sin(wt[j] * t + pt[j]);
+        return charge * s + acc;
+    }
+};
+
 template <typename T>
 __device__ __forceinline__ T e_drift(const DriftCoeffs<T> &c, T e, int solver) {
     if (solver == 0) return c.coeff * e;
@@ -45,8 +79,14 @@ __device__ __forceinline__ T e_drift(const DriftCoeffs<T> &c, T e, int solver) {
 // bunch (few distinct bins per wavefront) thus costs a handful of atomics
 // instead of 64 conflicting ones; a wide one costs at most max_peel extra
 // scalar steps.  PACK16: bin b is the (b & 1) half of word b >> 1.
+// The LDS word of u16-pair word w: with pad_shift < 31 one pad word follows
+// every 2^pad_shift words (the transposed partial-row store then reads at an
+// odd stride, on distinct banks); pad_shift = 31 is the plain layout.
+__host__ __device__ __forceinline__ int lds_word(int w, int pad_shift) { return w + (w >> pad_shift); }
+
 template <bool PACK16>
-__device__ __forceinline__ void lds_count(unsigned *__restrict__ h, int bin, int max_peel) {
+__device__ __forceinline__ void lds_count(unsigned *__restrict__ h, int bin, int max_peel,
+                                          int pad_shift = 31) {
     unsigned long long pending = __ballot(bin >= 0);
     const int lane = (int)__lane_id();
     for (int it = 0; it < max_peel && pending != 0ull; ++it) {
@@ -56,7 +96,7 @@ __device__ __forceinline__ void lds_count(unsigned *__restrict__ h, int bin, int
         if (lane == leader) {
             const unsigned c = (unsigned)__popcll(same);
             if (PACK16)
-                atomicAdd(&h[lb >> 1], c << (16 * (lb & 1)));
+                atomicAdd(&h[lds_word(lb >> 1, pad_shift)], c << (16 * (lb & 1)));
             else
                 atomicAdd(&h[lb], c);
         }
@@ -64,9 +104,26 @@ __device__ __forceinline__ void lds_count(unsigned *__restrict__ h, int bin, int
     }
     if ((pending >> lane) & 1ull) {
         if (PACK16)
-            atomicAdd(&h[bin >> 1], 1u << (16 * (bin & 1)));
+            atomicAdd(&h[lds_word(bin >> 1, pad_shift)], 1u << (16 * (bin & 1)));
         else
             atomicAdd(&h[bin], 1u);
+    }
+}
+
+// A workgroup's u16-pair row stored transposed for the first pass of the
+// four-step transform (whose column m2 takes the words m = M2 m1 + m2):
+// word m of row `row` goes to P[(m2 rows + row) L + m1], M2 = 2^shift,
+// L = 2^lshift words per column, L * M2 words in all.  Lanes take
+// consecutive m1: the stores are contiguous, and the LDS reads (padded
+// layout, stride M2 + 1 words) fall on distinct banks.
+__device__ __forceinline__ void store_partial_perm(const unsigned *__restrict__ h,
+                                                   unsigned *__restrict__ P, int row, int rows,
+                                                   int shift, int lshift) {
+    const int L = 1 << lshift, W = L << shift;
+    for (int idx = threadIdx.x; idx < W; idx += blockDim.x) {
+        const int m2 = idx >> lshift, m1 = idx & (L - 1);
+        const int m = (m1 << shift) + m2;
+        P[((size_t)m2 * rows + row) * L + m1] = h[lds_word(m, shift)];
     }
 }
 

@@ -12,18 +12,27 @@ checkpoint, and a reverse pass over the particles.  With
 macroparticle per turn: 8 GB for 1e6 particles x 500 turns) and the reverse
 pass recomputes nothing.
 
-The map is the fused engine's, with one difference: the profile is
-deposited, and the induced voltage interpolated for the kick, with one
-quadratic B-spline on bin centres (weights (1/2-d)^2/2, 3/4-d^2, (1/2+d)^2/2
-on bins k-1, k, k+1), which has a continuous first derivative.  The
-reference's smooth_histogram (histogram.cpp:75-132) moves a particle's whole
-weight from bin k-1 to bin k+1 as it crosses the centre of bin k, so a loss
-built on it has no derivative there; linear weights (and the linear
-interpolation of linear_interp_kick.cpp:21-71) are continuous but their
-kinks make finite differences converge only at first order.  The induced voltage is -scale * (g circular-convolved with
-the profile), g = irfft(Z, n_fft) (impedance.py:457-489), Z from the
-library's resonator kernel (fast_resonator.cpp:45-71) divided by the bin
-size (impedance.py:881-890).
+Two maps (``map``):
+
+* ``"reference"`` (default): the reference's own smooth profile and
+  interpolated kick -- smooth_histogram (histogram.cpp:75-132, with its
+  negative-weight quirk) and linear_interp_kick (linear_interp_kick.cpp:21-71)
+  on the profile's bin centres.  Both are piecewise linear in dt: the
+  profile jumps when a particle crosses a bin centre or a bin edge and the
+  kick has kinks at bin centres.  The adjoint differentiates each piece (the
+  pathwise derivative of the loss), which is exact wherever no particle sits
+  on a centre or edge; central finite differences agree with it only while
+  no particle crosses one within the step.
+* ``"spline"``: the profile deposited, and the induced voltage interpolated
+  for the kick, with one quadratic B-spline on bin centres (weights
+  (1/2-d)^2/2, 3/4-d^2, (1/2+d)^2/2 on bins k-1, k, k+1), which has a
+  continuous first derivative, so finite differences converge to the
+  adjoint at second order.
+
+The induced voltage is -scale * (g circular-convolved with the profile),
+g = irfft(Z, n_fft) (impedance.py:457-489), Z from the library's resonator
+kernel (fast_resonator.cpp:45-71) divided by the bin size
+(impedance.py:881-890).
 """
 from __future__ import annotations
 
@@ -57,8 +66,12 @@ def resonator_derivatives(freq, R, fr, Q):
     return out
 
 
+MAPS = {"spline": 0, "reference": 1}
+
+
 class DifferentiableTracker:
-    def __init__(self, tracker, profile, n_fft: int | None = None, checkpoint_every: int = 50):
+    def __init__(self, tracker, profile, n_fft: int | None = None, checkpoint_every: int = 50,
+                 map: str = "reference"):
         import torch
 
         from ..toolbox.next_regular import next_regular
@@ -90,6 +103,14 @@ class DifferentiableTracker:
                   ct.c_void_p(dE.data_ptr()))
         _lib.call("blmi_adjoint_set_slices", self._h, float(profile.cut_left),
                   float(profile.cut_right))
+        if map not in MAPS:
+            raise ValueError(f"map {map!r} not in {tuple(MAPS)}")
+        self.map = map
+        bc = profile.bin_centers
+        bc = np.ascontiguousarray(bc.detach().cpu().numpy() if hasattr(bc, "detach") else bc,
+                                  dtype=np.float64)
+        self._bc = bc
+        _lib.call("blmi_adjoint_set_map", self._h, MAPS[map], _dptr(bc))
         self._ring = ring_program_arrays(rp)
         self._acc = np.zeros(rp.n_turns + 1)
         self._acc[: len(tracker.acceleration_kick)] = tracker.acceleration_kick

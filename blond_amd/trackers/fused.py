@@ -142,8 +142,6 @@ class FusedTracker:
                  method=None, sort_every=0, statistics=False, monitor=None):
         import torch
 
-        if tracker.interpolation:
-            raise NotImplementedError("interpolation=True is tracked object by object")
         self.tracker = tracker
         self.profile = profile
         self.tiv = total_induced_voltage
@@ -191,6 +189,10 @@ class FusedTracker:
             self._setup_induced()
         if self.fb is not None:
             self._setup_feedback()
+        if tracker.interpolation:
+            if sort_every:
+                raise NotImplementedError("interpolation=True with the bin-ordered working set")
+            self._setup_interpolation()
         _lib.call("blmi_engine_set_launch", self._h, int(block), int(blocks_per_cu),
                   int(graph_chunk))
         _lib.call("blmi_engine_set_hist", self._h, int(groups), int(max_peel))
@@ -244,7 +246,10 @@ class FusedTracker:
         self._phi_prog = self._phase_program()
         rf = rf_program_array(rp, self._phi_prog)
         acc = np.zeros(nt, dtype=np.float64)
-        acc[: len(self.tracker.acceleration_kick)] = self.tracker.acceleration_kick
+        if not rp.empty:
+            # tracker.py:585: without voltage the whole kick is skipped; the
+            # engine's kick of zero voltage and zero acceleration changes nothing
+            acc[: len(self.tracker.acceleration_kick)] = self.tracker.acceleration_kick
         arrs = ring_program_arrays(rp)
         self._keep = (rf, acc, arrs)
         _lib.call("blmi_engine_set_programs", self._h, _dptr(rf), _dptr(acc), _dptr(arrs[0]),
@@ -296,6 +301,25 @@ class FusedTracker:
         self._coef = coef
         _lib.call("blmi_engine_set_wake", self._h, n_mem, n_add, time_ptr, mem_ptr,
                   _dptr(coef) if coef is not None else None, int(mode))
+
+    def _setup_interpolation(self):
+        """tracker.py:585-601: the kick of every turn is the linear
+        interpolation of the RF voltage on the bin centres (plus the induced
+        voltage, which is then only summed, not kicked with separately)."""
+        torch = self._torch
+        if self.profile is None:
+            raise ValueError("interpolation=True needs the profile")
+        if self.f32:
+            raise NotImplementedError("interpolation=True needs FP64 particles")
+        if self.tiv is not None and self.tracker.totalInducedVoltage is not self.tiv:
+            raise ValueError("with interpolation=True the tracker's total_induced_voltage must "
+                             "be the map's induced voltage")
+        bc = self.profile.bin_centers
+        if not (hasattr(bc, "is_cuda") and bc.dtype == torch.float64 and bc.is_contiguous()):
+            bc = torch.as_tensor(np.asarray(_host_array(bc)), dtype=torch.float64,
+                                 device="cuda").contiguous()
+        self._interp_bc = bc
+        _lib.call("blmi_engine_set_interpolation", self._h, _ptr(bc))
 
     def _setup_feedback(self):
         fb, rp = self.fb, self.rf

@@ -78,8 +78,13 @@ class Workload:
     _fused: object = None
 
     def map(self):
-        """The per-turn map as a BLonD main script runs it."""
+        """The per-turn map as a BLonD main script runs it.  With an
+        interpolating tracker the induced voltage is only summed
+        (induced_voltage_sum) and the tracker kicks with it
+        (unittests/impedances/test_impedance.py:174-176)."""
         if self.intensity:
+            if self.tracker.interpolation:
+                return [self.profile, _InducedSum(self.tiv), self.tracker]
             return [self.profile, self.tiv, self.tracker]
         return [self.tracker] + ([self.profile] if self.profile is not None else [])
 
@@ -105,10 +110,12 @@ class Workload:
         st = {
             "name": self.name,
             "intensity": self.intensity,
+            "interpolation": bool(self.tracker.interpolation),
             "turn": int(rp.counter[0]),
             "dt": _h(self.beam.dt).copy(),
             "dE": _h(self.beam.dE).copy(),
             "n_rf": rp.n_rf,
+            "empty": bool(rp.empty),
             "charge": float(rp.particle.charge),
             "voltage": np.array(rp.host_array("voltage"), dtype=np.float64),
             "omega_rf": np.array(rp.host_array("omega_rf"), dtype=np.float64),
@@ -149,6 +156,16 @@ class Workload:
 
     def oracle_copy(self) -> dict:
         return self.snapshot()
+
+
+class _InducedSum:
+    """A map element that sums the induced voltage without kicking."""
+
+    def __init__(self, tiv):
+        self.tiv = tiv
+
+    def track(self):
+        self.tiv.induced_voltage_sum()
 
 
 def _feedback_state(fb, rp, profile) -> dict:
@@ -215,10 +232,14 @@ def _profile(beam, cut_left, cut_right, n_slices):
 
 def build_workload(name: str, n_particles: int | None = None, n_turns: int | None = None,
                    n_slices: int | None = None, n_fft: int | None = None, seed: int = 1,
-                   n_resonators: int = 20, to_gpu: bool = True) -> Workload:
+                   n_resonators: int = 20, to_gpu: bool = True,
+                   intensity: float | None = None, interpolation: bool = False) -> Workload:
     """Seeded synthetic workload ``c1``, ``c2`` or ``c3`` (see module doc).
-    Device residency (``to_gpu``) needs a GPU; with ``to_gpu=False`` only the
-    host setup runs."""
+    ``intensity`` replaces the workload's bunch intensity (c2, c3);
+    ``interpolation`` gives the tracker interpolation=True (tracker.py:585-601:
+    the RF voltage on the bin centres, plus the induced voltage, kicks by
+    linear interpolation).  Device residency (``to_gpu``) needs a GPU; with
+    ``to_gpu=False`` only the host setup runs."""
     if name == "c1":
         n_particles = n_particles or 1_000_000
         n_turns = n_turns or 1000
@@ -228,7 +249,7 @@ def build_workload(name: str, n_particles: int | None = None, n_turns: int | Non
         beam = Beam(ring, n_particles, 1e9)
         bigaussian(ring, rf, beam, sigma_dt=1.0e-9 / 4, seed=seed)
         prof = _profile(beam, 0.0, float(rf.t_rf[0, 0]), n_slices)
-        tr = RingAndRFTracker(rf, beam)
+        tr = RingAndRFTracker(rf, beam, interpolation=interpolation, profile=prof)
         w = Workload("c1", ring, rf, beam, tr, prof, None, False)
     elif name in ("c2", "c3"):
         if name == "c2":
@@ -238,7 +259,7 @@ def build_workload(name: str, n_particles: int | None = None, n_turns: int | Non
             n_fft = n_fft or 16384
             ring = Ring(157.08, 1 / 4.4 ** 2, 310e6, Proton(), n_turns)
             rf = RFStation(ring, [1], [8e3], [np.pi])
-            intensity = 1e13
+            intensity = intensity or 1e13
         else:
             n_particles = n_particles or 10_000_000
             n_turns = n_turns or 500
@@ -246,7 +267,7 @@ def build_workload(name: str, n_particles: int | None = None, n_turns: int | Non
             n_fft = n_fft or 131072
             ring = Ring(26658.883, 1 / 55.759505 ** 2, 450e9, Proton(), n_turns)
             rf = RFStation(ring, [35640, 142560], [6e6, 1.5e6], [0.0, 0.0], n_rf=2)
-            intensity = 1e11
+            intensity = intensity or 1e11
         beam = Beam(ring, n_particles, intensity)
         bigaussian(ring, rf, beam, sigma_dt=C2_SIGMA_DT if name == "c2" else 1.0e-9 / 4,
                    seed=seed)
@@ -257,7 +278,8 @@ def build_workload(name: str, n_particles: int | None = None, n_turns: int | Non
         f_res = (1 + 1e-9) / (n_fft * prof.bin_size)
         ivf = InducedVoltageFreq(beam, prof, [res], frequency_resolution=f_res)
         tiv = TotalInducedVoltage(beam, prof, [ivf])
-        tr = RingAndRFTracker(rf, beam)
+        tr = RingAndRFTracker(rf, beam, interpolation=interpolation, profile=prof,
+                              total_induced_voltage=tiv if interpolation else None)
         w = Workload(name, ring, rf, beam, tr, prof, tiv, True)
         w.meta["n_fft"] = ivf.n_fft
     else:

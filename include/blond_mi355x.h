@@ -294,6 +294,13 @@ int blmi_engine_set_launch(void *engine, int block, int blocks_per_cu, int graph
  * multi-turn memory with n_mem * n_slices > 2^26 moves 1 to the FFT. */
 int blmi_engine_set_method(void *engine, int method);
 int blmi_engine_get_method(void *engine, int *method);
+/* The tracker's interpolation=True (tracker.py:585-601: rf_voltage_calculation,
+ * then linear_interp_kick of the RF voltage on the bin centres plus the
+ * induced voltage, with the acceleration kick, in place of kick()).  Every
+ * turn one launch forms the table from bin_centers (device, n_slices f64,
+ * borrowed) and the turn's programs; the particle kernel kicks by the table
+ * alone.  Needs set_slices first; FP64 particles only. */
+int blmi_engine_set_interpolation(void *engine, const double *bin_centers);
 /* Histogram: `groups` copies of the global counts (1..16, or 0 = automatic:
  * 4 up to 1024 slices, else 1; workgroup b adds into copy b % groups, the
  * consumer sums them) and `max_peel` wavefront-aggregation
@@ -368,22 +375,24 @@ int blmi_ensemble_autotune(void *ensemble, int turns, int *block, int *wg_per_cu
                            float *ms_per_turn, void *stream);
 
 /* ---- differentiable tracking (adjoint) -------------------------------------
- * The intensity map of the fused engine -- profile, induced voltage, induced
- * kick, RF kick (kick.cpp:16-35), "simple" drift (drift.cpp:34-37) -- with
- * the profile deposited and the induced voltage interpolated by one
- * quadratic B-spline on bin centres (continuous first derivative; the
- * reference's smooth_histogram, histogram.cpp:75-132, jumps at bin centres,
- * and linear_interp_kick.cpp:21-71's slope jumps there) and V = -scale * (g circular-convolved with the
- * profile), g = irfft(Z, n_fft) (impedance.py:457-489).  forward() tracks
- * `turns` turns from the particles given to set_particles (copied), keeping
- * (dt, dE) every `checkpoint_every` turns (1: every state, and backward()
- * recomputes nothing; n_turns x 16 B per macroparticle of HBM); backward() takes dL/d(final
- * profile) and returns dL/dV_j, dL/dphi_j summed over turns
- * (grad_rf[2j], grad_rf[2j+1], host) and corr[m] = sum_t sum_i
- * dL/dV_t[i] p_t[(i - m) mod n_fft] (host), from which dL/dtheta of any
- * impedance parameter is -scale * sum_m irfft(dZ/dtheta)[m] corr[m].
- * n_rf 1..4, 2..1024 slices, n_fft <= 8192.  No reference counterpart: the
- * reference has no adjoint. */
+ * The intensity map of the fused engine with a smooth profile -- profile,
+ * induced voltage, induced kick, RF kick (kick.cpp:16-35), "simple" drift
+ * (drift.cpp:34-37) -- with V = -scale * (g circular-convolved with the
+ * profile), g = irfft(Z, n_fft) (impedance.py:457-489).  The profile and the
+ * induced kick are, by set_map: map 1, the reference's smooth_histogram
+ * (histogram.cpp:75-132, its quirk kept) and linear_interp_kick
+ * (linear_interp_kick.cpp:21-71) on bin_centers (host array of n_slices),
+ * differentiated on their piecewise-linear pieces; map 0 (the default), one
+ * quadratic B-spline on bin centres for both (continuous first derivative).
+ * forward() tracks `turns` turns from the particles given to set_particles
+ * (copied), keeping (dt, dE) every `checkpoint_every` turns (1: every state,
+ * and backward() recomputes nothing; n_turns x 16 B per macroparticle of
+ * HBM); backward() takes dL/d(final profile) and returns dL/dV_j, dL/dphi_j
+ * summed over turns (grad_rf[2j], grad_rf[2j+1], host) and corr[m] =
+ * sum_t sum_i dL/dV_t[i] p_t[(i - m) mod n_fft] (host), from which
+ * dL/dtheta of any impedance parameter is
+ * -scale * sum_m irfft(dZ/dtheta)[m] corr[m].  n_rf 1..4, 2..1024 slices,
+ * n_fft <= 8192.  No reference counterpart: the reference has no adjoint. */
 void *blmi_adjoint_create(int64_t n, int n_rf, int n_turns, int n_slices, int checkpoint_every);
 int blmi_adjoint_destroy(void *adjoint);
 int blmi_adjoint_set_particles(void *adjoint, const double *dt, const double *dE);
@@ -394,6 +403,7 @@ int blmi_adjoint_set_programs(void *adjoint, const double *rf_host, const double
                               const double *alpha1_host, const double *alpha2_host,
                               const double *beta_host, const double *energy_host, double charge);
 int blmi_adjoint_set_slices(void *adjoint, double cut_left, double cut_right);
+int blmi_adjoint_set_map(void *adjoint, int map, const double *bin_centers_host);
 int blmi_adjoint_set_wake(void *adjoint, int64_t n_fft, const double *g_host, double scale);
 int blmi_adjoint_forward(void *adjoint, int turns, double *profile_out, void *stream);
 int blmi_adjoint_backward(void *adjoint, const double *dprofile, double *grad_rf_host,

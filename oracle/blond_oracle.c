@@ -241,6 +241,7 @@ double beam_phase_fast(const double *bin_centers, const double *profile, double 
 void kick(const double *dt, double *dE, int n_rf, double charge,
           const double *voltage, const double *omega_rf, const double *phi_rf,
           int n, double acc_kick) {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
         const double t = dt[i];
         double acc = 0.0;
@@ -268,12 +269,14 @@ void drift(double *dt, const double *dE, int solver, double t_rev,
     const double T = t_rev * length_ratio;
     if (solver == 0) {
         const double coeff = T * eta_0 / (beta * beta * energy);
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) dt[i] = dt[i] + coeff * dE[i];
     } else if (solver == 1) {
         const double k = 1.0 / (beta * beta * energy);
         const double e0 = eta_0 * k;
         const double e1 = eta_1 * k * k;
         const double e2 = eta_2 * k * k * k;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) {
             const double d = dE[i];
             double den;
@@ -288,6 +291,7 @@ void drift(double *dt, const double *dE, int solver, double t_rev,
     } else {
         const double ib2 = 1 / (beta * beta);
         const double ie2 = 1 / (energy * energy);
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) {
             const double d = dE[i];
             const double delta = sqrt(1.0 + ib2 * (d * d * ie2 + 2.0 * d / energy)) - 1.0;
@@ -305,14 +309,24 @@ void histogram(const double *input, double *output, double cut_left,
                double cut_right, int n_slices, int n) {
     const double inv_bw = n_slices / (cut_right - cut_left);
     int *cnt = (int *)calloc((size_t)n_slices, sizeof(int));
-    for (int i = 0; i < n; ++i) {
-        float fb = (float)floor((input[i] - cut_left) * inv_bw);
-        /* the reference converts with (int); values outside int range are
-           out of the histogram either way */
-        if (!(fb >= 0.0f) || fb >= (float)n_slices) continue;
-        int b = (int)fb;
-        if (b < 0 || b >= n_slices) continue;
-        cnt[b] += 1;
+    /* integer counts: per-thread tables summed in any order give the same
+       result as the serial loop */
+#pragma omp parallel
+    {
+        int *mine = (int *)calloc((size_t)n_slices, sizeof(int));
+#pragma omp for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            float fb = (float)floor((input[i] - cut_left) * inv_bw);
+            /* the reference converts with (int); values outside int range are
+               out of the histogram either way */
+            if (!(fb >= 0.0f) || fb >= (float)n_slices) continue;
+            int b = (int)fb;
+            if (b < 0 || b >= n_slices) continue;
+            mine[b] += 1;
+        }
+#pragma omp critical
+        for (int b = 0; b < n_slices; ++b) cnt[b] += mine[b];
+        free(mine);
     }
     for (int b = 0; b < n_slices; ++b) output[b] = (double)cnt[b];
     free(cnt);
@@ -351,6 +365,7 @@ void linear_interp_kick(const double *dt, double *dE, const double *voltage,
         vk[i] = charge * (voltage[i + 1] - voltage[i]) * inv_bw;
         fc[i] = (charge * voltage[i] - bin_centers[i] * vk[i]) + acc_kick;
     }
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
         const double f = floor((dt[i] - bin_centers[0]) * inv_bw);
         /* the reference casts to unsigned and keeps k < n_slices-1: negative

@@ -378,16 +378,24 @@ def track(state: dict, n_turns: int, lib: Lib | None = None) -> dict:
         omega_p, phi_p = state["omega_rf"], state["phi_rf"]
     sources = state.get("sources")
     memories = [s.get("memory") for s in sources] if sources else None
+    # profile.py:700-719: slice_smooth for a smooth profile, slice_beam otherwise
+    slicer = lib.slice_smooth if state.get("smooth") else lib.slice_beam
+    interp = bool(state.get("interpolation"))
     for t in range(t0, t0 + n_turns):
         if state.get("intensity"):
-            lib.slice_beam(dt, prof, state["cut_left"], state["cut_right"])
+            slicer(dt, prof, state["cut_left"], state["cut_right"])
             if sources:
                 volt_ind = induced_voltage_sources(state, prof, t, memories)
             else:
                 volt_ind = induced_voltage_1turn(prof, state["Z"], state["n_fft"],
                                                  state["charge"], state["ratio"],
                                                  state["n_induced_voltage"], state["n_slices"])
-            lib.linear_interp_kick(dt, dE, volt_ind, state["bin_centers"], state["charge"], 0.0)
+            if not interp:
+                # TotalInducedVoltage.track (impedance.py:133-146); with an
+                # interpolating tracker the voltage is only summed
+                # (induced_voltage_sum) and the tracker kicks with it
+                lib.linear_interp_kick(dt, dE, volt_ind, state["bin_centers"], state["charge"],
+                                       0.0)
         if fb is not None:
             # RingAndRFTracker.track (tracker.py:551-565) before the kick
             domega = None
@@ -395,15 +403,28 @@ def track(state: dict, n_turns: int, lib: Lib | None = None) -> dict:
                 domega = phase_loop_turn(fb, t, omega_p, phi_p, prof, state["bin_centers"],
                                          fb["bin_size"], fb.get("lib") or lib, dE)
             phase_update(fb, t, omega_p, phi_p, domega)
-        lib.kick(dt, dE, state["voltage"][:, t], omega_p[:, t], phi_p[:, t],
-                 state["charge"], state["n_rf"], state["acc"][t])
+        if not state.get("empty", False) and interp:
+            # tracker.py:585-601: the RF voltage on the bin centres (plus the
+            # induced voltage) interpolated, the acceleration kick inside
+            # the table (particles outside it get no kick at all)
+            total = lib.rf_volt_comp(state["voltage"][:, t], omega_p[:, t], phi_p[:, t],
+                                     state["bin_centers"])
+            if state.get("intensity"):
+                total = total + volt_ind
+            lib.linear_interp_kick(dt, dE, total, state["bin_centers"], state["charge"],
+                                   state["acc"][t])
+        elif not state.get("empty", False):
+            # tracker.py:585: a station without voltage kicks nothing, the
+            # acceleration kick included
+            lib.kick(dt, dE, state["voltage"][:, t], omega_p[:, t], phi_p[:, t],
+                     state["charge"], state["n_rf"], state["acc"][t])
         k = t + 1
         lib.drift(dt, dE, state["solver"], state["t_rev"][k], state["length_ratio"],
                   state["alpha_order"], state["eta_0"][k], state["eta_1"][k], state["eta_2"][k],
                   state["alpha_0"][k], state["alpha_1"][k], state["alpha_2"][k],
                   state["beta"][k], state["energy"][k])
         if not state.get("intensity") and prof is not None:
-            lib.slice_beam(dt, prof, state["cut_left"], state["cut_right"])
+            slicer(dt, prof, state["cut_left"], state["cut_right"])
     st.update(dt=dt, dE=dE, turn=t0 + n_turns)
     if prof is not None:
         st["profile"] = prof

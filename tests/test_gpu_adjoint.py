@@ -1,9 +1,17 @@
-"""GPU parity of the differentiable tracker: the forward profile against
-the float64 CPU reference map, the hand-written adjoint against the CPU
-reference gradient and against central finite differences of the GPU
+"""GPU parity of the differentiable tracker, both maps: the forward profile
+against the float64 CPU reference map (oracle/adjoint.py, itself pinned
+against the oracle's turn loop), the hand-written adjoint against the CPU
+reference gradient, and against central finite differences of the GPU
 forward.
 
-Finite differences: over 100 turns, central differences with step 1e-6
+The reference map (smooth_histogram + linear_interp_kick, the c5
+configuration) is piecewise linear in the coordinates: its loss jumps when a
+particle crosses a bin centre or edge, so finite differences are checked on
+cases where the CPU oracle counts no crossing between the two runs (the
+count is asserted to be zero), with step 1e-6 relative; the adjoint is
+checked against the CPU reference gradient at full length.
+
+The spline map: over 100 turns, central differences with step 1e-6
 relative agree with the adjoint to 1e-6 relative.  Over 500 turns the
 filamented bunch makes the loss vary on a scale of ~1e-7 relative in V: on
 the CPU reference (float64, 1e5 particles) central differences of dL/dV
@@ -28,6 +36,7 @@ from blond_amd.trackers.differentiable import DifferentiableTracker  # noqa: E40
 from blond_amd.workloads import build_adjoint_case  # noqa: E402
 
 KEYS = ("V", "phi", "R", "fr", "Q")
+MAPS = ("reference", "spline")
 
 
 def params(th):
@@ -35,15 +44,21 @@ def params(th):
             for k in KEYS]
 
 
-def setup(n, turns, n_slices, every, seed=11):
-    c = build_adjoint_case(n_particles=n, n_turns=turns, n_slices=n_slices, seed=seed)
-    dtr = DifferentiableTracker(c.tracker, c.profile, checkpoint_every=every)
+def setup(n, turns, n_slices, every, seed=11, map="reference", intensity=1e9):
+    c = build_adjoint_case(n_particles=n, n_turns=turns, n_slices=n_slices, seed=seed,
+                           intensity=intensity)
+    dtr = DifferentiableTracker(c.tracker, c.profile, checkpoint_every=every, map=map)
     return c, dtr
 
 
-def test_forward_profile_matches_cpu_reference():
-    c, dtr = setup(20000, 40, 64, 16)
-    m = adj.DiffMap(c.state)
+def _cpu(c, dtr):
+    return adj.DiffMap(c.state, mode=dtr.map)
+
+
+@pytest.mark.parametrize("map", MAPS)
+def test_forward_profile_matches_cpu_reference(map):
+    c, dtr = setup(20000, 40, 64, 16, map=map, intensity=1e11)
+    m = _cpu(c, dtr)
     th = c.theta0
     p_gpu = dtr.profile_after(*params(th), turns=30).detach().cpu().numpy()
     p_cpu = m.forward(c.beam.dt, c.beam.dE, th["V"], th["phi"], th["R"], th["fr"], th["Q"], 30)
@@ -52,28 +67,40 @@ def test_forward_profile_matches_cpu_reference():
     dtr.close()
 
 
-def test_adjoint_matches_cpu_reference_gradient():
-    c, dtr = setup(20000, 40, 64, 16)
-    m = adj.DiffMap(c.state)
+def _vs_cpu_gradient(n, turns, n_slices, every, map, intensity=1e9, tol=1e-6):
+    c, dtr = setup(n, turns + 2, n_slices, every, map=map, intensity=intensity)
+    m = _cpu(c, dtr)
     th, ts = c.theta0, c.theta_star
-    target = m.forward(c.beam.dt, c.beam.dE, ts["V"], ts["phi"], ts["R"], ts["fr"], ts["Q"], 35)
+    target = m.forward(c.beam.dt, c.beam.dE, ts["V"], ts["phi"], ts["R"], ts["fr"], ts["Q"],
+                       turns)
     ps = params(th)
-    p = dtr.profile_after(*ps, turns=35)
-    loss = torch.sum((p - torch.from_numpy(target).cuda()) ** 2)
-    loss.backward()
-    ref = m.gradient(c.beam.dt, c.beam.dE, th["V"], th["phi"], th["R"], th["fr"], th["Q"], 35,
+    p = dtr.profile_after(*ps, turns=turns)
+    torch.sum((p - torch.from_numpy(target).cuda()) ** 2).backward()
+    ref = m.gradient(c.beam.dt, c.beam.dE, th["V"], th["phi"], th["R"], th["fr"], th["Q"], turns,
                      2 * (p.detach().cpu().numpy() - target))
     for k, t in zip(KEYS, ps):
-        got = t.grad.cpu().numpy()[0]
-        want = ref[k][0]
-        print(f"\n{k}: gpu {got:.12e} cpu {want:.12e} rel {abs(got - want) / abs(want):.2e}")
-        assert abs(got - want) <= 1e-6 * abs(want)
+        got, want = t.grad.cpu().numpy()[0], ref[k][0]
+        print(f"\n{map} {k}: gpu {got:.12e} cpu {want:.12e} rel {abs(got - want) / abs(want):.2e}")
+        assert abs(got - want) <= tol * abs(want)
     dtr.close()
 
 
-def _fd_check(n, turns, every, step, tol):
+@pytest.mark.parametrize("map", MAPS)
+def test_adjoint_matches_cpu_reference_gradient(map):
+    _vs_cpu_gradient(20000, 35, 64, 16, map, intensity=1e11)
+
+
+@pytest.mark.parametrize("map", MAPS)
+def test_adjoint_matches_cpu_reference_500_turns(map):
+    """The c5 length and slicing (500 turns, 256 slices, checkpoints every
+    50) at 2e4 macroparticles."""
+    _vs_cpu_gradient(20000, 500, 256, 50, map)
+
+
+def _fd_check(n, turns, every, step, tol, map, n_slices=256, seed=11, intensity=1e9,
+              count_crossings=False):
     """step: relative FD step, one value or a dict per parameter."""
-    c, dtr = setup(n, turns, 256, every)
+    c, dtr = setup(n, turns + 2, n_slices, every, seed=seed, map=map, intensity=intensity)
     th, ts = c.theta0, c.theta_star
     with torch.no_grad():
         target = dtr.profile_after(*params(ts), turns=turns).clone()
@@ -87,6 +114,7 @@ def _fd_check(n, turns, every, step, tol):
     p = dtr.profile_after(*ps, turns=turns)
     loss = torch.sum((p - target) ** 2)
     loss.backward()
+    m = _cpu(c, dtr) if count_crossings else None
     for k, t in zip(KEYS, ps):
         x0 = th[k][0]
         rel = step[k] if isinstance(step, dict) else step
@@ -96,40 +124,47 @@ def _fd_check(n, turns, every, step, tol):
         tm[k] = np.array([x0 - h])
         fd = (loss_at(tp) - loss_at(tm)) / (2 * h)
         got = t.grad.cpu().numpy()[0]
-        print(f"\n{k}: adjoint {got:.12e} fd {fd:.12e} rel {abs(got - fd) / abs(fd):.2e}")
+        msg = f"\n{map} {k}: adjoint {got:.12e} fd {fd:.12e} rel {abs(got - fd) / abs(fd):.2e}"
+        if m is not None:
+            cells = []
+            for tt in (tp, tm):
+                _, hist, _ = m.forward(c.beam.dt, c.beam.dE, tt["V"], tt["phi"], tt["R"],
+                                       tt["fr"], tt["Q"], turns, keep=True)
+                _, xT, _ = m.forward(c.beam.dt, c.beam.dE, tt["V"], tt["phi"], tt["R"],
+                                     tt["fr"], tt["Q"], turns, coords=True)
+                cells.append([m.cells(hh[0]) for hh in hist] + [m.cells(xT)])
+            crossings = sum(int(np.count_nonzero(a != b)) for a, b in zip(*cells))
+            msg += f" crossings {crossings}"
+            print(msg)
+            assert crossings == 0, (k, crossings)
+        else:
+            print(msg)
         assert abs(got - fd) <= tol * abs(fd)
     dtr.close()
 
 
-def test_adjoint_vs_central_differences_with_checkpoints():
+@pytest.mark.parametrize("seed", [11, 12])
+def test_reference_adjoint_vs_central_differences_without_crossings(seed):
+    """The reference map, step 1e-6 relative, 400 macroparticles x 30 turns,
+    64 slices, intensity 1e11: no crossing (counted on the CPU oracle), the
+    GPU adjoint within 1e-6 of the GPU forward's central differences."""
+    _fd_check(400, 30, 8, 1e-6, 1e-6, "reference", n_slices=64, seed=seed, intensity=1e11,
+              count_crossings=True)
+
+
+def test_spline_adjoint_vs_central_differences_with_checkpoints():
     """100 turns, checkpoints every 30 (segments of 30, 30, 30, 10), step 1e-6."""
-    _fd_check(100000, 100, 30, 1e-6, 1e-6)
+    _fd_check(100000, 100, 30, 1e-6, 1e-6, "spline")
 
 
-def test_adjoint_vs_central_differences_500_turns():
+def test_spline_adjoint_vs_central_differences_500_turns():
     """The c5 length: 500 turns, checkpoints every 50."""
-    _fd_check(100000, 500, 50, dict(V=1e-8, phi=1e-8, R=1e-6, fr=1e-6, Q=1e-6), 2e-6)
+    _fd_check(100000, 500, 50, dict(V=1e-8, phi=1e-8, R=1e-6, fr=1e-6, Q=1e-6), 2e-6, "spline")
 
 
-def test_adjoint_matches_cpu_reference_500_turns():
-    c, dtr = setup(20000, 500, 256, 50)
-    m = adj.DiffMap(c.state)
-    th, ts = c.theta0, c.theta_star
-    target = m.forward(c.beam.dt, c.beam.dE, ts["V"], ts["phi"], ts["R"], ts["fr"], ts["Q"], 500)
-    ps = params(th)
-    p = dtr.profile_after(*ps, turns=500)
-    torch.sum((p - torch.from_numpy(target).cuda()) ** 2).backward()
-    ref = m.gradient(c.beam.dt, c.beam.dE, th["V"], th["phi"], th["R"], th["fr"], th["Q"], 500,
-                     2 * (p.detach().cpu().numpy() - target))
-    for k, t in zip(KEYS, ps):
-        got, want = t.grad.cpu().numpy()[0], ref[k][0]
-        print(f"\n{k}: gpu {got:.12e} cpu {want:.12e} rel {abs(got - want) / abs(want):.2e}")
-        assert abs(got - want) <= 1e-6 * abs(want)
-    dtr.close()
-
-
-def test_backward_after_another_forward_recomputes():
-    c, dtr = setup(5000, 20, 32, 8)
+@pytest.mark.parametrize("map", MAPS)
+def test_backward_after_another_forward_recomputes(map):
+    c, dtr = setup(5000, 20, 32, 8, map=map)
     th, ts = c.theta0, c.theta_star
     p1 = dtr.profile_after(*(a := params(th)), turns=15)
     l1 = torch.sum(p1 ** 2)
@@ -144,8 +179,9 @@ def test_backward_after_another_forward_recomputes():
     dtr.close()
 
 
+@pytest.mark.parametrize("map", MAPS)
 @pytest.mark.parametrize("turns", [1, 23])
-def test_every_state_kept_equals_checkpoint_segments(turns):
+def test_every_state_kept_equals_checkpoint_segments(turns, map):
     """checkpoint_every=1 keeps every state and recomputes nothing; the
     profile is the same bits as with segments of 7 turns recomputed from
     their checkpoints, and the gradients agree to 1e-13 relative (the
@@ -153,7 +189,7 @@ def test_every_state_kept_equals_checkpoint_segments(turns):
     of a state differently: one gradient of 23 turns differs by 1 ulp)."""
     out = []
     for every in (1, 7):
-        c, dtr = setup(6001, 30, 48, every)
+        c, dtr = setup(6001, 30, 48, every, map=map)
         th, ts = c.theta0, c.theta_star
         with torch.no_grad():
             target = dtr.profile_after(*params(ts), turns=turns).clone()
@@ -165,3 +201,20 @@ def test_every_state_kept_equals_checkpoint_segments(turns):
     np.testing.assert_array_equal(out[0][0], out[1][0])
     for a, b in zip(out[0][1], out[1][1]):
         np.testing.assert_allclose(a, b, rtol=1e-13, atol=0)
+
+
+def test_reference_map_first_turn_profile_is_smooth_histogram():
+    """One turn from known coordinates: the GPU's final profile equals the
+    reference's smooth_histogram (the oracle's) of the CPU map's final
+    coordinates, within the fixed-point deposit's rounding."""
+    c, dtr = setup(50001, 3, 256, 1, map="reference", intensity=1e11)
+    m = _cpu(c, dtr)
+    th = c.theta0
+    p_gpu = dtr.profile_after(*params(th), turns=1).detach().cpu().numpy()
+    _, x, _ = m.forward(c.beam.dt, c.beam.dE, th["V"], th["phi"], th["R"], th["fr"], th["Q"], 1,
+                        coords=True)
+    prof = np.zeros(256)
+    m.lib.slice_smooth(x, prof, c.state["cut_left"], c.state["cut_right"])
+    assert (prof < 0).any()  # the reference's negative weights are there
+    np.testing.assert_allclose(p_gpu, prof, rtol=0, atol=1e-10)
+    dtr.close()

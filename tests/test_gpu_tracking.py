@@ -264,3 +264,95 @@ def test_ordered_intensity_matches_unordered():
     np.testing.assert_array_equal(host(a.beam.dt), host(b.beam.dt))
     np.testing.assert_array_equal(host(a.beam.dE), host(b.beam.dE))
     np.testing.assert_array_equal(host(a.tiv.induced_voltage), host(b.tiv.induced_voltage))
+
+
+# ---------------------------------------------------------------------------
+# interpolation=True (tracker.py:585-601)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("n_slices", [100, 65536])
+def test_interpolation_plain_turns_bit_exact(fused, n_slices):
+    """The RF voltage on the bin centres interpolated (rf_volt_comp +
+    linear_interp_kick with the acceleration kick), no intensity: the same
+    bits as the oracle's turns, fused (table in LDS / in L2) and object by
+    object."""
+    w = build_workload("c1", n_particles=100001, n_turns=30, n_slices=n_slices,
+                       interpolation=True)
+    snap = w.snapshot()
+    assert snap["interpolation"]
+    ref = orc.track(snap, 20, ORC_LIB)
+    if fused:
+        w.fused(20)
+    else:
+        for _ in range(20):
+            for m in w.map():
+                m.track()
+    np.testing.assert_array_equal(host(w.beam.dt), ref["dt"])
+    np.testing.assert_array_equal(host(w.beam.dE), ref["dE"])
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("n_slices,n_fft", [(4096, 16384), (65536, 131072)])
+def test_interpolation_intensity_turns(fused, n_slices, n_fft):
+    """interpolation=True with intensity effects: the induced voltage is
+    summed (not kicked with by itself) and added to the RF voltage on the
+    bin centres (the reference's pattern, unittests/impedances/
+    test_impedance.py:174-176).  First turn: profile bit-identical, voltage
+    within 1e-10 of its maximum, dE within 1e-12; after 20 turns the
+    statistical bars of the other intensity tests."""
+    name = "c2" if n_slices == 4096 else "c3"
+    w = build_workload(name, n_particles=60001, n_turns=30, n_slices=n_slices, n_fft=n_fft,
+                       intensity=1e11, interpolation=True)
+    snap = w.snapshot()
+    ref1 = orc.track(snap, 1, ORC_LIB)
+
+    def turns(k):
+        if fused:
+            w.fused(k)
+        else:
+            for _ in range(k):
+                for m in w.map():
+                    m.track()
+    turns(1)
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref1["profile"])
+    v = host(w.tiv.induced_voltage)
+    assert np.max(np.abs(v - ref1["induced_voltage"])) <= 1e-10 * np.abs(ref1["induced_voltage"]).max()
+    sc = np.abs(ref1["dE"]).max()
+    np.testing.assert_allclose(host(w.beam.dE), ref1["dE"], rtol=0, atol=1e-12 * sc)
+    turns(19)
+    _check_intensity(w, orc.track(snap, 20, ORC_LIB), 1e-9)
+
+
+@needs_ref32
+@pytest.mark.parametrize("name,n_slices,n_fft", [("c2", 4096, 16384), ("c3", 65536, 131072)])
+def test_intensity_turn_f32_bit_exact_given_the_voltage(name, n_slices, n_fft):
+    """FP32 mode, one intensity turn.  The reference's single build holds
+    the induced voltage as a float array and forms the interpolation table
+    in float (linear_interp_kick.cpp:33-48 with real_t = float); the engine
+    does the same from its own voltage.  Replaying the turn on the CPU with
+    the reference's single-precision kernels from the engine's published
+    (float) voltage gives the engine's coordinates bit for bit, and the
+    profile is the single build's histogram.  (The voltage itself differs:
+    numpy transforms the float profile in single precision, the engine in
+    double.)"""
+    w = build_workload(name, n_particles=100001, n_turns=4, n_slices=n_slices, n_fft=n_fft)
+    _to_f32(w)
+    snap = w.snapshot()
+    w.fused(1)
+    v32 = host(w.tiv.induced_voltage)
+    assert v32.dtype == np.float32
+    dt = np.array(snap["dt"], dtype=np.float32)
+    dE = np.array(snap["dE"], dtype=np.float32)
+    prof = np.zeros(n_slices, dtype=np.float32)
+    REF32.slice_beam(dt, prof, snap["cut_left"], snap["cut_right"])
+    REF32.linear_interp_kick(dt, dE, v32, snap["bin_centers"], snap["charge"], 0.0)
+    REF32.kick(dt, dE, snap["voltage"][:, 0], snap["omega_rf"][:, 0], snap["phi_rf"][:, 0],
+               snap["charge"], snap["n_rf"], snap["acc"][0])
+    REF32.drift(dt, dE, snap["solver"], snap["t_rev"][1], snap["length_ratio"],
+                snap["alpha_order"], snap["eta_0"][1], snap["eta_1"][1], snap["eta_2"][1],
+                snap["alpha_0"][1], snap["alpha_1"][1], snap["alpha_2"][1], snap["beta"][1],
+                snap["energy"][1])
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), prof)
+    np.testing.assert_array_equal(host(w.beam.dE), dE)
+    np.testing.assert_array_equal(host(w.beam.dt), dt)
