@@ -912,13 +912,26 @@ template <typename T>
 using TurnFn = void (*)(T *, T *, int64_t, TurnArgs<T>);
 
 template <typename T, int S>
-static TurnFn<T> pick_turn(int interp, bool hist, int ldsh) {
+static TurnFn<T> pick_turn(int interp, bool hist, int ldsh, bool publish) {
+    // publish: plain turns (no intensity effects), whose histogram is the
+    // caller's profile; with intensity effects the induced-voltage launch
+    // consumes the counts instead
     if (!hist)
         return interp == 2   ? turn_kernel<T, S, 2, false, 0, false>
                : interp == 1 ? turn_kernel<T, S, 1, false, 0, false>
                              : turn_kernel<T, S, 0, false, 0, false>;
-    if (interp == 2)
+    if (interp == 2) {
+        if (publish)
+            return ldsh == 1 ? turn_kernel<T, S, 2, true, 1, true> : turn_kernel<T, S, 2, true, 0, true>;
         return ldsh == 1 ? turn_kernel<T, S, 2, true, 1, false> : turn_kernel<T, S, 2, true, 0, false>;
+    }
+    if (interp == 1 && publish) {
+        switch (ldsh) {
+            case 0: return turn_kernel<T, S, 1, true, 0, true>;
+            case 1: return turn_kernel<T, S, 1, true, 1, true>;
+            default: return turn_kernel<T, S, 1, true, 2, true>;
+        }
+    }
     switch (interp * 3 + ldsh) {
         case 0: return turn_kernel<T, S, 0, true, 0, true>;
         case 1: return turn_kernel<T, S, 0, true, 1, true>;
@@ -982,9 +995,10 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
         imode = 2;
         lds = hist_bytes + tab_bytes;
     }
-    TurnFn<T> fn = e.solver == 0   ? pick_turn<T, 0>(imode, hist, ldsh)
-                   : e.solver == 1 ? pick_turn<T, 1>(imode, hist, ldsh)
-                                   : pick_turn<T, 2>(imode, hist, ldsh);
+    const bool publish = !e.intensity;
+    TurnFn<T> fn = e.solver == 0   ? pick_turn<T, 0>(imode, hist, ldsh, publish)
+                   : e.solver == 1 ? pick_turn<T, 1>(imode, hist, ldsh, publish)
+                                   : pick_turn<T, 2>(imode, hist, ldsh, publish);
     int rc = lds_attribute((const void *)fn, lds);
     if (rc) return rc;
     launch(fn, dim3(grid), dim3(e.block_size), lds, s, ev, x, de, e.n, a);
