@@ -36,6 +36,7 @@ SIGNATURES = {
     "blmi_smooth_histogram_f32": [_p, _p, _f, _f, _i, _i64, _p],
     "blmi_linear_interp_kick": [_p, _p, _p, _p, _d, _i, _i64, _d, _p],
     "blmi_linear_interp_kick_f32": [_p, _p, _p, _p, _f, _i, _i64, _f, _p],
+    "blmi_linear_interp_kick_noalloc": [_p, _p, _p, _p, _d, _i, _i64, _d, _p],
     "blmi_rf_volt_comp": [_p, _p, _p, _p, _i, _i, _p, _p],
     "blmi_rf_volt_comp_f32": [_p, _p, _p, _p, _i, _i, _p, _p],
     "blmi_fast_resonator": [_p, _p, _p, _p, _p, _p, _i, _i, _p],

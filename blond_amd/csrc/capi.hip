@@ -98,6 +98,40 @@ void *workspace(size_t bytes, int slot) {
     return w.ptr[slot];
 }
 
+// ---------------------------------------------------------------------------
+// Fixed interpolation table for blmi_linear_interp_kick_noalloc: the slopes,
+// intercepts and (bc0, inv_bw) of up to kNoAllocMaxSlices slices, for
+// callers that cannot allocate device memory.  Every device the library runs
+// on holds its own copy (1 MiB) until the process ends, at its own address:
+// host code uses the address hipGetSymbolAddress returns for the current
+// device (never the array's name, which is a host-side stand-in).  Calls on
+// one device share the table, so calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+static constexpr int kNoAllocMaxSlices = 65536;
+static constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxSlices + 2];
+static double *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Address of the current device's table, looked up once per device.
+static int scratch_table(double **out) {
+    int dev = 0;
+    BLMI_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "device %d beyond the fixed table's %d", dev,
+                              kNoAllocMaxDevices);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return blmi_set_error(BLMI_ERR_HIP, "fixed interpolation table: %s",
+                                  hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return BLMI_OK;
+}
+
 }  // namespace blmi
 
 using namespace blmi;
@@ -118,7 +152,8 @@ int blmi_version(void) { return 100; }
 int blmi_init(int device) {
     BLMI_HIP_TRY(hipSetDevice(device));
     (void)info();
-    return BLMI_OK;
+    double *tab = nullptr;
+    return scratch_table(&tab);
 }
 
 int blmi_device_info(int *cus, int *max_lds) {
@@ -266,6 +301,29 @@ int blmi_linear_interp_kick_f32(const float *dt, float *dE, const float *voltage
                                         par, S(stream));
     if (rc) return rc;
     return launch_interp_kick<float>(dt, dE, n, vk, fac, par, n_slices - 1, S(stream));
+}
+
+// No device allocation: the table lives in the fixed per-device array above
+// (at most kNoAllocMaxSlices slices); otherwise as blmi_linear_interp_kick.
+// Every table entry the kick reads is written by this call's table launch.
+int blmi_linear_interp_kick_noalloc(const double *dt, double *dE, const double *voltage,
+                                    const double *bin_centers, double charge, int n_slices,
+                                    int64_t n, double acc_kick, void *stream) {
+    BLMI_REQUIRE(n >= 0, "linear_interp_kick_noalloc: negative size");
+    BLMI_REQUIRE(n_slices >= 2, "linear_interp_kick_noalloc: needs at least 2 slices (got %d)",
+                 n_slices);
+    BLMI_REQUIRE(n_slices <= kNoAllocMaxSlices,
+                 "linear_interp_kick_noalloc: %d slices exceed the fixed table of %d", n_slices,
+                 kNoAllocMaxSlices);
+    BLMI_REQUIRE(voltage && bin_centers, "linear_interp_kick_noalloc: null voltage/bin_centers");
+    double *tab = nullptr;
+    int rc = scratch_table(&tab);
+    if (rc) return rc;
+    double *vk = tab, *fac = tab + n_slices, *par = tab + 2 * n_slices;
+    rc = launch_interp_table<double>(voltage, bin_centers, n_slices, charge, acc_kick, vk, fac,
+                                     par, S(stream));
+    if (rc) return rc;
+    return launch_interp_kick<double>(dt, dE, n, vk, fac, par, n_slices - 1, S(stream));
 }
 
 // --------------------------------------------------------- rf_volt_comp ----

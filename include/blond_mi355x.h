@@ -105,6 +105,17 @@ int blmi_linear_interp_kick(const double *dt, double *dE, const double *voltage,
 int blmi_linear_interp_kick_f32(const float *dt, float *dE, const float *voltage,
                                 const float *bin_centers, float charge, int n_slices, int64_t n,
                                 float acc_kick, void *stream);
+/* blmi_linear_interp_kick without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The slope/intercept table lives in a fixed per-device array of at most
+ * 65536 slices (1 MiB per device, held until the process ends); larger
+ * n_slices is refused with an error.  Every call on a device shares that
+ * array, so calls that overlap on one device race.  Same arguments and
+ * results as blmi_linear_interp_kick. */
+int blmi_linear_interp_kick_noalloc(const double *dt, double *dE, const double *voltage,
+                                    const double *bin_centers, double charge, int n_slices,
+                                    int64_t n, double acc_kick, void *stream);
 
 /* RF voltage on bin centres, accumulated into rf_voltage[n_bins].
  * Replaces rf_volt_comp(), kick.cpp:37-51 (butils_wrap_cpp.py:835-860). */
