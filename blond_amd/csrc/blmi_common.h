@@ -113,10 +113,12 @@ __device__ __forceinline__ double cephes_sin(double x) {
     double sp, cp;
     int j;
     cephes_reduce(x, sp, cp, j);
-    double out = (((j - 2) & 2) == 0) ? cp : sp;
-    if (j & 4) out = -out;
-    if (x < 0.0) out = -out;
-    return out;
+    const double out = (((j - 2) & 2) == 0) ? cp : sp;
+    // the reference's two negations (octant j & 4, then x < 0) as one flip
+    // of the sign bit; x < 0 (not the sign bit of x) keeps sin(-0.0) = +0.0
+    const unsigned flip = ((unsigned)j << 29) ^ (x < 0.0 ? 0x80000000u : 0u);
+    return __hiloint2double(__double2hiint(out) ^ (int)(flip & 0x80000000u),
+                            __double2loint(out));
 }
 
 // Both functions from one reduction (the adjoint needs the cosine).

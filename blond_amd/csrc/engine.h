@@ -18,6 +18,8 @@ namespace blmi {
 constexpr int kLdsHistBytes = 131072;    // histogram words in LDS at most
 constexpr int kLdsHistU32Bytes = 65536;  // u32 counts up to this, u16 pairs above
 constexpr int kMaxParticlesPerGroup16 = 65535;
+// the particle kernels' sink: 2 pairs (dt, dE) of doubles for each lane of a wavefront
+constexpr size_t kSinkBytes = 64 * 2 * 2 * sizeof(double);
 // LDS the particle kernel may use for its histogram plus a staged copy of the
 // interpolation table (random 16-byte gathers served by LDS instead of L2)
 constexpr int kLdsTableBudget = 144 * 1024;
@@ -42,6 +44,7 @@ struct Engine {
     double *d_rf = nullptr, *d_acc = nullptr;
     DriftCoeffs<double> *d_drift = nullptr;
     int *d_base = nullptr;         // [0] turn base, [1..2] pending flags of the count buffers
+    void *d_sink = nullptr;        // kSinkBytes: the particle kernels' store target for idle lanes
     int host_turn = 0;             // mirror of the turn base, kept by set_turn/track
     // slicing: counts double buffered by turn parity
     int n_slices = 0;

@@ -86,6 +86,10 @@ struct TurnArgs {
     // sum((dE - shift_e)^2) of its particles to stats[(t + 1) * grid + b]
     double *stats;
     double shift_t, shift_e;
+    // store target of the lanes that hold no pair in a wavefront's last
+    // iterations (every lane of a wavefront runs the same iteration count);
+    // at least 64 pairs
+    void *sink;
 };
 
 // Workgroup sums of the four moment accumulators in a fixed order (wavefront
@@ -183,6 +187,11 @@ __global__ void __launch_bounds__(512) partials_reduce_kernel(const unsigned *__
     }
 }
 
+// a pointer into global memory (address space 1): stores through it are
+// global_store, which unlike flat stores leave the LDS wait counter alone
+template <typename V>
+using GlobalPtr = __attribute__((address_space(1))) V *;
+
 template <typename T, int SOLVER, int INTERP, bool HIST, int LDSH, bool PUBLISH>
 __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__restrict__ dE,
                                                     int64_t n, TurnArgs<T> a) {
@@ -210,20 +219,36 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) a.pending[(t + 1) & 1] = t + 1;
     }
-    const RfProg<T> rfp(V, w, phi, a.n_rf);
+    RfProg<T> rfp(V, w, phi, a.n_rf);
     T bc0 = T(0), ibw = T(0);
     if (INTERP) {
         bc0 = (T)a.tab_params[0];
         ibw = (T)a.tab_params[1];
     }
-    // the first pair of this thread is loaded before the LDS set-up below, so
-    // its memory latency overlaps the clearing and the table staging
+    // Software pipeline over a thread's pairs i_first, i_first + stride, ...
+    // with two register slots used in turn (no register rotation, which
+    // would force a wait on data still in flight).  A slot holds a pair and
+    // its table entries; while one slot's pair is computed, the other slot's
+    // pair is already loaded and its table loads are in flight.  Every wait
+    // targets a load issued one pair's computation earlier: vector-memory
+    // waits complete in issue order, so a load waited for too soon would
+    // also wait for everything issued before it (the previous stores among
+    // them).  For the compiler to place those waits exactly, the loop's
+    // control is wavefront-uniform: every lane runs the iteration count of
+    // the wavefront's first lane, loads past a lane's last pair reload pair
+    // n2 - 1, and that lane's results go to the sink.  The first pair is
+    // loaded before the LDS set-up below, so its latency overlaps the
+    // clearing and the table staging.
     const int64_t i_first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n2 = n / 2;
-    V2 x_first{}, e_first{};
-    if (i_first < n2) {
-        x_first = reinterpret_cast<const V2 *>(dt)[i_first];
-        e_first = reinterpret_cast<const V2 *>(dE)[i_first];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t i_wave = __builtin_amdgcn_readfirstlane(i_first);
+    const int64_t iters = i_wave < n2 ? (n2 - 1 - i_wave) / stride + 1 : 0;
+    auto clamp = [&](int64_t j) { return j < n2 ? j : n2 - 1; };
+    V2 xa{}, ea{}, xb{}, eb{};
+    if (iters > 0) {
+        xa = reinterpret_cast<const V2 *>(dt)[clamp(i_first)];
+        ea = reinterpret_cast<const V2 *>(dE)[clamp(i_first)];
     }
     // INTERP == 2: the table is staged in LDS after the histogram words
     const int hwords = (HIST && LDSH) ? hist_words(LDSH, a.n_slices) : 0;
@@ -236,25 +261,47 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
             for (int k = threadIdx.x; k < a.n_tab; k += blockDim.x) ltab[k] = a.tab[k];
         __syncthreads();
     }
-    auto step = [&](T &x, T &e) -> int {
+    // linear_interp_kick.cpp:44-64 (acceleration kick 0): the table entry of
+    // x, loaded unconditionally (bin 0 stands in outside the table; `in`
+    // says whether the kick applies)
+    struct Tab {
+        double2 k;
+        bool in;
+    };
+    // the per-particle constants, moved into vector registers like the RF
+    // programs (scalar ones are spilled in this loop and read back with a
+    // v_readlane at every use)
+    T q_charge = a.charge, q_acc = acc, q_cl = a.cut_left, q_ibw = a.inv_bw;
+    T q_ntab = (T)a.n_tab, q_ns = (T)a.n_slices, q_bc0 = bc0, q_tibw = ibw;
+    auto pin_constants = [&]() {
+        asm volatile("" : "+v"(q_charge), "+v"(q_acc), "+v"(q_cl), "+v"(q_ibw));
+        if (INTERP) asm volatile("" : "+v"(q_ntab), "+v"(q_bc0), "+v"(q_tibw));
+        if (HIST) asm volatile("" : "+v"(q_ns));
+    };
+    auto gather = [&](T x) -> Tab {
+        Tab g{make_double2(0.0, 0.0), false};
         if (INTERP) {
-            // linear_interp_kick.cpp:44-64 (acceleration kick 0).  The table
-            // entry is loaded unconditionally (bin 0 stands in outside the
-            // table) before the RF kick is evaluated, so the gather's latency
-            // overlaps the sines; the sums keep the order e + interp + rf.
-            const T f = floor((x - bc0) * ibw);
-            const bool in = f >= T(0) && f < (T)a.n_tab;
-            const double2 tk = INTERP == 2 ? ltab[in ? (int)f : 0] : a.tab[in ? (int)f : 0];
-            const T rf = rfp.kick(x, a.charge, acc);
-            if (in) e = e + (x * (T)tk.x + (T)tk.y);
-            e = e + rf;
-        } else {
-            e = e + rfp.kick(x, a.charge, acc);
+            const T f = floor((x - q_bc0) * q_tibw);
+            g.in = f >= T(0) && f < q_ntab;
+            g.k = INTERP == 2 ? ltab[g.in ? (int)f : 0] : a.tab[g.in ? (int)f : 0];
         }
+        return g;
+    };
+    // the rest of the turn of one particle given its table entry; the sums
+    // keep the order e + interp + rf
+    auto finish = [&](T &x, T &e, const Tab &g, auto nrf) -> int {
+        constexpr int NRF = decltype(nrf)::value;
+        const T rf = rfp.template kick_n<NRF>(x, q_charge, q_acc);
+        if (INTERP) {
+            // a select, not a branch: nothing waits on the gather before the sines
+            const T ei = e + (x * (T)g.k.x + (T)g.k.y);
+            e = g.in ? ei : e;
+        }
+        e = e + rf;
         x = x + e_drift(dc, e, SOLVER);
         if (HIST) {
-            const T f = floor((x - a.cut_left) * a.inv_bw);
-            return (f >= T(0) && f < (T)a.n_slices) ? (int)f : -1;
+            const T f = floor((x - q_cl) * q_ibw);
+            return (f >= T(0) && f < q_ns) ? (int)f : -1;
         }
         return -1;
     };
@@ -273,39 +320,54 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
         else if (bin >= 0)
             atomic_add_agent(&counts[bin], 1u);
     };
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    // With the table in LDS (the bunch-scale slice counts) the loop is
-    // software-pipelined, the next pair loaded while the current one is
-    // computed (c2: 33.0 against 33.7 us a turn); with the table in L2 and
-    // u16-pair rows (65536 slices) the extra live registers cost more than
-    // they hide (c3: 70.1 against 68.7 us), so only the first pair is early.
-    constexpr bool kPipeline = INTERP == 2;
-    for (int64_t i = i_first; i < n2; i += stride) {
-        V2 x, e;
-        if (kPipeline) {
-            x = x_first;
-            e = e_first;
-            if (i + stride < n2) {
-                x_first = reinterpret_cast<const V2 *>(dt)[i + stride];
-                e_first = reinterpret_cast<const V2 *>(dE)[i + stride];
+    // one copy of the particle loop per RF-system count (see with_rf_count)
+    with_rf_count(a.n_rf, [&](auto nrf) {
+    rfp.template pin<decltype(nrf)::value>();
+    pin_constants();
+    // one pair: RF kick, interpolated kick, drift, store, count; then the
+    // other slot's table loads and this slot's reload two pairs ahead
+    V2 *const sink = reinterpret_cast<V2 *>(a.sink) + 2 * (threadIdx.x % kWave);
+    auto pair = [&](V2 &x, V2 &e, Tab &g0, Tab &g1, const V2 &xo, Tab &go0, Tab &go1,
+                    int64_t i) {
+        const bool own = i < n2;
+        int b0 = finish(x.x, e.x, g0, nrf);
+        int b1 = finish(x.y, e.y, g1, nrf);
+        // the target is chosen as an integer and hidden from the optimiser,
+        // which would otherwise split the store into two guarded stores:
+        // with a divergent branch around them the waits above lose count
+        uint64_t at = own ? (uint64_t)(reinterpret_cast<V2 *>(dt) + i) : (uint64_t)sink;
+        uint64_t ae = own ? (uint64_t)(reinterpret_cast<V2 *>(dE) + i) : (uint64_t)(sink + 1);
+        asm volatile("" : "+v"(at), "+v"(ae));
+        using NV2 = T __attribute__((ext_vector_type(2)));
+        *reinterpret_cast<GlobalPtr<NV2>>(at) = NV2{x.x, x.y};
+        *reinterpret_cast<GlobalPtr<NV2>>(ae) = NV2{e.x, e.y};
+        if (own) {
+            if (a.stats) {
+                moment(x.x, e.x);
+                moment(x.y, e.y);
             }
-        } else if (i == i_first) {
-            x = x_first;
-            e = e_first;
         } else {
-            x = reinterpret_cast<const V2 *>(dt)[i];
-            e = reinterpret_cast<const V2 *>(dE)[i];
+            b0 = b1 = -1;
         }
-        const int b0 = step(x.x, e.x);
-        const int b1 = step(x.y, e.y);
-        reinterpret_cast<V2 *>(dt)[i] = x;
-        reinterpret_cast<V2 *>(dE)[i] = e;
-        if (a.stats) {
-            moment(x.x, e.x);
-            moment(x.y, e.y);
-        }
+        go0 = gather(xo.x);
+        go1 = gather(xo.y);
+        const int64_t j = clamp(i + 2 * stride);
+        x = reinterpret_cast<const V2 *>(dt)[j];
+        e = reinterpret_cast<const V2 *>(dE)[j];
         count(b0);
         count(b1);
+    };
+    if (iters > 0) {
+        Tab ga0 = gather(xa.x), ga1 = gather(xa.y), gb0{}, gb1{};
+        xb = reinterpret_cast<const V2 *>(dt)[clamp(i_first + stride)];
+        eb = reinterpret_cast<const V2 *>(dE)[clamp(i_first + stride)];
+        // both slots every iteration, no exit between them (a branch there
+        // would again leave the waits conservative): with an odd count the
+        // last iteration's second pair is past every lane's end and sunk
+        for (int64_t k = 0; k < iters; k += 2) {
+            pair(xa, ea, ga0, ga1, xb, gb0, gb1, i_first + k * stride);
+            pair(xb, eb, gb0, gb1, xa, ga0, ga1, i_first + (k + 1) * stride);
+        }
     }
     if ((n & 1) && blockIdx.x == gridDim.x - 1 && threadIdx.x < 64) {
         // odd tail: the first wavefront of the last block takes part in the
@@ -313,13 +375,15 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
         int b = -1;
         if (threadIdx.x == 0) {
             T x = dt[n - 1], e = dE[n - 1];
-            b = step(x, e);
+            const Tab g = gather(x);
+            b = finish(x, e, g, nrf);
             dt[n - 1] = x;
             dE[n - 1] = e;
             if (a.stats) moment(x, e);
         }
         count(b);
     }
+    });
     if (a.stats) stats_flush(st, a.stats + ((size_t)(t + 1) * gridDim.x + blockIdx.x) * 4);
     if (HIST && LDSH) {
         __syncthreads();
@@ -418,6 +482,8 @@ void Engine::release() {
     (void)hipFree(d_acc);
     (void)hipFree(d_drift);
     (void)hipFree(d_base);
+    (void)hipFree(d_sink);
+    d_sink = nullptr;
     (void)hipFree(d_counts);
     (void)hipFree(d_table);
     (void)hipFree(d_Z);
@@ -489,6 +555,7 @@ int Engine::init(int64_t n_particles, int n_rf_, int n_turns_, int solver_) {
     BLMI_HIP_TRY(hipMalloc(&d_trev, sizeof(double) * (size_t)(n_turns + 1)));
     // d_base[0] = turn base, d_base[1..2] = turn held by each count buffer (-1: none)
     BLMI_HIP_TRY(hipMalloc(&d_base, sizeof(int) * 4));
+    BLMI_HIP_TRY(hipMalloc(&d_sink, kSinkBytes));
     BLMI_HIP_TRY(hipMemset(d_base, 0, sizeof(int)));
     BLMI_HIP_TRY(hipMemset(d_base + 1, 0xff, sizeof(int) * 2));
     BLMI_HIP_TRY(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
@@ -979,6 +1046,7 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
     a.stats = e.stats_on ? e.d_stats : nullptr;
     a.shift_t = e.stat_shift_t;
     a.shift_e = e.stat_shift_e;
+    a.sink = e.d_sink;
     int grid = 0;
     size_t lds = 0;
     int ldsh = e.hist_layout(e.block_size, &grid, &lds);

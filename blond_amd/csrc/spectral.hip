@@ -123,11 +123,10 @@ template <bool INV> struct Dft<5, INV> {
 // guarantees batch * L <= 15 * blockDim.x (at most 16/R butterflies a thread).
 template <int R, bool INV>
 __device__ __forceinline__ void stockham_stage(double2 *buf, int L, int batch, int ld, int Ns,
-                                               const double2 *__restrict__ tw, int tw_len) {
+                                               const double2 *__restrict__ tw, int tw_len, int T) {
     constexpr int BMAX = (16 / R) > 0 ? (16 / R) : 1;
     const int nb = L / R;
     const int total = nb * batch;
-    const int T = blockDim.x;
     const int tstride = tw_len / (Ns * R);
     double2 v[BMAX][R];
 #pragma unroll
@@ -174,16 +173,19 @@ struct StageList {
 };
 
 template <bool INV>
+// nthreads: the workgroup size.  The FFT passes give it as a constant: read
+// from blockDim.x inside the stage loop it is reloaded from the dispatch
+// packet with a vector load and a wait at every stage.
 __device__ void fft_lds_batch(double2 *buf, int L, int batch, int ld, const StageList &st,
-                              const double2 *__restrict__ tw, int tw_len) {
+                              const double2 *__restrict__ tw, int tw_len, int nthreads) {
     int Ns = 1;
     for (int s = 0; s < st.n; ++s) {
         switch (st.radix[s]) {
-            case 8: stockham_stage<8, INV>(buf, L, batch, ld, Ns, tw, tw_len); break;
-            case 4: stockham_stage<4, INV>(buf, L, batch, ld, Ns, tw, tw_len); break;
-            case 2: stockham_stage<2, INV>(buf, L, batch, ld, Ns, tw, tw_len); break;
-            case 5: stockham_stage<5, INV>(buf, L, batch, ld, Ns, tw, tw_len); break;
-            default: stockham_stage<3, INV>(buf, L, batch, ld, Ns, tw, tw_len); break;
+            case 8: stockham_stage<8, INV>(buf, L, batch, ld, Ns, tw, tw_len, nthreads); break;
+            case 4: stockham_stage<4, INV>(buf, L, batch, ld, Ns, tw, tw_len, nthreads); break;
+            case 2: stockham_stage<2, INV>(buf, L, batch, ld, Ns, tw, tw_len, nthreads); break;
+            case 5: stockham_stage<5, INV>(buf, L, batch, ld, Ns, tw, tw_len, nthreads); break;
+            default: stockham_stage<3, INV>(buf, L, batch, ld, Ns, tw, tw_len, nthreads); break;
         }
         Ns *= st.radix[s];
     }
@@ -191,7 +193,7 @@ __device__ void fft_lds_batch(double2 *buf, int L, int batch, int ld, const Stag
 
 template <bool INV>
 __device__ void fft_lds(double2 *buf, int M, const StageList &st, const double2 *__restrict__ tw) {
-    fft_lds_batch<INV>(buf, M, 1, M, st, tw, M);
+    fft_lds_batch<INV>(buf, M, 1, M, st, tw, M, (int)blockDim.x);
 }
 
 // X[k] and X[M-k] of the real transform from the half-length complex
@@ -916,7 +918,7 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
     const unsigned *cin;
     unsigned *cclr;
     count_buffers(a, cin, cclr);
-    for (int j = threadIdx.x; j < M1; j += blockDim.x) twl[j] = twM[(size_t)j * M2];
+    for (int j = threadIdx.x; j < M1; j += kColThreads) twl[j] = twM[(size_t)j * M2];
     // transposed partial rows: thread (g, q) sums words 4q'..4q'+3 (q' the
     // quad q within the batch's columns) of rows g, g + G, ... with 16-byte
     // loads (consecutive lanes, consecutive words); the G row groups are
@@ -925,13 +927,12 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
     const int L = a.part ? 1 << a.part_lshift : 0;
     const int QC = cols * (L / 4);  // quads of the batch (host: QC divides blockDim)
     if (a.part) {
-        const int G = blockDim.x / QC;
+        const int G = kColThreads / QC;
         const int q = threadIdx.x % QC, g = threadIdx.x / QC;
         const int c = q / (L / 4), w = 4 * (q % (L / 4));
         unsigned lo[4] = {0u, 0u, 0u, 0u}, hi[4] = {0u, 0u, 0u, 0u};
         const unsigned *src = a.part + (size_t)(c0 + c) * a.part_rows * L + w;
-        for (int r = g; r < a.part_rows; r += G) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(src + (size_t)r * L);
+        auto add = [&](const uint4 v) {
             lo[0] += v.x & 0xffffu;
             hi[0] += v.x >> 16;
             lo[1] += v.y & 0xffffu;
@@ -940,7 +941,21 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
             hi[2] += v.z >> 16;
             lo[3] += v.w & 0xffffu;
             hi[3] += v.w >> 16;
+        };
+        // kRowBatch rows' loads are issued before any is summed: a loop of
+        // one load and one wait per row leaves each thread with a single
+        // request in flight and the pass bound by memory latency
+        constexpr int kRowBatch = 16;
+        int r = g;
+        for (; r + (kRowBatch - 1) * G < a.part_rows; r += kRowBatch * G) {
+            uint4 v[kRowBatch];
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u)
+                v[u] = *reinterpret_cast<const uint4 *>(src + (size_t)(r + u * G) * L);
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u) add(v[u]);
         }
+        for (; r < a.part_rows; r += G) add(*reinterpret_cast<const uint4 *>(src + (size_t)r * L));
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             red[(g * QC + q) * 8 + 2 * i] = lo[i];
@@ -948,7 +963,7 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
         }
         __syncthreads();
     }
-    for (int j = threadIdx.x; j < cols * M1; j += blockDim.x) {
+    for (int j = threadIdx.x; j < cols * M1; j += kColThreads) {
         const int c = j % cols, m1 = j / cols;
         const int m = M2 * m1 + c0 + c;
         const int j0 = 2 * m, j1 = j0 + 1;
@@ -959,7 +974,7 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
             if (m1 < L) {
                 const int qm = c * (L / 4) + m1 / 4, i = m1 % 4;
                 unsigned s0 = 0u, s1 = 0u;
-                for (int g = 0; g < (int)blockDim.x / QC; ++g) {
+                for (int g = 0; g < (int)kColThreads / QC; ++g) {
                     s0 += red[(g * QC + qm) * 8 + 2 * i];
                     s1 += red[(g * QC + qm) * 8 + 2 * i + 1];
                 }
@@ -989,8 +1004,8 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
         buf[c * ld + m1] = make_double2(p[0], p[1]);
     }
     __syncthreads();
-    fft_lds_batch<false>(buf, M1, cols, ld, st1, twl, M1);
-    for (int j = threadIdx.x; j < cols * M1; j += blockDim.x) {
+    fft_lds_batch<false>(buf, M1, cols, ld, st1, twl, M1, kColThreads);
+    for (int j = threadIdx.x; j < cols * M1; j += kColThreads) {
         const int c = j % cols, k1 = j / cols;
         const int m2 = c0 + c;
         T[(size_t)k1 * M2 + m2] = cmul(buf[c * ld + k1], twA[(size_t)m2 * M1 + k1]);
@@ -1010,13 +1025,13 @@ __global__ void __launch_bounds__(kPassBThreads) fft2_b_kernel(InducedArgs a, in
     const int r0 = q, r1 = (M1 - q) % M1;
     const int nrow = r1 == r0 ? 1 : 2;
     const int ne = nrow * M2;
-    constexpr int EMAX = 4;  // host guarantees ne <= EMAX * blockDim.x
+    constexpr int EMAX = 4;  // host guarantees ne <= EMAX * kPassBThreads
     // everything the split/merge needs is requested up front, so its latency
     // overlaps the row loads and the forward transforms
     double2 zk[EMAX], zmk[EMAX], wk[EMAX], wmk[EMAX], wo[EMAX];
 #pragma unroll
     for (int e = 0; e < EMAX; ++e) {
-        const int j = threadIdx.x + e * blockDim.x;
+        const int j = threadIdx.x + e * kPassBThreads;
         if (j < ne) {
             const int row = j >= M2, k2 = j - row * M2;
             const int r = row ? r1 : r0;
@@ -1034,19 +1049,19 @@ __global__ void __launch_bounds__(kPassBThreads) fft2_b_kernel(InducedArgs a, in
             wo[e] = twB[2 * (size_t)M + rk];  // output twiddle of element (row, m2 = k2)
         }
     }
-    for (int j = threadIdx.x; j < M2; j += blockDim.x) twl[j] = twM[(size_t)j * M1];
-    for (int j = threadIdx.x; j < ne; j += blockDim.x) {
+    for (int j = threadIdx.x; j < M2; j += kPassBThreads) twl[j] = twM[(size_t)j * M1];
+    for (int j = threadIdx.x; j < ne; j += kPassBThreads) {
         const int row = j >= M2, k2 = j - row * M2;
         buf[j] = T[(size_t)(row ? r1 : r0) * M2 + k2];
     }
     __syncthreads();
-    fft_lds_batch<false>(buf, M2, nrow, M2, st2, twl, M2);
+    fft_lds_batch<false>(buf, M2, nrow, M2, st2, twl, M2, kPassBThreads);
     // buf[row][k2] = Zc[r + M1 k2]; the mirror bin M - k sits at
     // row 0, (M2 - k2) % M2 for r = 0 and at row of M1 - r, M2 - 1 - k2 otherwise
     double2 u[EMAX];
 #pragma unroll
     for (int e = 0; e < EMAX; ++e) {
-        const int j = threadIdx.x + e * blockDim.x;
+        const int j = threadIdx.x + e * kPassBThreads;
         if (j < ne) {
             const int row = j >= M2, k2 = j - row * M2;
             const int r = row ? r1 : r0;
@@ -1072,14 +1087,14 @@ __global__ void __launch_bounds__(kPassBThreads) fft2_b_kernel(InducedArgs a, in
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < EMAX; ++e) {
-        const int j = threadIdx.x + e * blockDim.x;
+        const int j = threadIdx.x + e * kPassBThreads;
         if (j < ne) buf[j] = u[e];
     }
     __syncthreads();
-    fft_lds_batch<true>(buf, M2, nrow, M2, st2, twl, M2);
+    fft_lds_batch<true>(buf, M2, nrow, M2, st2, twl, M2, kPassBThreads);
 #pragma unroll
     for (int e = 0; e < EMAX; ++e) {
-        const int j = threadIdx.x + e * blockDim.x;
+        const int j = threadIdx.x + e * kPassBThreads;
         if (j < ne) {
             const int row = j >= M2, m2 = j - row * M2;
             const int r = row ? r1 : r0;
@@ -1101,14 +1116,14 @@ __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int 
     const int M = M1 * M2;
     const int c0 = xcd_block(blockIdx.x, gridDim.x) * cols;
     const bool last = c0 + cols == M2;
-    for (int j = threadIdx.x; j < M1; j += blockDim.x) twl[j] = twM[(size_t)j * M2];
-    for (int j = threadIdx.x; j < NC * M1; j += blockDim.x) {
+    for (int j = threadIdx.x; j < M1; j += kColThreads) twl[j] = twM[(size_t)j * M2];
+    for (int j = threadIdx.x; j < NC * M1; j += kColThreads) {
         const int c = j % NC, k1 = j / NC;
         const int col = (c0 + c) % M2;
         buf[c * ld + k1] = S[(size_t)k1 * M2 + col];
     }
     __syncthreads();
-    fft_lds_batch<true>(buf, M1, NC, ld, st1, twl, M1);
+    fft_lds_batch<true>(buf, M1, NC, ld, st1, twl, M1, kColThreads);
     // V = -(charge*e*ratio * irfft(...))[:n_out]   (impedance.py:475-489)
     const double fct = 1.0 / (double)M;
     const int ns = a.n_slices;
@@ -1121,7 +1136,7 @@ __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int 
             a.tab_params[1] = inv_bw;
         }
     }
-    for (int j = threadIdx.x; j < cols * M1; j += blockDim.x) {
+    for (int j = threadIdx.x; j < cols * M1; j += kColThreads) {
         const int c = j % cols, m1 = j / cols;
         const int m = M2 * m1 + c0 + c;
         const double2 u = buf[c * ld + m1];
@@ -1205,6 +1220,8 @@ int fft2_plan_create(int64_t n_real, Fft2Plan *plan) {
     // at least 512 threads (c3's 2^17 transform: passes at 512 threads take
     // the three launches from 33.1 to 31.6 us, with kColThreads = 512)
     p.threads_b = tb < 512 ? 512 : tb;  // <= kPassBThreads for M2 <= 1024
+    if (p.threads_b != kPassBThreads)
+        return blmi_set_error(BLMI_ERR_UNSUPPORTED, "four-step: pass B needs %d threads", p.threads_b);
     auto cis = [](long double num, long double den) {
         const long double two_pi = 6.283185307179586476925286766559L;
         long double x = two_pi * num / den;
@@ -1275,14 +1292,15 @@ int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t
                               a.n_out, p.n_real);
     if (a.counts0 && a.groups > kMaxCountGroups)
         return blmi_set_error(BLMI_ERR_ARG, "too many count groups");
+    const int cols_a = p.cols;
     if (a.part) {
         const int L = 1 << a.part_lshift;
-        if (L < 4 || kColThreads % (p.cols * (L / 4)) || (int64_t)L * p.M2 * 2 < a.n_slices ||
+        if (L < 4 || kColThreads % (cols_a * (L / 4)) || (int64_t)L * p.M2 * 2 < a.n_slices ||
             L > p.M1)
             return blmi_set_error(BLMI_ERR_ARG, "four-step: partial-row layout does not fit");
     }
     // the transposed partial rows add the row groups' sums (8 words a thread)
-    const size_t lds_a = sizeof(double2) * (p.cols * (size_t)(p.M1 + 1) + p.M1) +
+    const size_t lds_a = sizeof(double2) * (cols_a * (size_t)(p.M1 + 1) + p.M1) +
                          (a.part ? sizeof(unsigned) * 8 * kColThreads : 0);
     const size_t lds_c = sizeof(double2) * ((p.cols + 1) * (size_t)(p.M1 + 1) + p.M1);
     const size_t lds_b = sizeof(double2) * 3 * (size_t)p.M2;
@@ -1293,8 +1311,8 @@ int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t
     const StageList st1 = stage_list(p.r1, p.n1), st2 = stage_list(p.r2, p.n2);
     const int cols = p.M2 / p.cols;
     const KernelEvents first{ev.start, nullptr}, last{nullptr, ev.stop};
-    launch(fft2_a_kernel, dim3(cols), dim3(kColThreads), lds_a, s, first, a, p.M1, p.M2, p.cols,
-           st1, p.twM, p.twA, p.T);
+    launch(fft2_a_kernel, dim3(p.M2 / cols_a), dim3(kColThreads), lds_a, s, first, a, p.M1, p.M2,
+           cols_a, st1, p.twM, p.twA, p.T);
     hipLaunchKernelGGL(fft2_b_kernel, dim3(p.M1 / 2 + 1), dim3(p.threads_b), lds_b, s, a, p.M1,
                        p.M2, st2, p.twM, p.twB, p.T, p.S);
     launch(fft2_c_kernel, dim3(cols), dim3(kColThreads), lds_c, s, last, a, p.M1, p.M2, p.cols,

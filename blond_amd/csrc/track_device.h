@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "blmi_common.h"
 #include "particles.h"
 
@@ -20,11 +22,11 @@ __device__ __forceinline__ T e_rf_kick(T t, int n_rf, T charge, const T *__restr
 }
 
 // The RF programs of one turn, held in registers for a whole launch: the
-// first kRfRegs systems are loaded once (the values are wave-uniform, so
-// they sit in scalar registers) and the loop over systems is unrolled under
-// a uniform guard, so no load or wait precedes a particle's sines; further
-// systems are read from the table.  The same sum in the same order as
-// e_rf_kick (kick.cpp:27-33).
+// first kRfRegs systems are loaded once and the loop over systems is
+// unrolled (kick_n: for a count known at compile time, no guard at all), so
+// no load or wait precedes a particle's sines; further systems are read
+// from the table.  The same sum in the same order as e_rf_kick
+// (kick.cpp:27-33).
 constexpr int kRfRegs = 4;
 template <typename T>
 struct RfProg {
@@ -51,7 +53,48 @@ struct RfProg {
         for (int j = kRfRegs; j < n; ++j) s = s + Vt[j] * Trig<T>::sin(wt[j] * t + pt[j]);
         return charge * s + acc;
     }
+    // Moves the first NRF systems into vector registers for the loop that
+    // follows: they are uniform, but the scalar file is taken by the sine's
+    // constants and the loop state, and spilled scalars cost a v_readlane
+    // per use inside the particle loop.
+    template <int NRF>
+    __device__ __forceinline__ void pin() {
+#pragma unroll
+        for (int j = 0; j < (NRF < 0 ? 0 : NRF); ++j) {
+            asm volatile("" : "+v"(V[j]));
+            asm volatile("" : "+v"(w[j]));
+            asm volatile("" : "+v"(phi[j]));
+        }
+    }
+    // NRF >= 0: the caller has checked that exactly NRF (<= kRfRegs) systems
+    // are present, so the sum is unrolled with no guard at all; NRF < 0: the
+    // count is read at run time as in kick().
+    template <int NRF>
+    __device__ __forceinline__ T kick_n(T t, T charge, T acc) const {
+        static_assert(NRF <= kRfRegs, "NRF exceeds the systems held in registers");
+        if constexpr (NRF < 0) {
+            return kick(t, charge, acc);
+        } else {
+            T s = T(0);
+#pragma unroll
+            for (int j = 0; j < NRF; ++j) s = s + V[j] * Trig<T>::sin(w[j] * t + phi[j]);
+            return charge * s + acc;
+        }
+    }
 };
+
+// Calls f(std::integral_constant<int, NRF>) with NRF = n_rf for the counts
+// that have a specialised particle loop (0, 1, 2) and NRF = -1 otherwise.
+// The dispatch is uniform and happens once per launch, outside the loop.
+template <typename F>
+__device__ __forceinline__ void with_rf_count(int n_rf, F &&f) {
+    switch (n_rf) {
+        case 0: f(std::integral_constant<int, 0>{}); break;
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        default: f(std::integral_constant<int, -1>{}); break;
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ T e_drift(const DriftCoeffs<T> &c, T e, int solver) {
