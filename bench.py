@@ -278,6 +278,7 @@ def form_large(torch, a, turns=200) -> dict:
         w = _workload(torch, "c3", n, t + 120, f32)
         out[key] = _measure_turns(torch, w, t, DESCR["c3"] + f", {n:.0e} macroparticles, "
                                   f"{'FP32' if f32 else 'FP64'}", f32=f32)
+        out[key]["traffic_bytes"] = measured_traffic(f"c3_{n:.0e}" + ("_f32" if f32 else ""), n)
         del w
         torch.cuda.empty_cache()
     return out
@@ -351,7 +352,7 @@ def form_batched(torch, a, turns=100, seq_turns=20) -> dict:
            "particle_kernel_frac": frac(BYTES_PER_PARTICLE_TURN * n, ms_turn * 1e-3),
            "sequential_value": sequential, "sequential_turns": seq_turns,
            "speedup_vs_sequential": batched / sequential,
-           "working_set": working_set(n)}
+           "working_set": working_set(n), "traffic_bytes": measured_traffic("c4", n)}
     if cpu is not None:
         out["cpu_baseline"] = cpu
     return out
