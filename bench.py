@@ -23,7 +23,8 @@ The JSON line carries the contract keys plus
   forms         every form of the metric on its own workload, each with its
                 particle working set against the 256 MB last-level cache
                 and, where one is defined, its own CPU baseline:
-                  plain     c1: 1e6 macroparticles, 100 slices, no intensity
+                  plain     c1: 1e6 macroparticles, 100 slices, no intensity,
+                            25 turns a launch (and a launch a turn)
                   intensity c2: 4e6, 4096 slices, FFT 16384, 21 resonators
                   large     the headline's variants: 3e7 FP64, 1e7 and 3e7 FP32
                   batched   c4: 256 bunches (1e4..1e6 each), one launch pair
@@ -74,6 +75,8 @@ def parse():
     ap.add_argument("--blocks-per-cu", type=int, default=1)
     ap.add_argument("--groups", type=int, default=0, help="count copies (0: automatic)")
     ap.add_argument("--max-peel", type=int, default=0)
+    ap.add_argument("--turn-block", type=int, default=25,
+                    help="plain turns: at most this many turns a launch (1: a launch a turn)")
     ap.add_argument("--method", choices=["auto", "fft", "direct", "fourstep"], default="auto")
     ap.add_argument("--forms", default="plain,intensity,large,batched,adjoint,wake,feedback",
                     help="comma list of extra forms to measure ('' for none)")
@@ -247,12 +250,43 @@ def cpu_baseline_adjoint(case, target_s: float) -> dict:
 # ---------------------------------------------------------------------------
 # forms
 # ---------------------------------------------------------------------------
+# FP64 operations of one plain c1 macroparticle-turn (1 RF system, simple
+# drift), contraction off: the sine's argument (2), Cephes reduction (1 + 6),
+# r^2 (1), the two degree-6 polynomials (20), their sine and cosine forms
+# (3 + 5), V sin and the sum (2), charge and acceleration (2), dE += (1),
+# drift (2)
+C1_FP64_OPS = 45
+FP64_PEAK_TFLOPS = 78.6  # published, an FMA counted as two operations
+
+
 def form_plain(torch, a, turns=1000) -> dict:
     """c1 as configured: 1e6 macroparticles, 100 slices, 1000 turns of
-    kick + drift + profile, one launch per turn."""
-    w = _workload(torch, "c1", 1_000_000, turns + 200)
+    kick + drift + profile.  The engine runs 25 turns a launch with the
+    particles in registers (only the last turn of a launch is
+    histogrammed); the variant runs a launch a turn.  With 25 turns a
+    launch the bound is FP64 issue, not HBM: the roofline is operations
+    per second against the FP64 vector peak."""
+    w = _workload(torch, "c1", 1_000_000, 2 * turns + 400)
     snap = None if a.no_cpu_baseline else w.snapshot()
-    out = _measure_turns(torch, w, turns, DESCR["c1"] + ", 1e6 macroparticles, FP64")
+    k = 25
+    out = _measure_turns(torch, w, turns, DESCR["c1"] + f", 1e6 macroparticles, FP64, "
+                         f"{k} turns a launch", turn_block=k)
+    n = out["macroparticles"]
+    # dt, dE cross the memory system once a launch
+    out["bytes_per_macroparticle_turn"] = BYTES_PER_PARTICLE_TURN / k
+    out["whole_turn_frac"] = frac(out["bytes_per_macroparticle_turn"] * n,
+                                  out["ms_per_turn"] * 1e-3)
+    out["particle_kernel_frac"] = frac(out["bytes_per_macroparticle_turn"] * n,
+                                       out["particle_kernel_ms"] * 1e-3)
+    tflops = C1_FP64_OPS * n / (out["particle_kernel_ms"] * 1e-3) / 1e12
+    out["roofline"] = {"bound": "fp64_valu", "ops_per_macroparticle_turn": C1_FP64_OPS,
+                       "achieved_tflops": tflops, "peak_tflops": FP64_PEAK_TFLOPS,
+                       "frac": tflops / FP64_PEAK_TFLOPS,
+                       "note": "no FMA (bit parity with the reference's build): "
+                               "multiplies and adds alone reach at most 0.5"}
+    one = _measure_turns(torch, w, turns, DESCR["c1"] + ", 1e6 macroparticles, FP64, "
+                         "a launch a turn", turn_block=1)
+    out["variants"] = {"a_launch_a_turn": one}
     if snap is not None:
         out["cpu_baseline"] = cpu_baseline(snap, a.cpu_seconds)
     return out
@@ -460,7 +494,7 @@ def main():
     snap = None if (a.no_cpu_baseline or a.f32) else w.snapshot()
     ft = FusedTracker(w.tracker, w.profile, w.tiv, use_graph=not a.no_graph, block=a.block,
                       blocks_per_cu=a.blocks_per_cu, groups=a.groups, max_peel=a.max_peel,
-                      method=None if a.method == "auto" else a.method)
+                      method=None if a.method == "auto" else a.method, turn_block=a.turn_block)
     ft.track(a.warmup)
     torch.cuda.synchronize()
     times, dev_ms = [], []
@@ -476,7 +510,10 @@ def main():
 
     # live per-kernel durations (HIP events on the engine's stream)
     ms_ind, ms_turn = ft.time_kernels(a.timing_turns)
-    alg = bpp * n
+    # plain turns run turn_block to a launch: the coordinates cross the
+    # memory system once a launch
+    blocked = not w.intensity and a.turn_block > 1
+    alg = bpp * n / (a.turn_block if blocked else 1)
     turn_gbs = alg / turn_s / 1e9
     kern_gbs = alg / (ms_turn * 1e-3) / 1e9
     key = f"{a.workload}_{n:.0e}" + ("_f32" if a.f32 else "")
@@ -497,7 +534,9 @@ def main():
                                f"{'FP32' if a.f32 else 'FP64'}, fused "
                                f"{'intensity' if w.intensity else 'plain'} turn",
                    "macroparticles": n, "n_slices": w.meta.get("n_slices"),
-                   "n_fft": w.meta.get("n_fft"), "form": "plain",
+                   "n_fft": w.meta.get("n_fft"),
+                   "form": "intensity" if w.intensity else "plain",
+                   "turn_block": a.turn_block if blocked else 1,
                    "graph": not a.no_graph, "repeats": a.repeats,
                    "block": a.block, "blocks_per_cu": a.blocks_per_cu, "groups": a.groups,
                    "max_peel": a.max_peel, "working_set": working_set(n, bpp // 2)},
@@ -505,8 +544,8 @@ def main():
                      "frac": turn_gbs / HBM_PEAK_GBS,
                      "traffic": measured_traffic(key, n),
                      "algorithmic_bytes": alg,
-                     "what": "whole fused turn (slice axis + particle kernel), 32 B per "
-                             "macroparticle-turn",
+                     "what": f"whole fused turn (slice axis + particle kernel), {alg / n:g} B "
+                             "per macroparticle-turn",
                      "turn_ms": turn_s * 1e3,
                      "kernel": {"name": "turn_kernel (interp kick + RF kick + drift + "
                                         "histogram)",

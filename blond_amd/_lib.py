@@ -66,6 +66,7 @@ SIGNATURES = {
     "blmi_engine_set_statistics": [_p, _i, _d, _d],
     "blmi_engine_get_statistics": [_p, _i, _i, _p],
     "blmi_engine_set_launch": [_p, _i, _i, _i],
+    "blmi_engine_set_turn_block": [_p, _i],
     "blmi_engine_set_turn": [_p, _i],
     "blmi_engine_get_turn": [_p, ct.POINTER(_i)],
     "blmi_engine_prime": [_p, _p],

@@ -619,6 +619,15 @@ int blmi_engine_set_wake(void *h, int64_t n_mem, int64_t n_add, const double *ti
     return e->set_wake(n_mem, n_add, time, memory, ind_coef_host, deriv_mode);
 }
 
+int blmi_engine_set_turn_block(void *h, int max_turns) {
+    Engine *e = reinterpret_cast<Engine *>(h);
+    BLMI_REQUIRE(e, "engine: null handle");
+    BLMI_REQUIRE(max_turns >= 1 && max_turns <= 1000, "engine: turn block in [1, 1000]");
+    e->turn_block = max_turns;
+    e->invalidate_graph();
+    return BLMI_OK;
+}
+
 int blmi_engine_set_launch(void *h, int block, int blocks_per_cu, int graph_chunk) {
     Engine *e = reinterpret_cast<Engine *>(h);
     BLMI_REQUIRE(e, "engine: null handle");

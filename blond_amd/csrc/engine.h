@@ -18,6 +18,8 @@ namespace blmi {
 constexpr int kLdsHistBytes = 131072;    // histogram words in LDS at most
 constexpr int kLdsHistU32Bytes = 65536;  // u32 counts up to this, u16 pairs above
 constexpr int kMaxParticlesPerGroup16 = 65535;
+// turns a plain-tracking launch takes at most, by default (odd)
+constexpr int kTurnBlockDefault = 25;
 // the particle kernels' sink: 2 pairs (dt, dE) of doubles for each lane of a wavefront
 constexpr size_t kSinkBytes = 64 * 2 * 2 * sizeof(double);
 // LDS the particle kernel may use for its histogram plus a staged copy of the
@@ -135,6 +137,9 @@ struct Engine {
     // launch shape (tunable)
     int block_size = 1024;
     int blocks_per_cu = 1;
+    // plain tracking: at most this many turns per particle-kernel launch
+    // (the particles stay in registers between them; 1 = a launch a turn)
+    int turn_block = kTurnBlockDefault;
     // graph replay
     int graph_chunk = 50;
     int graph_turns = 0;
@@ -191,6 +196,13 @@ struct Engine {
     int launch_turns(hipStream_t s, int count, bool advance);
     int finish_chunk(hipStream_t s, int count);
     int launch_turn_kernel(hipStream_t s, int off, bool interp, const KernelEvents &ev);
+    // whether plain turns may run several to a launch (nothing observes the
+    // coordinates or the histogram between turns), and the turns of the
+    // next launch of a chunk with `left` turns to go (odd, so the last turn's
+    // counts use the buffer parity a launch a turn would use)
+    bool multi_turn_ok() const;
+    int multi_turn_piece(int left) const;
+    int launch_turns_block(hipStream_t s, int off, int turns, const KernelEvents &ev);
     // histogram mode of the particle kernel (0 global atomics, 1 LDS u32,
     // 2 LDS u16 pairs) and its grid and LDS bytes for `block` threads
     int hist_layout(int block, int *grid, size_t *lds) const;

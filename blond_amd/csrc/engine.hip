@@ -397,6 +397,98 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
     }
 }
 
+// Plain tracking, several turns a launch.  With no intensity effects,
+// statistics, loop, interpolation table or sort, nothing observes a turn's
+// coordinates or histogram before the last turn of a chunk, so each thread
+// keeps its pair in registers across the launch's K turns and only the last
+// turn is histogrammed: the particle data cross the memory system once per
+// K turns instead of once per turn.  Every turn reads its own RF programs,
+// acceleration kick and drift coefficients from the tables, and each
+// particle's arithmetic is turn_kernel's (kick.cpp:27-33 then drift.cpp),
+// in the same order, so the coordinates are bit-identical.  The counts of
+// turn t0 + K go to buffer (t0 + K) & 1 = (t0 + 1) & 1 (K odd) and are
+// published as turn_kernel's would be.
+template <typename T, int SOLVER, int LDSH>
+__global__ void __launch_bounds__(1024) turns_kernel(T *__restrict__ dt, T *__restrict__ dE,
+                                                     int64_t n, TurnArgs<T> a, int K) {
+    using V2 = typename Vec2<T>::type;
+    extern __shared__ unsigned lds_hist[];
+    int t0 = *a.base + a.off;
+    if (t0 < 0) t0 = 0;
+    if (t0 + K - 1 > a.last_turn) t0 = a.last_turn - K + 1;  // host checks the range
+    const int tl = t0 + K;
+    unsigned *counts = ((tl & 1) ? a.counts1 : a.counts0) +
+                       (size_t)(blockIdx.x % a.groups) * a.n_slices;
+    if (a.pending[t0 & 1] == t0) {
+        unsigned *prev = (t0 & 1) ? a.counts1 : a.counts0;
+        const int per = (a.n_slices + (int)gridDim.x - 1) / (int)gridDim.x;
+        const int b0 = (int)blockIdx.x * per;
+        const int b1 = min(b0 + per, a.n_slices);
+        publish_range(prev, a.profile, a.n_slices, a.groups, b0, b1);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.pending[tl & 1] = tl;
+    if (LDSH) {
+        for (int b = threadIdx.x; b < a.n_slices; b += blockDim.x) lds_hist[b] = 0u;
+        __syncthreads();
+    }
+    const T q_charge = a.charge, q_cl = a.cut_left, q_ibw = a.inv_bw, q_ns = (T)a.n_slices;
+    auto count = [&](int bin) {
+        if (LDSH)
+            lds_count<false>(lds_hist, bin, a.max_peel);
+        else if (bin >= 0)
+            atomic_add_agent(&counts[bin], 1u);
+    };
+    auto bin_of = [&](T x) -> int {
+        const T f = floor((x - q_cl) * q_ibw);
+        return (f >= T(0) && f < q_ns) ? (int)f : -1;
+    };
+    const int64_t n2 = n / 2;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    with_rf_count(a.n_rf, [&](auto nrf) {
+        constexpr int NRF = decltype(nrf)::value;
+        // turns t0 .. tl-1 of one particle pair (or of one particle: y unused)
+        auto run = [&](T &x0, T &e0, T &x1, T &e1, bool two) {
+            for (int t = t0; t < tl; ++t) {
+                const T *V = a.rf + (size_t)t * 3 * a.n_rf;
+                const RfProg<T> rfp(V, V + a.n_rf, V + 2 * a.n_rf, a.n_rf);
+                const T acc = a.acc[t];
+                const DriftCoeffs<T> dc = a.drift[t + 1];
+                e0 = e0 + rfp.template kick_n<NRF>(x0, q_charge, acc);
+                x0 = x0 + e_drift(dc, e0, SOLVER);
+                if (two) {
+                    e1 = e1 + rfp.template kick_n<NRF>(x1, q_charge, acc);
+                    x1 = x1 + e_drift(dc, e1, SOLVER);
+                }
+            }
+        };
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) {
+            V2 x = reinterpret_cast<const V2 *>(dt)[i];
+            V2 e = reinterpret_cast<const V2 *>(dE)[i];
+            run(x.x, e.x, x.y, e.y, true);
+            reinterpret_cast<V2 *>(dt)[i] = x;
+            reinterpret_cast<V2 *>(dE)[i] = e;
+            count(bin_of(x.x));
+            count(bin_of(x.y));
+        }
+        if ((n & 1) && blockIdx.x == gridDim.x - 1 && threadIdx.x < 64) {
+            // odd tail: lane 0 of the last block's first wavefront owns it
+            int b = -1;
+            if (threadIdx.x == 0) {
+                T x = dt[n - 1], e = dE[n - 1], y = T(0), f = T(0);
+                run(x, e, y, f, false);
+                dt[n - 1] = x;
+                dE[n - 1] = e;
+                b = bin_of(x);
+            }
+            count(b);
+        }
+    });
+    if (LDSH) {
+        __syncthreads();
+        hist_flush<1>(lds_hist, counts, a.n_slices);
+    }
+}
+
 // Plain mode, end of a track() call: publish the last turn's counts (one
 // workgroup, so the pending flag is cleared after every bin is done).
 __global__ void publish_kernel(const int *base, int off, unsigned *counts0, unsigned *counts1,
@@ -1073,6 +1165,73 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
     return blmi_check_launch("turn");
 }
 
+template <typename T>
+using TurnsFn = void (*)(T *, T *, int64_t, TurnArgs<T>, int);
+
+template <typename T, int S>
+static TurnsFn<T> pick_turns(int ldsh) {
+    return ldsh == 1 ? turns_kernel<T, S, 1> : turns_kernel<T, S, 0>;
+}
+
+// `turns` plain turns from base + off in one launch (multi_turn_ok holds)
+template <typename T>
+static int launch_turns_t(const Engine &e, T *x, T *de, const T *rf, const T *acc,
+                          const DriftCoeffs<T> *drift, T cl, T ibw, hipStream_t s, int off,
+                          int turns, const KernelEvents &ev) {
+    TurnArgs<T> a{};
+    a.n_rf = e.n_rf;
+    a.charge = (T)e.charge;
+    a.rf = rf;
+    a.acc = acc;
+    a.drift = drift;
+    a.base = e.d_base;
+    a.off = off;
+    a.last_turn = e.n_turns - 1;
+    a.counts0 = e.d_counts;
+    a.counts1 = e.d_counts + (size_t)e.groups * e.n_slices;
+    a.groups = e.groups;
+    a.n_slices = e.n_slices;
+    a.cut_left = cl;
+    a.inv_bw = ibw;
+    a.max_peel = e.max_peel;
+    a.profile = e.profile;
+    a.pending = e.d_base + 1;
+    int grid = 0;
+    size_t lds = 0;
+    const int ldsh = e.hist_layout(e.block_size, &grid, &lds);
+    if (ldsh > 1) return blmi_set_error(BLMI_ERR_ARG, "multi-turn launch with u16 histogram rows");
+    TurnsFn<T> fn = e.solver == 0   ? pick_turns<T, 0>(ldsh)
+                    : e.solver == 1 ? pick_turns<T, 1>(ldsh)
+                                    : pick_turns<T, 2>(ldsh);
+    int rc = lds_attribute((const void *)fn, lds);
+    if (rc) return rc;
+    launch(fn, dim3(grid), dim3(e.block_size), lds, s, ev, x, de, e.n, a, turns);
+    return blmi_check_launch("turns");
+}
+
+bool Engine::multi_turn_ok() const {
+    if (turn_block < 2 || intensity || interp_rf || fb_on || stats_on || ordered() || wake_on())
+        return false;
+    if (n_slices <= 0 || !d_counts) return false;
+    int grid = 0;
+    size_t lds = 0;
+    return hist_layout(block_size, &grid, &lds) <= 1;
+}
+
+int Engine::multi_turn_piece(int left) const {
+    int k = left < turn_block ? left : turn_block;
+    if (k > 1 && !(k & 1)) --k;
+    return k;
+}
+
+int Engine::launch_turns_block(hipStream_t s, int off, int turns, const KernelEvents &ev) {
+    if (f32)
+        return launch_turns_t<float>(*this, dt32, dE32, d_rf32, d_acc32, d_drift32, cut_left32,
+                                     inv_bw32, s, off, turns, ev);
+    return launch_turns_t<double>(*this, dt, dE, d_rf, d_acc, d_drift, cut_left, inv_bw, s, off,
+                                  turns, ev);
+}
+
 InducedArgs Engine::induced_args(int off) const {
     InducedArgs ia{};
     ia.src = nullptr;
@@ -1468,6 +1627,15 @@ int Engine::launch_turn_kernel(hipStream_t s, int off, bool interp, const Kernel
 // without intensity effects, the publish launch for the last turn.
 int Engine::launch_turns(hipStream_t s, int count, bool advance) {
     const KernelEvents none;
+    if (multi_turn_ok()) {
+        for (int k = 0; k < count;) {
+            const int piece = multi_turn_piece(count - k);
+            int rc = launch_turns_block(s, k, piece, none);
+            if (rc) return rc;
+            k += piece;
+        }
+        return advance ? finish_chunk(s, count) : BLMI_OK;
+    }
     for (int k = 0; k < count; ++k) {
         int rc = launch_turn_seq(s, k, none, none);
         if (rc) return rc;
@@ -1538,9 +1706,18 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
     std::vector<hipEvent_t> ev(4 * (size_t)turns, nullptr);
     for (auto &x : ev) BLMI_HIP_TRY(hipEventCreate(&x));
     int rc = BLMI_OK;
-    for (int k = 0; k < turns && rc == BLMI_OK; ++k) {
-        KernelEvents ei{ev[4 * k], ev[4 * k + 1]}, et{ev[4 * k + 2], ev[4 * k + 3]};
-        rc = launch_turn_seq(own_stream, k, ei, et);
+    // with several plain turns to a launch, launch k's events time its
+    // `piece` turns (the other launches' events stay unused: elapsed 0)
+    const bool multi = multi_turn_ok();
+    int launches = 0;
+    for (int k = 0; k < turns && rc == BLMI_OK;) {
+        KernelEvents ei{ev[4 * launches], ev[4 * launches + 1]},
+            et{ev[4 * launches + 2], ev[4 * launches + 3]};
+        const int piece = multi ? multi_turn_piece(turns - k) : 1;
+        rc = multi ? launch_turns_block(own_stream, k, piece, et)
+                   : launch_turn_seq(own_stream, k, ei, et);
+        k += piece;
+        ++launches;
     }
     if (rc == BLMI_OK) rc = finish_chunk(own_stream, turns);
     if (rc == BLMI_OK && ord) rc = order_out(own_stream);
@@ -1548,7 +1725,7 @@ int Engine::time_kernels(int turns, hipStream_t user, float *ms_induced, float *
         host_turn += turns;
         BLMI_HIP_TRY(hipStreamSynchronize(own_stream));
         double a = 0, b = 0;
-        for (int k = 0; k < turns; ++k) {
+        for (int k = 0; k < launches; ++k) {
             float x = 0, y = 0;
             if (intensity) BLMI_HIP_TRY(hipEventElapsedTime(&x, ev[4 * k], ev[4 * k + 1]));
             BLMI_HIP_TRY(hipEventElapsedTime(&y, ev[4 * k + 2], ev[4 * k + 3]));

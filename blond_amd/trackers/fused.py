@@ -9,8 +9,11 @@
 
 as at most two kernels per turn (induced voltage, then one streaming pass
 doing interpolated kick + RF kick + drift + histogram), replayed from a
-captured HIP graph.  After ``track(k)`` the objects hold exactly what k passes
-of the map leave in them: ``beam.dt/dE``, ``profile.n_macroparticles``,
+captured HIP graph.  Without intensity effects, statistics, a phase loop or
+an interpolation table, up to ``turn_block`` turns run in one launch with
+the particles held in registers (only the last turn is histogrammed, which
+is all a caller can see).  After ``track(k)`` the objects hold exactly what
+k passes of the map leave in them: ``beam.dt/dE``, ``profile.n_macroparticles``,
 ``total_induced_voltage.induced_voltage``, ``rf_station.counter`` and the
 beam's synchronous energy.
 """
@@ -139,7 +142,7 @@ def wake_setup(tiv, n_turns: int):
 class FusedTracker:
     def __init__(self, tracker, profile=None, total_induced_voltage=None, use_graph=True,
                  block=1024, blocks_per_cu=1, graph_chunk=50, groups=0, max_peel=0,
-                 method=None, sort_every=0, statistics=False, monitor=None):
+                 method=None, sort_every=0, statistics=False, monitor=None, turn_block=25):
         import torch
 
         self.tracker = tracker
@@ -195,6 +198,8 @@ class FusedTracker:
             self._setup_interpolation()
         _lib.call("blmi_engine_set_launch", self._h, int(block), int(blocks_per_cu),
                   int(graph_chunk))
+        # plain turns: up to turn_block turns a launch (1: a launch a turn)
+        _lib.call("blmi_engine_set_turn_block", self._h, int(turn_block))
         _lib.call("blmi_engine_set_hist", self._h, int(groups), int(max_peel))
         _lib.call("blmi_engine_set_order", self._h, int(sort_every))
         if method is not None and self.tiv is not None:

@@ -297,6 +297,14 @@ int blmi_engine_get_statistics(void *engine, int turn0, int turns, double *momen
 /* Launch shape of the particle kernel (block in {64..1024}, workgroups per CU)
  * and the number of turns per captured graph. */
 int blmi_engine_set_launch(void *engine, int block, int blocks_per_cu, int graph_chunk);
+/* Plain tracking (no intensity effects, statistics, phase loop,
+ * interpolation table or sort, at most 65536/4 slices' u32 histogram): up to
+ * max_turns turns (rounded down to odd) in one particle-kernel launch, the
+ * particles kept in registers between them and only the last turn's
+ * histogram formed -- the coordinates are those of one launch a turn, bit
+ * for bit, and nothing reads them in between within a track() call.  1 = a
+ * launch a turn.  Default 25. */
+int blmi_engine_set_turn_block(void *engine, int max_turns);
 /* Induced-voltage method: 0 = one-workgroup LDS FFT (rfft, x Z, irfft),
  * 1 = direct convolution over all CUs (n_slices <= 8192; with a multi-turn
  * memory it forms all n_mem values and runs the wake step in the same
