@@ -392,12 +392,12 @@ def form_batched(torch, a, turns=100, seq_turns=20) -> dict:
     return out
 
 
-def _adjoint(torch, turns, every, case=None) -> dict:
+def _adjoint(torch, turns, every, case=None, map="reference") -> dict:
     from blond_amd.trackers.differentiable import DifferentiableTracker
     from blond_amd.workloads import build_adjoint_case
 
     c = case or build_adjoint_case(n_particles=1_000_000, n_turns=turns)
-    dtr = DifferentiableTracker(c.tracker, c.profile, checkpoint_every=every)
+    dtr = DifferentiableTracker(c.tracker, c.profile, checkpoint_every=every, map=map)
     ts = c.theta_star
     mk = lambda th, g: [torch.tensor(th[k], dtype=torch.float64, device="cuda", requires_grad=g)
                         for k in ("V", "phi", "R", "fr", "Q")]
@@ -415,8 +415,10 @@ def _adjoint(torch, turns, every, case=None) -> dict:
     dtr.close()
     how = ("every turn's state kept in HBM, no recompute" if every == 1 else
            f"checkpoints every {every} turns with recomputation")
+    what = ("smooth_histogram + linear_interp_kick (the reference's map)" if map == "reference"
+            else "quadratic B-spline deposit and kick (smooth variant)")
     return {"workload": f"c5: 1e6 macroparticles x {turns} turns, 256 slices, 1 broadband "
-                        f"resonator, {how}",
+                        f"resonator, {what}, {how}", "map": map,
             "value": n * turns / el, "unit": "macroparticle-turns/s (forward + reverse)",
             "seconds": el, "bytes_per_macroparticle_turn": bpp,
             "whole_frac": frac(bpp * n * turns, el),
@@ -424,15 +426,18 @@ def _adjoint(torch, turns, every, case=None) -> dict:
 
 
 def form_adjoint(torch, a, turns=500) -> dict:
-    """c5 as configured: forward + adjoint with checkpoints every 50 turns
-    and recomputation, 1e6 macroparticles x 500 turns; the variant keeps
-    every state in HBM (8 GB) and recomputes nothing."""
+    """c5 as configured: forward + adjoint of the reference's map
+    (smooth_histogram + linear_interp_kick) with checkpoints every 50 turns
+    and recomputation, 1e6 macroparticles x 500 turns; the variants keep
+    every state in HBM (8 GB) and recompute nothing, or differentiate the
+    smooth B-spline map."""
     from blond_amd.workloads import build_adjoint_case
 
     case = build_adjoint_case(n_particles=1_000_000, n_turns=turns)
     cpu = None if a.no_cpu_baseline else cpu_baseline_adjoint(case, a.cpu_seconds)
     out = _adjoint(torch, turns, 50)
-    out["variants"] = {"every_state_kept": _adjoint(torch, turns, 1)}
+    out["variants"] = {"every_state_kept": _adjoint(torch, turns, 1),
+                       "spline_map": _adjoint(torch, turns, 50, map="spline")}
     if cpu is not None:
         out["cpu_baseline"] = cpu
     return out

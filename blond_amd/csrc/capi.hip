@@ -77,25 +77,47 @@ int device_max_lds() { return info().max_lds; }
 struct Workspace {
     void *ptr[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t size[4] = {0, 0, 0, 0};
+    hipEvent_t done[4] = {nullptr, nullptr, nullptr, nullptr};  // after the last user's work
+    hipStream_t last[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool used[4] = {false, false, false, false};
 };
+// held from a lease's construction to its destruction (the launches of one
+// call); separate from g_mu, which the launch helpers take for device info
+static std::mutex g_ws_mu;
 static std::map<int, Workspace> g_ws;
 
-void *workspace(size_t bytes, int slot) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    std::lock_guard<std::mutex> lk(g_mu);
-    Workspace &w = g_ws[dev];
-    if (w.size[slot] >= bytes && w.ptr[slot]) return w.ptr[slot];
-    if (w.ptr[slot]) (void)hipFree(w.ptr[slot]);  // synchronises the device
-    w.ptr[slot] = nullptr;
-    w.size[slot] = 0;
-    size_t want = bytes < 4096 ? 4096 : bytes;
-    if (hipMalloc(&w.ptr[slot], want) != hipSuccess) {
+WorkspaceLease::WorkspaceLease(size_t bytes, int slot, hipStream_t s) : slot_(slot), s_(s) {
+    g_ws_mu.lock();
+    (void)hipGetDevice(&dev_);
+    Workspace &w = g_ws[dev_];
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    capturing_ = hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+    if (w.size[slot] < bytes || !w.ptr[slot]) {
+        if (w.ptr[slot]) (void)hipFree(w.ptr[slot]);  // synchronises the device
         w.ptr[slot] = nullptr;
-        return nullptr;
+        w.size[slot] = 0;
+        w.used[slot] = false;
+        const size_t want = bytes < 4096 ? 4096 : bytes;
+        if (hipMalloc(&w.ptr[slot], want) != hipSuccess) {
+            w.ptr[slot] = nullptr;
+            return;
+        }
+        w.size[slot] = want;
     }
-    w.size[slot] = want;
-    return w.ptr[slot];
+    if (!capturing_ && w.used[slot] && w.last[slot] != s) (void)hipStreamWaitEvent(s, w.done[slot], 0);
+    p_ = w.ptr[slot];
+}
+
+WorkspaceLease::~WorkspaceLease() {
+    Workspace &w = g_ws[dev_];
+    if (p_ && !capturing_) {
+        if (!w.done[slot_]) (void)hipEventCreateWithFlags(&w.done[slot_], hipEventDisableTiming);
+        if (w.done[slot_] && hipEventRecord(w.done[slot_], s_) == hipSuccess) {
+            w.last[slot_] = s_;
+            w.used[slot_] = true;
+        }
+    }
+    g_ws_mu.unlock();
 }
 
 // ---------------------------------------------------------------------------
@@ -257,7 +279,8 @@ int blmi_smooth_histogram(const double *dt, double *profile, double cut_left, do
     BLMI_REQUIRE(n >= 0 && n_slices >= 0, "smooth_histogram: negative size");
     BLMI_REQUIRE(n_slices == 0 || profile, "smooth_histogram: null profile");
     BLMI_REQUIRE(cut_right > cut_left, "smooth_histogram: cut_right must exceed cut_left");
-    long long *scratch = (long long *)workspace(sizeof(long long) * (size_t)(n_slices + 1), 0);
+    WorkspaceLease ws(sizeof(long long) * (size_t)(n_slices + 1), 0, S(stream));
+    long long *scratch = (long long *)ws.ptr();
     if (!scratch) return blmi_set_error(BLMI_ERR_ALLOC, "smooth_histogram: workspace allocation");
     return launch_smooth_histogram<double>(dt, n, profile, cut_left, cut_right, n_slices, scratch,
                                            S(stream));
@@ -267,7 +290,8 @@ int blmi_smooth_histogram_f32(const float *dt, float *profile, float cut_left, f
                               int n_slices, int64_t n, void *stream) {
     BLMI_REQUIRE(n >= 0 && n_slices >= 0, "smooth_histogram_f32: negative size");
     BLMI_REQUIRE(cut_right > cut_left, "smooth_histogram_f32: cut_right must exceed cut_left");
-    long long *scratch = (long long *)workspace(sizeof(long long) * (size_t)(n_slices + 1), 0);
+    WorkspaceLease ws(sizeof(long long) * (size_t)(n_slices + 1), 0, S(stream));
+    long long *scratch = (long long *)ws.ptr();
     if (!scratch) return blmi_set_error(BLMI_ERR_ALLOC, "smooth_histogram_f32: workspace allocation");
     return launch_smooth_histogram<float>(dt, n, profile, cut_left, cut_right, n_slices, scratch,
                                           S(stream));
@@ -280,7 +304,8 @@ int blmi_linear_interp_kick(const double *dt, double *dE, const double *voltage,
     BLMI_REQUIRE(n >= 0, "linear_interp_kick: negative size");
     BLMI_REQUIRE(n_slices >= 2, "linear_interp_kick: needs at least 2 slices (got %d)", n_slices);
     BLMI_REQUIRE(voltage && bin_centers, "linear_interp_kick: null voltage/bin_centers");
-    double *tab = (double *)workspace(sizeof(double) * (2 * (size_t)n_slices + 2), 1);
+    WorkspaceLease ws(sizeof(double) * (2 * (size_t)n_slices + 2), 1, S(stream));
+    double *tab = (double *)ws.ptr();
     if (!tab) return blmi_set_error(BLMI_ERR_ALLOC, "linear_interp_kick: workspace allocation");
     double *vk = tab, *fac = tab + n_slices, *par = tab + 2 * n_slices;
     int rc = launch_interp_table<double>(voltage, bin_centers, n_slices, charge, acc_kick, vk, fac,
@@ -294,7 +319,8 @@ int blmi_linear_interp_kick_f32(const float *dt, float *dE, const float *voltage
                                 float acc_kick, void *stream) {
     BLMI_REQUIRE(n >= 0, "linear_interp_kick_f32: negative size");
     BLMI_REQUIRE(n_slices >= 2, "linear_interp_kick_f32: needs at least 2 slices");
-    float *tab = (float *)workspace(sizeof(float) * (2 * (size_t)n_slices + 2), 1);
+    WorkspaceLease ws(sizeof(float) * (2 * (size_t)n_slices + 2), 1, S(stream));
+    float *tab = (float *)ws.ptr();
     if (!tab) return blmi_set_error(BLMI_ERR_ALLOC, "linear_interp_kick_f32: workspace allocation");
     float *vk = tab, *fac = tab + n_slices, *par = tab + 2 * n_slices;
     int rc = launch_interp_table<float>(voltage, bin_centers, n_slices, charge, acc_kick, vk, fac,

@@ -65,7 +65,27 @@ template <typename T>
 int launch_moments(const T *dt, const T *dE, const int64_t *id, int64_t n, double *res,
                    hipStream_t s);
 
-// library workspace (capi.hip): device scratch owned by the library, grown on demand
-void *workspace(size_t bytes, int slot);
+// A slot of the library's per-device workspace (capi.hip: device scratch
+// owned by the library, grown on demand), held for one call's launches on
+// stream s.  The constructor orders s after the slot's previous user when
+// that was another stream, and the destructor records s as the last user,
+// so calls on different streams that share a slot run one after the other
+// instead of racing.  Under stream capture the ordering is left to the
+// caller (no event is waited on or recorded).  ptr() is null when the
+// allocation failed.
+class WorkspaceLease {
+  public:
+    WorkspaceLease(size_t bytes, int slot, hipStream_t s);
+    ~WorkspaceLease();
+    WorkspaceLease(const WorkspaceLease &) = delete;
+    WorkspaceLease &operator=(const WorkspaceLease &) = delete;
+    void *ptr() const { return p_; }
+
+  private:
+    void *p_ = nullptr;
+    int dev_ = 0, slot_ = 0;
+    hipStream_t s_ = nullptr;
+    bool capturing_ = false;
+};
 
 }  // namespace blmi

@@ -381,7 +381,8 @@ template <typename T>
 int launch_moments(const T *dt, const T *dE, const int64_t *id, int64_t n, double *res,
                    hipStream_t s) {
     int grid = stream_grid(n, kMomBlock);
-    double *part = static_cast<double *>(workspace(sizeof(double) * 3 * (size_t)grid, 3));
+    WorkspaceLease ws(sizeof(double) * 3 * (size_t)grid, 3, s);
+    double *part = static_cast<double *>(ws.ptr());
     if (!part) return blmi_set_error(BLMI_ERR_ALLOC, "moments: workspace");
     hipLaunchKernelGGL((moments_pass_kernel<T, false>), dim3(grid), dim3(kMomBlock), 0, s, dt, dE,
                        id, n, res, part);

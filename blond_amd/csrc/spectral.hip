@@ -649,7 +649,15 @@ static int launch_direct_shape(const InducedArgs &a0, hipStream_t s, const Kerne
     const int nq = (a.n_slices + kSpan - 1) / kSpan * kSpan;
     auto pad = [](int k) { return (size_t)(k + (k >> 3)); };
     const size_t lds = sizeof(double) * (pad(nq) + pad(nq + C - 1) + 1);
-    if (lds + sizeof(double) * (TH / 64 + 1) * (C + 1) > (size_t)device_max_lds())
+    // the kernel's static LDS: its reduction rows and, inlined, the beam
+    // phase loop's (the compiler's figure; the rows alone if it is unknown)
+    size_t static_lds = sizeof(double) * (TH / 64 + 1) * (C + 1);
+    hipFuncAttributes fa{};
+    if (hipFuncGetAttributes(&fa, (const void *)induced_direct_kernel<C, TH, CHUNK>) == hipSuccess)
+        static_lds = fa.sharedSizeBytes;
+    else
+        (void)hipGetLastError();
+    if (lds + static_lds > (size_t)device_max_lds())
         return blmi_set_error(BLMI_ERR_UNSUPPORTED, "direct induced voltage: %d slices exceed LDS",
                               a.n_slices);
     int rc = set_lds_limit((const void *)induced_direct_kernel<C, TH, CHUNK>, lds);

@@ -115,12 +115,22 @@ def test_feedback_turns_fused(kind):
     _check(w, ref)
 
 
-@pytest.mark.parametrize("kind", ("LHC", "PSB"))
-def test_feedback_turns_fused_with_intensity(kind):
-    w = build_feedback_workload(kind, n_particles=40001, n_turns=40, intensity=True)
-    ref = _ref(w, 25)
-    ft = w.fused(25)
-    assert ft.method is not None
+@pytest.mark.parametrize("n_slices", (1024, 4096))
+@pytest.mark.parametrize("kind", ("LHC", "LHC_F", "PSB"))
+def test_feedback_turns_fused_with_intensity(kind, n_slices):
+    """The loop runs inside the direct convolution's launch, its beam phase
+    sums split over ceil(bins / workgroup size) parts (4 at 1024 slices, 8
+    at 4096).  The PSB bunch at 4096 slices (4e4 macroparticles, ~10 a bin)
+    amplifies rounding by ~2.7 a turn with identical profiles on both sides
+    (dE: 9e-14 of its maximum after turn 0, 1.3e-10 after 7 turns, 7e-3
+    after 25, measured against the oracle turn by turn): that case is held
+    to its first 6 turns."""
+    turns = 6 if (kind == "PSB" and n_slices == 4096) else 25
+    w = build_feedback_workload(kind, n_particles=40001, n_turns=40, n_slices=n_slices,
+                                intensity=True)
+    ref = _ref(w, turns)
+    ft = w.fused(turns)
+    assert ft.method == "direct"
     _check(w, ref)
 
 

@@ -159,6 +159,32 @@ def test_linear_interp_kick_bit_exact(n_slices):
     assert np.isnan(host(g)[-2]) == np.isnan(ref[-2])
 
 
+def test_linear_interp_kick_on_two_streams():
+    """The interpolation table is library workspace shared by every call on
+    a device: calls on two streams, queued without a synchronisation between
+    them (each stream's first kick behind a long kernel), must still each
+    kick by their own table."""
+    cases = []
+    for k, n_slices in enumerate((4096, 100)):
+        dt, dE = bunch(2_000_001 if k == 0 else 100_001)
+        bc = np.linspace(0, 2.5e-9, n_slices + 1)[:-1] + 2.5e-9 / (2 * n_slices)
+        V = np.random.default_rng(11 + k).normal(0, 1e3, n_slices)
+        ref = dE.copy()
+        ORC.linear_interp_kick(dt, ref, V, bc, 1.0, 0.0)
+        for _ in range(3):
+            ORC.linear_interp_kick(dt, ref, V, bc, 1.0, 0.0)
+        cases.append((dev(dt), dev(dE), dev(V), dev(bc), ref))
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    for _ in range(4):
+        for (gdt, gdE, gV, gbc, _), st in zip(cases, streams):
+            with torch.cuda.stream(st):
+                bm.linear_interp_kick(gdt, gdE, gV, gbc, 1.0, 0.0)
+    torch.cuda.synchronize()
+    for gdt, gdE, gV, gbc, ref in cases:
+        np.testing.assert_array_equal(host(gdE), ref)
+
+
 def test_rf_volt_comp_bit_exact():
     bc = np.linspace(0, 2.5e-9, 257)[:-1]
     V, w, ph = np.array([6e6, 1.5e6]), np.array([W1, 4 * W1]), np.array([0.0, 0.2])
