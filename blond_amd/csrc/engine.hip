@@ -446,39 +446,52 @@ __global__ void __launch_bounds__(1024) turns_kernel(T *__restrict__ dt, T *__re
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     with_rf_count(a.n_rf, [&](auto nrf) {
         constexpr int NRF = decltype(nrf)::value;
-        // turns t0 .. tl-1 of one particle pair (or of one particle: y unused)
-        auto run = [&](T &x0, T &e0, T &x1, T &e1, bool two) {
+        // turns t0 .. tl-1 of M particles in lock step: M independent chains
+        // of dependent FP64 operations keep the vector pipeline busy
+        auto run = [&](T *x, T *e, auto m) {
+            constexpr int M = decltype(m)::value;
             for (int t = t0; t < tl; ++t) {
                 const T *V = a.rf + (size_t)t * 3 * a.n_rf;
                 const RfProg<T> rfp(V, V + a.n_rf, V + 2 * a.n_rf, a.n_rf);
                 const T acc = a.acc[t];
                 const DriftCoeffs<T> dc = a.drift[t + 1];
-                e0 = e0 + rfp.template kick_n<NRF>(x0, q_charge, acc);
-                x0 = x0 + e_drift(dc, e0, SOLVER);
-                if (two) {
-                    e1 = e1 + rfp.template kick_n<NRF>(x1, q_charge, acc);
-                    x1 = x1 + e_drift(dc, e1, SOLVER);
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    e[j] = e[j] + rfp.template kick_n<NRF>(x[j], q_charge, acc);
+                    x[j] = x[j] + e_drift(dc, e[j], SOLVER);
                 }
             }
         };
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) {
-            V2 x = reinterpret_cast<const V2 *>(dt)[i];
-            V2 e = reinterpret_cast<const V2 *>(dE)[i];
-            run(x.x, e.x, x.y, e.y, true);
-            reinterpret_cast<V2 *>(dt)[i] = x;
-            reinterpret_cast<V2 *>(dE)[i] = e;
-            count(bin_of(x.x));
-            count(bin_of(x.y));
+        // two pairs a thread at a time, i and i + stride (past the end the
+        // second slot repeats pair i, computed again and neither stored nor
+        // counted)
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += 2 * stride) {
+            const bool two = i + stride < n2;
+            const int64_t j = two ? i + stride : i;
+            const V2 xa = reinterpret_cast<const V2 *>(dt)[i], ea = reinterpret_cast<const V2 *>(dE)[i];
+            const V2 xb = reinterpret_cast<const V2 *>(dt)[j], eb = reinterpret_cast<const V2 *>(dE)[j];
+            T x[4] = {xa.x, xa.y, xb.x, xb.y}, e[4] = {ea.x, ea.y, eb.x, eb.y};
+            run(x, e, std::integral_constant<int, 4>{});
+            reinterpret_cast<V2 *>(dt)[i] = V2{x[0], x[1]};
+            reinterpret_cast<V2 *>(dE)[i] = V2{e[0], e[1]};
+            if (two) {
+                reinterpret_cast<V2 *>(dt)[j] = V2{x[2], x[3]};
+                reinterpret_cast<V2 *>(dE)[j] = V2{e[2], e[3]};
+            }
+            count(bin_of(x[0]));
+            count(bin_of(x[1]));
+            count(two ? bin_of(x[2]) : -1);
+            count(two ? bin_of(x[3]) : -1);
         }
         if ((n & 1) && blockIdx.x == gridDim.x - 1 && threadIdx.x < 64) {
             // odd tail: lane 0 of the last block's first wavefront owns it
             int b = -1;
             if (threadIdx.x == 0) {
-                T x = dt[n - 1], e = dE[n - 1], y = T(0), f = T(0);
-                run(x, e, y, f, false);
-                dt[n - 1] = x;
-                dE[n - 1] = e;
-                b = bin_of(x);
+                T x[1] = {dt[n - 1]}, e[1] = {dE[n - 1]};
+                run(x, e, std::integral_constant<int, 1>{});
+                dt[n - 1] = x[0];
+                dE[n - 1] = e[0];
+                b = bin_of(x[0]);
             }
             count(b);
         }
