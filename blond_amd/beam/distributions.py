@@ -14,30 +14,23 @@ from ..utils import bmath as bm
 
 
 def _get_dE_from_dt(ring, rf_station, dt_amplitude: float) -> float:
-    """distributions.py:1367-1430: dE amplitude matched to a dt amplitude in a
-    single-harmonic bucket."""
+    """distributions.py:1367-1430: the dE amplitude matched to a dt
+    amplitude in the bucket of the first RF system, from the separatrix
+    relation with the synchronous phase; only the first section and the
+    first RF system enter."""
     if ring.n_sections > 1:
-        warnings.warn("WARNING in bigaussian(): the usage of several sections is not yet "
-                      "implemented. Ignoring all but the first!")
+        warnings.warn("bigaussian(): only the first of several ring sections is used for the "
+                      "matched energy spread")
     if rf_station.n_rf > 1:
-        warnings.warn("WARNING in bigaussian(): the usage of multiple RF systems is not yet "
-                      "implemented. Ignoring higher harmonics!")
-    counter = rf_station.counter[0]
-    harmonic = rf_station.harmonic[0, counter]
-    energy = rf_station.energy[counter]
-    beta = rf_station.beta[counter]
-    omega_rf = rf_station.host_array("omega_rf")[0, counter]
-    phi_s = rf_station.phi_s[counter]
-    phi_rf = rf_station.host_array("phi_rf")[0, counter]
-    eta0 = rf_station.eta_0[counter]
-    if eta0 < 0:
-        phi_rf -= np.pi
-    voltage = rf_station.charge * rf_station.host_array("voltage")[0, counter]
-    eta0 = rf_station.eta_0[counter]
-    phi_b = omega_rf * dt_amplitude + phi_s
-    return np.sqrt(voltage * energy * beta ** 2
-                   * (np.cos(phi_b) - np.cos(phi_s) + (phi_b - phi_s) * np.sin(phi_s))
-                   / (np.pi * harmonic * np.fabs(eta0)))
+        warnings.warn("bigaussian(): only the first RF system is used for the matched energy "
+                      "spread (higher harmonics ignored)")
+    t = rf_station.counter[0]
+    voltage = rf_station.charge * rf_station.host_array("voltage")[0, t]
+    phi_s = rf_station.phi_s[t]
+    phi_b = rf_station.host_array("omega_rf")[0, t] * dt_amplitude + phi_s
+    num = voltage * rf_station.energy[t] * rf_station.beta[t] ** 2 * (
+        np.cos(phi_b) - np.cos(phi_s) + (phi_b - phi_s) * np.sin(phi_s))
+    return np.sqrt(num / (np.pi * rf_station.harmonic[0, t] * np.fabs(rf_station.eta_0[t])))
 
 
 def bigaussian(ring, rf_station, beam, sigma_dt: float, sigma_dE=None, seed: int = 1234,
