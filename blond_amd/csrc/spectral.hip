@@ -1011,12 +1011,21 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
         }
         buf[c * ld + m1] = make_double2(p[0], p[1]);
     }
+    // with one output element a thread (the usual split), its twiddle is
+    // loaded before the transforms, so that load's latency overlaps them
+    const bool one = cols * M1 <= kColThreads;
+    double2 tw_out = make_double2(0.0, 0.0);
+    if (one && (int)threadIdx.x < cols * M1) {
+        const int c = threadIdx.x % cols, k1 = threadIdx.x / cols;
+        tw_out = twA[(size_t)(c0 + c) * M1 + k1];
+    }
     __syncthreads();
     fft_lds_batch<false>(buf, M1, cols, ld, st1, twl, M1, kColThreads);
     for (int j = threadIdx.x; j < cols * M1; j += kColThreads) {
         const int c = j % cols, k1 = j / cols;
         const int m2 = c0 + c;
-        T[(size_t)k1 * M2 + m2] = cmul(buf[c * ld + k1], twA[(size_t)m2 * M1 + k1]);
+        const double2 w = one ? tw_out : twA[(size_t)m2 * M1 + k1];
+        T[(size_t)k1 * M2 + m2] = cmul(buf[c * ld + k1], w);
     }
 }
 
@@ -1130,19 +1139,28 @@ __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int 
         const int col = (c0 + c) % M2;
         buf[c * ld + k1] = S[(size_t)k1 * M2 + col];
     }
+    // with one output element a thread (the usual split), the bin centres
+    // its table entries need are loaded before the transforms
+    const int ns = a.n_slices;
+    const bool one = cols * M1 <= kColThreads;
+    double bc_pre[2] = {0.0, 0.0};
+    if (one && a.tab && (int)threadIdx.x < cols * M1) {
+        const int c = threadIdx.x % cols, m1 = threadIdx.x / cols;
+        const int m = M2 * m1 + c0 + c;
+        if (2 * m < ns - 1) bc_pre[0] = a.bc[2 * m];
+        if (2 * m + 1 < ns - 1) bc_pre[1] = a.bc[2 * m + 1];
+    }
+    // linear_interp_kick.cpp:33-48 on V (acc = 0 for the induced kick); the
+    // scale is formed before the transforms (its loads overlap them)
+    double inv_bw = 0.0;
+    if (a.tab) inv_bw = interp_inv_bw(a.bc, ns, a.f32);
     __syncthreads();
     fft_lds_batch<true>(buf, M1, NC, ld, st1, twl, M1, kColThreads);
     // V = -(charge*e*ratio * irfft(...))[:n_out]   (impedance.py:475-489)
     const double fct = 1.0 / (double)M;
-    const int ns = a.n_slices;
-    double inv_bw = 0.0;
-    if (a.tab) {
-        // linear_interp_kick.cpp:33-48 on V (acc = 0 for the induced kick)
-        inv_bw = (double)(ns - 1) / (a.bc[ns - 1] - a.bc[0]);
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            a.tab_params[0] = a.bc[0];
-            a.tab_params[1] = inv_bw;
-        }
+    if (a.tab && blockIdx.x == 0 && threadIdx.x == 0) {
+        a.tab_params[0] = a.bc[0];
+        a.tab_params[1] = inv_bw;
     }
     for (int j = threadIdx.x; j < cols * M1; j += kColThreads) {
         const int c = j % cols, m1 = j / cols;
@@ -1152,14 +1170,15 @@ __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int 
         if (2 * m < a.n_out) a.volt[2 * m] = v0;
         if (2 * m + 1 < a.n_out) a.volt[2 * m + 1] = v1;
         if (a.tab && 2 * m < ns - 1) {
-            a.tab[2 * m] = interp_entry(v0, v1, a.bc[2 * m], a.charge, inv_bw, a.f32);
+            const double b0 = one ? bc_pre[0] : a.bc[2 * m];
+            a.tab[2 * m] = interp_entry(v0, v1, b0, a.charge, inv_bw, a.f32);
             if (2 * m + 1 < ns - 1) {
                 // V[2m+2]: next column, same row (column 0, next row after the last)
                 const double2 un = c + 1 < cols ? buf[(c + 1) * ld + m1]
                                    : buf[cols * ld + (last ? m1 + 1 : m1)];
                 const double v2 = -(a.scale * (un.x * fct));
-                a.tab[2 * m + 1] = interp_entry(v1, v2, a.bc[2 * m + 1], a.charge, inv_bw,
-                                                a.f32);
+                const double b1 = one ? bc_pre[1] : a.bc[2 * m + 1];
+                a.tab[2 * m + 1] = interp_entry(v1, v2, b1, a.charge, inv_bw, a.f32);
             }
         }
     }
