@@ -397,6 +397,14 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
     }
 }
 
+// LDS bytes of one turn's programs in turns_kernel (at most two RF systems
+// staged, plus the acceleration kick and the drift coefficients)
+template <typename T>
+constexpr size_t turn_prog_bytes() {
+    return ((sizeof(T) * 7 + sizeof(DriftCoeffs<T>)) + 15) / 16 * 16;
+}
+constexpr int kTurnBlockMax = 63;  // turns a multi-turn launch stages in LDS at most
+
 // Plain tracking, several turns a launch.  With no intensity effects,
 // statistics, loop, interpolation table or sort, nothing observes a turn's
 // coordinates or histogram before the last turn of a chunk, so each thread
@@ -446,19 +454,66 @@ __global__ void __launch_bounds__(1024) turns_kernel(T *__restrict__ dt, T *__re
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     with_rf_count(a.n_rf, [&](auto nrf) {
         constexpr int NRF = decltype(nrf)::value;
+        // a turn's programs: RF systems, acceleration kick, drift coefficients
+        struct TurnProg {
+            T V[NRF > 0 ? NRF : 1], w[NRF > 0 ? NRF : 1], phi[NRF > 0 ? NRF : 1];
+            T acc;
+            DriftCoeffs<T> dc;
+        };
+        static_assert(NRF < 0 || sizeof(TurnProg) <= turn_prog_bytes<T>(), "LDS program slot");
+        // with a known RF-system count the launch's K turns of programs are
+        // staged in LDS after the histogram words (one turn a thread): the
+        // turn loop then reads them with broadcast LDS loads instead of a
+        // global round trip at every turn
+        TurnProg *progs = reinterpret_cast<TurnProg *>(
+            reinterpret_cast<char *>(lds_hist) +
+            ((LDSH ? sizeof(unsigned) * (size_t)a.n_slices : 0) + 15) / 16 * 16);
+        if constexpr (NRF >= 0) {
+            for (int k = threadIdx.x; k < K; k += blockDim.x) {
+                const int t = t0 + k;
+                const T *V = a.rf + (size_t)t * 3 * a.n_rf;
+                TurnProg p;
+#pragma unroll
+                for (int j = 0; j < NRF; ++j) {
+                    p.V[j] = V[j];
+                    p.w[j] = V[a.n_rf + j];
+                    p.phi[j] = V[2 * a.n_rf + j];
+                }
+                p.acc = a.acc[t];
+                p.dc = a.drift[t + 1];
+                progs[k] = p;
+            }
+            __syncthreads();
+        }
         // turns t0 .. tl-1 of M particles in lock step: M independent chains
-        // of dependent FP64 operations keep the vector pipeline busy
+        // of dependent FP64 operations keep the vector pipeline busy; the sum
+        // is RfProg::kick_n's, in the same order
         auto run = [&](T *x, T *e, auto m) {
             constexpr int M = decltype(m)::value;
-            for (int t = t0; t < tl; ++t) {
-                const T *V = a.rf + (size_t)t * 3 * a.n_rf;
-                const RfProg<T> rfp(V, V + a.n_rf, V + 2 * a.n_rf, a.n_rf);
-                const T acc = a.acc[t];
-                const DriftCoeffs<T> dc = a.drift[t + 1];
+            if constexpr (NRF >= 0) {
+                for (int k = 0; k < K; ++k) {
+                    const TurnProg p = progs[k];
 #pragma unroll
-                for (int j = 0; j < M; ++j) {
-                    e[j] = e[j] + rfp.template kick_n<NRF>(x[j], q_charge, acc);
-                    x[j] = x[j] + e_drift(dc, e[j], SOLVER);
+                    for (int j = 0; j < M; ++j) {
+                        T s = T(0);
+#pragma unroll
+                        for (int r = 0; r < NRF; ++r)
+                            s = s + p.V[r] * Trig<T>::sin(p.w[r] * x[j] + p.phi[r]);
+                        e[j] = e[j] + (q_charge * s + p.acc);
+                        x[j] = x[j] + e_drift(p.dc, e[j], SOLVER);
+                    }
+                }
+            } else {
+                for (int t = t0; t < tl; ++t) {
+                    const T *V = a.rf + (size_t)t * 3 * a.n_rf;
+                    const RfProg<T> rfp(V, V + a.n_rf, V + 2 * a.n_rf, a.n_rf);
+                    const T acc = a.acc[t];
+                    const DriftCoeffs<T> dc = a.drift[t + 1];
+#pragma unroll
+                    for (int j = 0; j < M; ++j) {
+                        e[j] = e[j] + rfp.kick(x[j], q_charge, acc);
+                        x[j] = x[j] + e_drift(dc, e[j], SOLVER);
+                    }
                 }
             }
         };
@@ -1213,6 +1268,9 @@ static int launch_turns_t(const Engine &e, T *x, T *de, const T *rf, const T *ac
     size_t lds = 0;
     const int ldsh = e.hist_layout(e.block_size, &grid, &lds);
     if (ldsh > 1) return blmi_set_error(BLMI_ERR_ARG, "multi-turn launch with u16 histogram rows");
+    if (turns > kTurnBlockMax) return blmi_set_error(BLMI_ERR_ARG, "multi-turn launch of %d turns", turns);
+    // the histogram words (mode 1), then the turns' programs
+    lds = (ldsh == 1 ? (lds + 15) / 16 * 16 : 0) + turn_prog_bytes<T>() * (size_t)turns;
     TurnsFn<T> fn = e.solver == 0   ? pick_turns<T, 0>(ldsh)
                     : e.solver == 1 ? pick_turns<T, 1>(ldsh)
                                     : pick_turns<T, 2>(ldsh);
@@ -1232,7 +1290,8 @@ bool Engine::multi_turn_ok() const {
 }
 
 int Engine::multi_turn_piece(int left) const {
-    int k = left < turn_block ? left : turn_block;
+    const int cap = turn_block < kTurnBlockMax ? turn_block : kTurnBlockMax;
+    int k = left < cap ? left : cap;
     if (k > 1 && !(k & 1)) --k;
     return k;
 }

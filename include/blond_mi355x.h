@@ -303,7 +303,7 @@ int blmi_engine_set_launch(void *engine, int block, int blocks_per_cu, int graph
  * particles kept in registers between them and only the last turn's
  * histogram formed -- the coordinates are those of one launch a turn, bit
  * for bit, and nothing reads them in between within a track() call.  1 = a
- * launch a turn.  Default 25. */
+ * launch a turn; at most 63 turns are taken a launch.  Default 25. */
 int blmi_engine_set_turn_block(void *engine, int max_turns);
 /* Induced-voltage method: 0 = one-workgroup LDS FFT (rfft, x Z, irfft),
  * 1 = direct convolution over all CUs (n_slices <= 8192; with a multi-turn
