@@ -25,6 +25,8 @@ constexpr size_t kSinkBytes = 64 * 2 * 2 * sizeof(double);
 // LDS the particle kernel may use for its histogram plus a staged copy of the
 // interpolation table (random 16-byte gathers served by LDS instead of L2)
 constexpr int kLdsTableBudget = 144 * 1024;
+// LDS kept free of dynamic use beside the particle kernel's static arrays
+constexpr size_t kStaticLdsMargin = 1024;
 // Bin-ordered working set (slice counts above kSortWindow): bins of a
 // workgroup's LDS window (u32 histogram + interpolation table) and the
 // coarse-digit width of the counting sort (2^shift bins).
@@ -57,6 +59,13 @@ struct Engine {
     unsigned *d_counts = nullptr;  // [2][groups][n_slices]
     unsigned *d_partials = nullptr;  // u16-pair mode: [grid][words] per-workgroup histograms
     size_t partials_bytes = 0;
+    // u16-pair rows for the four-step transform above 2^15 slices: the LDS
+    // histograms (and the rows) cover a window of half the columns' words,
+    // rows win_m1 .. win_m1 + L/2 - 1 of the transform's columns (-1: no
+    // window, set at prime() around the bunch); bins outside it are counted
+    // in d_cnt_out with device atomics ([n_slices], cleared by pass A)
+    int win_m1 = -1;
+    unsigned *d_cnt_out = nullptr;
     double *profile = nullptr;     // borrowed: f64 profile seen by the caller
     // induced voltage
     bool intensity = false;
@@ -220,7 +229,11 @@ struct Engine {
     // whether the u16-pair rows are stored transposed for the four-step
     // transform's first pass, which then sums them itself (no reduction
     // launch); shift = log2 M2 and lshift = log2 of the words per column
-    bool part_perm(int *shift, int *lshift) const;
+    bool part_perm(int *shift, int *lshift, int *m1_lo = nullptr) const;
+    // whether the windowed rows apply (part_perm holds and the full rows
+    // exceed 64 KiB of LDS), and the window's size in column words
+    bool part_window_possible(int *lw) const;
+    int choose_window(hipStream_t s);
     int launch_reduce(hipStream_t s, int off);
     int launch_induced(hipStream_t s, int off, const KernelEvents &ev);
     InducedArgs induced_args(int off) const;

@@ -81,6 +81,13 @@ struct TurnArgs {
     unsigned *partials;
     int part_words;
     int part_shift, part_lshift;
+    // windowed rows: LDS counts bins [win_lo, win_lo + win_bins), the others
+    // go to cnt_out with device atomics (win_bins 0: no window)
+    int win_lo, win_bins;
+    unsigned *cnt_out;
+    // interpolation table from L2: entries [tw_lo, tw_lo + tw_n) staged in
+    // LDS after the histogram (tw_n 0: none)
+    int tw_lo, tw_n;
     // bunch moments of the coordinates after the drift: workgroup b writes
     // sum(dt - shift_t), sum((dt - shift_t)^2), sum(dE - shift_e),
     // sum((dE - shift_e)^2) of its particles to stats[(t + 1) * grid + b]
@@ -250,17 +257,27 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
         xa = reinterpret_cast<const V2 *>(dt)[clamp(i_first)];
         ea = reinterpret_cast<const V2 *>(dE)[clamp(i_first)];
     }
-    // INTERP == 2: the table is staged in LDS after the histogram words
-    const int hwords = (HIST && LDSH) ? hist_words(LDSH, a.n_slices) : 0;
-    double2 *ltab = reinterpret_cast<double2 *>(lds_hist + ((hwords + 3) & ~3));
-    // the padded layout adds one word per 2^part_shift (none when 31)
+    // the histogram words (with windowed rows, the window's); the padded
+    // layout adds one word per 2^part_shift (none when 31)
+    const int hwords = (HIST && LDSH) ? ((LDSH == 2 && a.win_bins) ? a.win_bins / 2
+                                                                  : hist_words(LDSH, a.n_slices))
+                                      : 0;
     const int lwords = LDSH == 2 ? lds_word(hwords, a.part_shift) : hwords;
+    // after them in LDS: INTERP == 2 the whole table; INTERP == 1 with u16
+    // rows a window of it (entries tw_lo .. tw_lo + tw_n - 1)
+    double2 *ltab = reinterpret_cast<double2 *>(lds_hist + ((lwords + 3) & ~3));
+    constexpr bool kTabWindow = INTERP == 1 && HIST && LDSH == 2;
+    const bool tw_on = kTabWindow && a.tw_n > 0;
     if ((HIST && LDSH) || INTERP == 2) {
         for (int b = threadIdx.x; b < lwords; b += blockDim.x) lds_hist[b] = 0u;
         if (INTERP == 2)
             for (int k = threadIdx.x; k < a.n_tab; k += blockDim.x) ltab[k] = a.tab[k];
+        if (tw_on)
+            for (int k = threadIdx.x; k < a.tw_n; k += blockDim.x) ltab[k] = a.tab[a.tw_lo + k];
         __syncthreads();
     }
+    // the table window (lds_hist itself, never read, when there is none)
+    const double2 *ltw = tw_on ? ltab : reinterpret_cast<const double2 *>(lds_hist);
     // linear_interp_kick.cpp:44-64 (acceleration kick 0): the table entry of
     // x, loaded unconditionally (bin 0 stands in outside the table; `in`
     // says whether the kick applies)
@@ -273,17 +290,34 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
     // v_readlane at every use)
     T q_charge = a.charge, q_acc = acc, q_cl = a.cut_left, q_ibw = a.inv_bw;
     T q_ntab = (T)a.n_tab, q_ns = (T)a.n_slices, q_bc0 = bc0, q_tibw = ibw;
+    int q_twlo = a.tw_lo, q_twn = a.tw_n, q_wlo = a.win_lo, q_wn = a.win_bins;
+    uint64_t q_tabp = (uint64_t)a.tab, q_cntp = (uint64_t)a.cnt_out;
     auto pin_constants = [&]() {
         asm volatile("" : "+v"(q_charge), "+v"(q_acc), "+v"(q_cl), "+v"(q_ibw));
         if (INTERP) asm volatile("" : "+v"(q_ntab), "+v"(q_bc0), "+v"(q_tibw));
         if (HIST) asm volatile("" : "+v"(q_ns));
+        if (kTabWindow) asm volatile("" : "+v"(q_twlo), "+v"(q_twn), "+v"(q_tabp));
+        if (HIST && LDSH == 2) asm volatile("" : "+v"(q_wlo), "+v"(q_wn), "+v"(q_cntp));
     };
     auto gather = [&](T x) -> Tab {
         Tab g{make_double2(0.0, 0.0), false};
         if (INTERP) {
             const T f = floor((x - q_bc0) * q_tibw);
             g.in = f >= T(0) && f < q_ntab;
-            g.k = INTERP == 2 ? ltab[g.in ? (int)f : 0] : a.tab[g.in ? (int)f : 0];
+            const int fi = g.in ? (int)f : 0;
+            if (kTabWindow) {
+                // both loads unconditional: lanes served by the LDS window
+                // all ask L2 for entry 0, one request for all of them
+                const int kw = fi - q_twlo;
+                const bool intw = (unsigned)kw < (unsigned)q_twn;
+                using NV2 = double __attribute__((ext_vector_type(2)));
+                const NV2 kv = reinterpret_cast<GlobalPtr<const NV2>>(q_tabp)[intw ? 0 : fi];
+                const double2 kg = make_double2(kv.x, kv.y);
+                const double2 kl = ltw[intw ? kw : 0];
+                g.k = intw ? kl : kg;
+            } else {
+                g.k = INTERP == 2 ? ltab[fi] : a.tab[fi];
+            }
         }
         return g;
     };
@@ -315,10 +349,19 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
     };
     auto count = [&](int bin) {
         if (!HIST) return;
-        if (LDSH)
+        if (LDSH == 2 && a.win_bins) {
+            // windowed rows: bins outside the window with device atomics
+            const int wb = bin - q_wlo;
+            const bool in = bin >= 0 && (unsigned)wb < (unsigned)q_wn;
+            lds_count<true>(lds_hist, in ? wb : -1, a.max_peel, a.part_shift);
+            if (bin >= 0 && !in)
+                __hip_atomic_fetch_add(&reinterpret_cast<GlobalPtr<unsigned>>(q_cntp)[bin], 1u,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (LDSH) {
             lds_count<LDSH == 2>(lds_hist, bin, a.max_peel, a.part_shift);
-        else if (bin >= 0)
+        } else if (bin >= 0) {
             atomic_add_agent(&counts[bin], 1u);
+        }
     };
     // one copy of the particle loop per RF-system count (see with_rf_count)
     with_rf_count(a.n_rf, [&](auto nrf) {
@@ -590,13 +633,15 @@ __global__ void __launch_bounds__(1024) prime_hist_kernel(const T *__restrict__ 
                                                           unsigned *counts1, int n_slices,
                                                           T cut_left, T inv_bw,
                                                           unsigned *partials, int part_words,
-                                                          int part_shift, int part_lshift) {
+                                                          int part_shift, int part_lshift,
+                                                          int win_lo, int win_bins,
+                                                          unsigned *cnt_out) {
     using V2 = typename Vec2<T>::type;
     extern __shared__ unsigned lds_hist[];
     const int t = *base;
     unsigned *counts = (t & 1) ? counts1 : counts0;
     if (LDSH) {
-        const int hw = hist_words(LDSH, n_slices);
+        const int hw = (LDSH == 2 && win_bins) ? win_bins / 2 : hist_words(LDSH, n_slices);
         const int words = LDSH == 2 ? lds_word(hw, part_shift) : hw;
         for (int b = threadIdx.x; b < words; b += blockDim.x) lds_hist[b] = 0u;
         __syncthreads();
@@ -606,10 +651,17 @@ __global__ void __launch_bounds__(1024) prime_hist_kernel(const T *__restrict__ 
         return (f >= T(0) && f < (T)n_slices) ? (int)f : -1;
     };
     auto count = [&](int bin) {
-        if (LDSH)
+        if (LDSH == 2 && win_bins) {
+            // windowed rows: bins outside the window with device atomics
+            const int wb = bin - win_lo;
+            const bool in = bin >= 0 && (unsigned)wb < (unsigned)win_bins;
+            lds_count<true>(lds_hist, in ? wb : -1, 0, part_shift);
+            if (bin >= 0 && !in) atomic_add_agent(&cnt_out[bin], 1u);
+        } else if (LDSH) {
             lds_count<LDSH == 2>(lds_hist, bin, 0, part_shift);
-        else if (bin >= 0)
+        } else if (bin >= 0) {
             atomic_add_agent(&counts[bin], 1u);
+        }
     };
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t n2 = n / 2;
@@ -653,6 +705,9 @@ void Engine::release() {
     (void)hipFree(d_partials);
     d_partials = nullptr;
     partials_bytes = 0;
+    (void)hipFree(d_cnt_out);
+    d_cnt_out = nullptr;
+    win_m1 = -1;
     (void)hipFree(d_ord);
     (void)hipFree(d_perm);
     (void)hipFree(d_win);
@@ -1046,9 +1101,10 @@ int Engine::ensure_partials() {
     return BLMI_OK;
 }
 
-bool Engine::part_perm(int *shift, int *lshift) const {
+bool Engine::part_perm(int *shift, int *lshift, int *m1_lo) const {
     *shift = 31;
     *lshift = 0;
+    if (m1_lo) *m1_lo = 0;
     if (!intensity || !have_conv || method != 2 || !have_fourstep || fb_on || ordered())
         return false;
     int grid = 0;
@@ -1064,6 +1120,23 @@ bool Engine::part_perm(int *shift, int *lshift) const {
     if ((size_t)(W + W / M2) * sizeof(unsigned) > (size_t)device_max_lds()) return false;
     *shift = __builtin_ctz(M2);
     *lshift = __builtin_ctz(L);
+    int lw = 0;
+    if (win_m1 >= 0 && d_cnt_out && part_window_possible(&lw) && win_m1 + lw <= L) {
+        *lshift = __builtin_ctz(lw);
+        if (m1_lo) *m1_lo = win_m1;
+    }
+    return true;
+}
+
+bool Engine::part_window_possible(int *lw) const {
+    // full rows above 64 KiB of LDS: half of each column's words in LDS,
+    // which leaves room for a window of the interpolation table
+    const int W = hist_words(2, n_slices);
+    const int M2 = plan2.M2;
+    if (!have_fourstep || M2 < 2 || W % M2 || (size_t)W * sizeof(unsigned) <= 65536) return false;
+    const int L = W / M2;
+    if (L < 8 || (L & (L - 1)) || kColThreads % (plan2.cols * (L / 8))) return false;
+    *lw = L / 2;
     return true;
 }
 
@@ -1084,8 +1157,17 @@ int Engine::launch_reduce(hipStream_t s, int off) {
 template <typename T>
 static int launch_prime(Engine *e, const T *x, T cl, T ibw, int mode, int grid, size_t lds,
                         unsigned *c1, int W, hipStream_t s) {
-    int ps = 31, pl = 0;
-    if (mode == 2 && e->part_perm(&ps, &pl)) lds = sizeof(unsigned) * (size_t)lds_word(W, ps);
+    int ps = 31, pl = 0, m1lo = 0, win_lo = 0, win_bins = 0;
+    unsigned *cnt_out = nullptr;
+    if (mode == 2 && e->part_perm(&ps, &pl, &m1lo)) {
+        const int We = (1 << pl) << ps;  // the rows' words (the window's, if any)
+        lds = sizeof(unsigned) * (size_t)lds_word(We, ps);
+        if (We < W) {
+            win_lo = 2 * (m1lo << ps);
+            win_bins = 2 * We;
+            cnt_out = e->d_cnt_out;
+        }
+    }
     const void *fn = mode == 2   ? (const void *)prime_hist_kernel<T, 2>
                      : mode == 1 ? (const void *)prime_hist_kernel<T, 1>
                                  : (const void *)prime_hist_kernel<T, 0>;
@@ -1095,15 +1177,79 @@ static int launch_prime(Engine *e, const T *x, T cl, T ibw, int mode, int grid, 
     if (mode == 2)
         hipLaunchKernelGGL((prime_hist_kernel<T, 2>), dim3(grid), dim3(blk), lds, s, x, e->n,
                            e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W,
-                           ps, pl);
+                           ps, pl, win_lo, win_bins, cnt_out);
     else if (mode == 1)
         hipLaunchKernelGGL((prime_hist_kernel<T, 1>), dim3(grid), dim3(blk), lds, s, x, e->n,
                            e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W,
-                           ps, pl);
+                           ps, pl, win_lo, win_bins, cnt_out);
     else
         hipLaunchKernelGGL((prime_hist_kernel<T, 0>), dim3(grid), dim3(blk), 0, s, x, e->n,
                            e->d_base, e->d_counts, c1, e->n_slices, cl, ibw, e->d_partials, W,
-                           ps, pl);
+                           ps, pl, win_lo, win_bins, cnt_out);
+    return BLMI_OK;
+}
+
+// Counts of the current coordinates per transform column row (bin >>
+// cshift), for choosing the windowed rows' position (nc <= 1024).
+template <typename T>
+__global__ void __launch_bounds__(1024) coarse_hist_kernel(const T *__restrict__ dt, int64_t n,
+                                                           T cut_left, T inv_bw, int n_slices,
+                                                           int cshift, int nc,
+                                                           unsigned *__restrict__ out) {
+    __shared__ unsigned h[1024];
+    for (int b = threadIdx.x; b < nc; b += blockDim.x) h[b] = 0u;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const T f = floor((dt[i] - cut_left) * inv_bw);
+        if (f >= T(0) && f < (T)n_slices) atomicAdd(&h[(int)f >> cshift], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nc; b += blockDim.x)
+        if (h[b]) atomic_add_agent(&out[b], h[b]);
+}
+
+// The windowed rows' position: the half of the transform's column rows
+// holding most of the bunch (one synchronous read of L counts, at prime).
+int Engine::choose_window(hipStream_t s) {
+    int lw = 0;
+    int ps = 31, pl = 0;
+    win_m1 = -1;
+    if (!part_perm(&ps, &pl) || !part_window_possible(&lw)) return BLMI_OK;
+    const int L = 1 << pl;  // full rows (no window set yet)
+    if (L > 1024) return BLMI_OK;
+    if (!d_cnt_out) BLMI_HIP_TRY(hipMalloc(&d_cnt_out, sizeof(unsigned) * (size_t)n_slices));
+    int rc = launch_fill<unsigned>(d_cnt_out, n_slices, 0u, s);
+    if (rc) return rc;
+    const int cshift = 1 + ps;  // bins per column row: 2 M2
+    const int grid = device_cu_count();
+    if (f32)
+        hipLaunchKernelGGL(coarse_hist_kernel<float>, dim3(grid), dim3(1024), 0, s, dt32, n,
+                           cut_left32, inv_bw32, n_slices, cshift, L, d_cnt_out);
+    else
+        hipLaunchKernelGGL(coarse_hist_kernel<double>, dim3(grid), dim3(1024), 0, s, dt, n,
+                           cut_left, inv_bw, n_slices, cshift, L, d_cnt_out);
+    rc = blmi_check_launch("coarse_hist");
+    if (rc) return rc;
+    std::vector<unsigned> h(L);
+    BLMI_HIP_TRY(hipMemcpyAsync(h.data(), d_cnt_out, sizeof(unsigned) * L, hipMemcpyDeviceToHost, s));
+    BLMI_HIP_TRY(hipStreamSynchronize(s));
+    rc = launch_fill<unsigned>(d_cnt_out, n_slices, 0u, s);
+    if (rc) return rc;
+    uint64_t cur = 0, best = 0;
+    for (int k = 0; k < lw; ++k) cur += h[k];
+    int at = 0;
+    best = cur;
+    for (int k = 1; k + lw <= L; ++k) {
+        cur += h[k + lw - 1];
+        cur -= h[k - 1];
+        if (cur > best) {
+            best = cur;
+            at = k;
+        }
+    }
+    win_m1 = at;
+    invalidate_graph();
     return BLMI_OK;
 }
 
@@ -1112,6 +1258,8 @@ int Engine::prime(hipStream_t s) {
     // Profile.track()) into counts[t & 1]; the other buffer cleared
     if (!intensity || n_slices <= 0) return BLMI_OK;
     int rc = ensure_partials();
+    if (rc) return rc;
+    rc = choose_window(s);
     if (rc) return rc;
     rc = launch_fill<unsigned>(d_counts, 2 * (int64_t)groups * n_slices, 0u, s);
     if (rc) return rc;
@@ -1202,7 +1350,11 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
     a.pending = e.d_base + 1;
     a.partials = e.d_partials;
     a.part_words = hist_words(2, e.n_slices);
-    const bool perm = e.part_perm(&a.part_shift, &a.part_lshift);
+    int m1lo = 0;
+    const bool perm = e.part_perm(&a.part_shift, &a.part_lshift, &m1lo);
+    a.win_lo = a.win_bins = 0;
+    a.cnt_out = nullptr;
+    a.tw_lo = a.tw_n = 0;
     a.stats = e.stats_on ? e.d_stats : nullptr;
     a.shift_t = e.stat_shift_t;
     a.shift_e = e.stat_shift_e;
@@ -1214,7 +1366,15 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
         ldsh = 0;
         lds = 0;
     }
-    if (ldsh == 2 && perm) lds = sizeof(unsigned) * (size_t)lds_word(a.part_words, a.part_shift);
+    if (ldsh == 2 && perm) {
+        const int We = (1 << a.part_lshift) << a.part_shift;  // the window's words, if any
+        lds = sizeof(unsigned) * (size_t)lds_word(We, a.part_shift);
+        if (We < a.part_words) {
+            a.win_lo = 2 * (m1lo << a.part_shift);
+            a.win_bins = 2 * We;
+            a.cnt_out = e.d_cnt_out;
+        }
+    }
     int imode = interp ? 1 : 0;
     // stage the interpolation table in LDS when it fits beside a u32 histogram
     const size_t tab_bytes = sizeof(double2) * (size_t)(e.n_slices > 1 ? e.n_slices - 1 : 0);
@@ -1222,6 +1382,22 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
     if (interp && ldsh != 2 && hist_bytes + tab_bytes <= (size_t)kLdsTableBudget) {
         imode = 2;
         lds = hist_bytes + tab_bytes;
+    }
+    if (imode == 1 && hist && ldsh == 2 && a.n_tab > 0) {
+        // the LDS the u16 rows leave holds a window of the table, centred on
+        // the rows' window (else on the slice range)
+        const size_t base = (lds + 15) / 16 * 16;
+        const size_t cap = (size_t)device_max_lds();
+        const size_t avail = cap > base + kStaticLdsMargin ? cap - base - kStaticLdsMargin : 0;
+        int tw = (int)std::min<size_t>(avail / sizeof(double2), (size_t)a.n_tab) & ~63;
+        if (tw >= 256) {
+            const int c = a.win_bins ? a.win_lo + a.win_bins / 2 : a.n_tab / 2;
+            int lo = c - tw / 2;
+            lo = std::max(0, std::min(lo, a.n_tab - tw));
+            a.tw_lo = lo;
+            a.tw_n = tw;
+            lds = base + sizeof(double2) * (size_t)tw;
+        }
     }
     const bool publish = !e.intensity;
     TurnFn<T> fn = e.solver == 0   ? pick_turn<T, 0>(imode, hist, ldsh, publish)
@@ -1324,15 +1500,21 @@ InducedArgs Engine::induced_args(int off) const {
     ia.tab = reinterpret_cast<double2 *>(d_table);
     ia.tab_params = d_table + 2 * n_slices;
     ia.f32 = f32 ? 1 : 0;
-    int ps = 31, pl = 0;
+    int ps = 31, pl = 0, m1lo = 0;
     ia.part = nullptr;
-    if (part_perm(&ps, &pl)) {
+    ia.cnt_out = nullptr;
+    ia.part_m1_lo = 0;
+    if (part_perm(&ps, &pl, &m1lo)) {
         int grid = 0;
         size_t lds = 0;
         (void)hist_layout(block_size, &grid, &lds);
         ia.part = d_partials;
         ia.part_rows = grid;
         ia.part_lshift = pl;
+        if (((1 << pl) << ps) < hist_words(2, n_slices)) {
+            ia.part_m1_lo = m1lo;
+            ia.cnt_out = d_cnt_out;
+        }
     }
     ia.gl = d_gl;
     ia.gl_len = gl_len;

@@ -69,6 +69,11 @@ struct InducedArgs {
     // part[(m2 part_rows + r) 2^part_lshift + m1]); nullptr: the counts
     const unsigned *part;
     int part_rows, part_lshift;
+    // windowed rows: they hold column words part_m1_lo .. part_m1_lo +
+    // 2^part_lshift - 1; the other bins' counts are in cnt_out ([n_slices],
+    // added and cleared here; nullptr: the rows cover every bin)
+    int part_m1_lo;
+    unsigned *cnt_out;
     // four-step method: Z in pass B's order, Zp[r M2 + k2] = Z[k] and
     // Zp[M + r M2 + k2] = Z[M - k], k = r + M1 k2 (nullptr: read Z)
     const double2 *Zp;

@@ -978,17 +978,30 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
         double p[2] = {0.0, 0.0};
         const int jj[2] = {j0, j1};
         if (a.part) {
-            // bins 2m (low halves) and 2m + 1 (high halves) of column c's word m1
-            if (m1 < L) {
-                const int qm = c * (L / 4) + m1 / 4, i = m1 % 4;
-                unsigned s0 = 0u, s1 = 0u;
+            // bins 2m (low halves) and 2m + 1 (high halves) of column c's word
+            // m1, held by the rows at word m1 - part_m1_lo; with windowed rows
+            // the bins outside are in cnt_out (read and cleared here)
+            const int mw = m1 - a.part_m1_lo;
+            unsigned s0 = 0u, s1 = 0u;
+            if (mw >= 0 && mw < L) {
+                const int qm = c * (L / 4) + mw / 4, i = mw % 4;
                 for (int g = 0; g < (int)kColThreads / QC; ++g) {
                     s0 += red[(g * QC + qm) * 8 + 2 * i];
                     s1 += red[(g * QC + qm) * 8 + 2 * i + 1];
                 }
-                p[0] = (double)s0;
-                p[1] = (double)s1;
             }
+            if (a.cnt_out) {
+                if (j0 < ns) {
+                    s0 += a.cnt_out[j0];
+                    a.cnt_out[j0] = 0u;
+                }
+                if (j1 < ns) {
+                    s1 += a.cnt_out[j1];
+                    a.cnt_out[j1] = 0u;
+                }
+            }
+            p[0] = (double)s0;
+            p[1] = (double)s1;
             if (a.profile_out) {
                 if (j0 < ns) a.profile_out[j0] = p[0];
                 if (j1 < ns) a.profile_out[j1] = p[1];
@@ -1322,8 +1335,9 @@ int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t
     const int cols_a = p.cols;
     if (a.part) {
         const int L = 1 << a.part_lshift;
-        if (L < 4 || kColThreads % (cols_a * (L / 4)) || (int64_t)L * p.M2 * 2 < a.n_slices ||
-            L > p.M1)
+        if (L < 4 || kColThreads % (cols_a * (L / 4)) || a.part_m1_lo < 0 ||
+            a.part_m1_lo + L > p.M1 ||
+            (!a.cnt_out && (a.part_m1_lo != 0 || (int64_t)L * p.M2 * 2 < a.n_slices)))
             return blmi_set_error(BLMI_ERR_ARG, "four-step: partial-row layout does not fit");
     }
     // the transposed partial rows add the row groups' sums (8 words a thread)
