@@ -191,6 +191,90 @@ __device__ void fft_lds_batch(double2 *buf, int L, int batch, int ld, const Stag
     }
 }
 
+// One transform of length L (a power of two, 16..1024) by one wavefront:
+// the Stockham stages of stockham_stage (a radix-2 stage first when log2 L
+// is odd, then radix 4) with compile-time lengths, each lane taking
+// butterflies lane, lane + 64, ...  A wavefront's LDS operations are
+// performed in the order it issues them, so a stage's writes land before the
+// next stage's reads without a workgroup barrier; the wavefront fences only
+// keep the compiler from moving LDS accesses across a stage boundary.  (The
+// workgroup-wide stages spend two barriers a stage and keep only L / R
+// threads busy.)
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int R, int L, int NS, bool INV>
+__device__ __forceinline__ void wave_stage(double2 *b, const double2 *__restrict__ tw, int lane) {
+    constexpr int NB = L / R;            // butterflies
+    constexpr int Q = (NB + 63) / 64;    // a lane's share
+    constexpr int TS = L / (NS * R);     // twiddle stride in the length-L table
+    double2 v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = lane + 64 * q;
+        if (NB % 64 == 0 || j < NB) {
+            const int k = j & (NS - 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double2 x = b[j + r * NB];
+                if (r > 0 && NS > 1) {  // first stage: every twiddle is 1
+                    double2 w = tw[r * k * TS];
+                    if (INV) w.y = -w.y;
+                    x = cmul(x, w);
+                }
+                v[q][r] = x;
+            }
+        }
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = lane + 64 * q;
+        if (NB % 64 == 0 || j < NB) {
+            const int k = j & (NS - 1);
+            Dft<R, INV>::run(v[q]);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) b[base + r * NS] = v[q][r];
+        }
+    }
+    wave_lds_fence();
+}
+
+template <int L, int NS, bool INV>
+__device__ __forceinline__ void wave_radix4_stages(double2 *b, const double2 *__restrict__ tw,
+                                                   int lane) {
+    if constexpr (NS < L) {
+        wave_stage<4, L, NS, INV>(b, tw, lane);
+        wave_radix4_stages<L, NS * 4, INV>(b, tw, lane);
+    }
+}
+
+template <int L, bool INV>
+__device__ __forceinline__ void wave_fft(double2 *b, const double2 *__restrict__ tw, int lane) {
+    static_assert(L >= 16 && L <= 1024 && (L & (L - 1)) == 0, "power of two in [16, 1024]");
+    if constexpr ((__builtin_ctz(L) & 1) != 0) {
+        wave_stage<2, L, 1, INV>(b, tw, lane);
+        wave_radix4_stages<L, 2, INV>(b, tw, lane);
+    } else {
+        wave_radix4_stages<L, 1, INV>(b, tw, lane);
+    }
+}
+
+// `batch` transforms of length L, transform f at buf + f * ld, wavefront w
+// taking f = w, w + nthreads / 64, ...; ends with a workgroup barrier.
+template <int L, bool INV>
+__device__ __forceinline__ void wave_fft_batch(double2 *buf, int batch, int ld,
+                                               const double2 *__restrict__ tw, int nthreads) {
+    const int lane = threadIdx.x & 63;
+    for (int f = threadIdx.x >> 6; f < batch; f += nthreads >> 6)
+        wave_fft<L, INV>(buf + f * ld, tw, lane);
+    __syncthreads();
+}
+
 template <bool INV>
 __device__ void fft_lds(double2 *buf, int M, const StageList &st, const double2 *__restrict__ tw) {
     fft_lds_batch<INV>(buf, M, 1, M, st, tw, M, (int)blockDim.x);
@@ -913,6 +997,9 @@ __device__ __forceinline__ void count_buffers(const InducedArgs &a, const unsign
     }
 }
 
+// FL, FL2: the column / row length as a constant when it is a power of two
+// in [16, 1024] (one wavefront a transform), else 0 (workgroup-wide stages).
+template <int FL>
 __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int M1, int M2,
                                                              int cols, StageList st1,
                                                              const double2 *__restrict__ twM,
@@ -1033,7 +1120,10 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
         tw_out = twA[(size_t)(c0 + c) * M1 + k1];
     }
     __syncthreads();
-    fft_lds_batch<false>(buf, M1, cols, ld, st1, twl, M1, kColThreads);
+    if constexpr (FL > 0)
+        wave_fft_batch<FL, false>(buf, cols, ld, twl, kColThreads);
+    else
+        fft_lds_batch<false>(buf, M1, cols, ld, st1, twl, M1, kColThreads);
     for (int j = threadIdx.x; j < cols * M1; j += kColThreads) {
         const int c = j % cols, k1 = j / cols;
         const int m2 = c0 + c;
@@ -1044,6 +1134,7 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
 
 constexpr int kPassBThreads = 512;
 
+template <int FL2>
 __global__ void __launch_bounds__(kPassBThreads) fft2_b_kernel(InducedArgs a, int M1, int M2,
                                                                StageList st2,
                               const double2 *__restrict__ twM, const double2 *__restrict__ twB,
@@ -1085,7 +1176,10 @@ __global__ void __launch_bounds__(kPassBThreads) fft2_b_kernel(InducedArgs a, in
         buf[j] = T[(size_t)(row ? r1 : r0) * M2 + k2];
     }
     __syncthreads();
-    fft_lds_batch<false>(buf, M2, nrow, M2, st2, twl, M2, kPassBThreads);
+    if constexpr (FL2 > 0)
+        wave_fft_batch<FL2, false>(buf, nrow, M2, twl, kPassBThreads);
+    else
+        fft_lds_batch<false>(buf, M2, nrow, M2, st2, twl, M2, kPassBThreads);
     // buf[row][k2] = Zc[r + M1 k2]; the mirror bin M - k sits at
     // row 0, (M2 - k2) % M2 for r = 0 and at row of M1 - r, M2 - 1 - k2 otherwise
     double2 u[EMAX];
@@ -1121,7 +1215,10 @@ __global__ void __launch_bounds__(kPassBThreads) fft2_b_kernel(InducedArgs a, in
         if (j < ne) buf[j] = u[e];
     }
     __syncthreads();
-    fft_lds_batch<true>(buf, M2, nrow, M2, st2, twl, M2, kPassBThreads);
+    if constexpr (FL2 > 0)
+        wave_fft_batch<FL2, true>(buf, nrow, M2, twl, kPassBThreads);
+    else
+        fft_lds_batch<true>(buf, M2, nrow, M2, st2, twl, M2, kPassBThreads);
 #pragma unroll
     for (int e = 0; e < EMAX; ++e) {
         const int j = threadIdx.x + e * kPassBThreads;
@@ -1135,6 +1232,7 @@ __global__ void __launch_bounds__(kPassBThreads) fft2_b_kernel(InducedArgs a, in
 
 // Pass C also transforms the column after its batch (column 0, one row down,
 // for the last batch), so every V[2m+2] a table entry needs is at hand.
+template <int FL>
 __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int M1, int M2,
                                                              int cols, StageList st1,
                                                              const double2 *__restrict__ twM,
@@ -1168,7 +1266,10 @@ __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int 
     double inv_bw = 0.0;
     if (a.tab) inv_bw = interp_inv_bw(a.bc, ns, a.f32);
     __syncthreads();
-    fft_lds_batch<true>(buf, M1, NC, ld, st1, twl, M1, kColThreads);
+    if constexpr (FL > 0)
+        wave_fft_batch<FL, true>(buf, NC, ld, twl, kColThreads);
+    else
+        fft_lds_batch<true>(buf, M1, NC, ld, st1, twl, M1, kColThreads);
     // V = -(charge*e*ratio * irfft(...))[:n_out]   (impedance.py:475-489)
     const double fct = 1.0 / (double)M;
     if (a.tab && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1324,6 +1425,39 @@ void fft2_plan_destroy(Fft2Plan *p) {
     *p = Fft2Plan();
 }
 
+// The pass kernels for the plan's lengths: one wavefront a transform where
+// a length is a power of two in [16, 1024], workgroup-wide stages otherwise.
+struct Fft2Kernels {
+    decltype(&fft2_a_kernel<0>) a;
+    decltype(&fft2_b_kernel<0>) b;
+    decltype(&fft2_c_kernel<0>) c;
+};
+
+static Fft2Kernels fft2_kernels(int M1, int M2) {
+    Fft2Kernels k{fft2_a_kernel<0>, fft2_b_kernel<0>, fft2_c_kernel<0>};
+    switch (M1) {
+        case 16: k.a = fft2_a_kernel<16>; k.c = fft2_c_kernel<16>; break;
+        case 32: k.a = fft2_a_kernel<32>; k.c = fft2_c_kernel<32>; break;
+        case 64: k.a = fft2_a_kernel<64>; k.c = fft2_c_kernel<64>; break;
+        case 128: k.a = fft2_a_kernel<128>; k.c = fft2_c_kernel<128>; break;
+        case 256: k.a = fft2_a_kernel<256>; k.c = fft2_c_kernel<256>; break;
+        case 512: k.a = fft2_a_kernel<512>; k.c = fft2_c_kernel<512>; break;
+        case 1024: k.a = fft2_a_kernel<1024>; k.c = fft2_c_kernel<1024>; break;
+        default: break;
+    }
+    switch (M2) {
+        case 16: k.b = fft2_b_kernel<16>; break;
+        case 32: k.b = fft2_b_kernel<32>; break;
+        case 64: k.b = fft2_b_kernel<64>; break;
+        case 128: k.b = fft2_b_kernel<128>; break;
+        case 256: k.b = fft2_b_kernel<256>; break;
+        case 512: k.b = fft2_b_kernel<512>; break;
+        case 1024: k.b = fft2_b_kernel<1024>; break;
+        default: break;
+    }
+    return k;
+}
+
 int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t s,
                             const KernelEvents &ev) {
     if (!p.M) return blmi_set_error(BLMI_ERR_ARG, "four-step plan not created");
@@ -1345,18 +1479,19 @@ int launch_induced_fourstep(const InducedArgs &a, const Fft2Plan &p, hipStream_t
                          (a.part ? sizeof(unsigned) * 8 * kColThreads : 0);
     const size_t lds_c = sizeof(double2) * ((p.cols + 1) * (size_t)(p.M1 + 1) + p.M1);
     const size_t lds_b = sizeof(double2) * 3 * (size_t)p.M2;
-    int rc = set_lds_limit((const void *)fft2_a_kernel, lds_a);
-    if (!rc) rc = set_lds_limit((const void *)fft2_c_kernel, lds_c);
-    if (!rc) rc = set_lds_limit((const void *)fft2_b_kernel, lds_b);
+    const Fft2Kernels k = fft2_kernels(p.M1, p.M2);
+    int rc = set_lds_limit((const void *)k.a, lds_a);
+    if (!rc) rc = set_lds_limit((const void *)k.c, lds_c);
+    if (!rc) rc = set_lds_limit((const void *)k.b, lds_b);
     if (rc) return rc;
     const StageList st1 = stage_list(p.r1, p.n1), st2 = stage_list(p.r2, p.n2);
     const int cols = p.M2 / p.cols;
     const KernelEvents first{ev.start, nullptr}, last{nullptr, ev.stop};
-    launch(fft2_a_kernel, dim3(p.M2 / cols_a), dim3(kColThreads), lds_a, s, first, a, p.M1, p.M2,
+    launch(k.a, dim3(p.M2 / cols_a), dim3(kColThreads), lds_a, s, first, a, p.M1, p.M2,
            cols_a, st1, p.twM, p.twA, p.T);
-    hipLaunchKernelGGL(fft2_b_kernel, dim3(p.M1 / 2 + 1), dim3(p.threads_b), lds_b, s, a, p.M1,
+    hipLaunchKernelGGL(k.b, dim3(p.M1 / 2 + 1), dim3(p.threads_b), lds_b, s, a, p.M1,
                        p.M2, st2, p.twM, p.twB, p.T, p.S);
-    launch(fft2_c_kernel, dim3(cols), dim3(kColThreads), lds_c, s, last, a, p.M1, p.M2, p.cols,
+    launch(k.c, dim3(cols), dim3(kColThreads), lds_c, s, last, a, p.M1, p.M2, p.cols,
            st1, p.twM, p.S);
     return blmi_check_launch("induced_fourstep");
 }
