@@ -353,11 +353,17 @@ __global__ void irfft_lds_kernel(const double2 *__restrict__ in, int64_t m_in, d
 // ---------------------------------------------------------------------------
 // The profile is read from a.src, or as the sum of a.groups copies of u32
 // counts at cin (then cleared at cclr, which may equal cin: each element is
-// read before it is cleared, by the same thread).
+// read before it is cleared, by the same thread).  FL: M as a constant when
+// it is a power of two in [16, 1024] (one wavefront a transform, twiddles
+// staged in LDS after the data), else 0 (workgroup-wide stages).
+template <int FL>
 __device__ void induced_lds_body(const InducedArgs &a, const unsigned *cin, unsigned *cclr, int M,
                                  const StageList &st, const double2 *__restrict__ tw,
                                  const double2 *__restrict__ tw2) {
     extern __shared__ double2 buf[];
+    double2 *twl = buf + M;
+    if constexpr (FL > 0)
+        for (int j = threadIdx.x; j < M; j += blockDim.x) twl[j] = tw[j];
     const int ns = a.n_slices;
     // load the profile as z[m] = p[2m] + i p[2m+1], zero padded to n = 2M
     auto prof = [&](int j) -> double {
@@ -384,7 +390,10 @@ __device__ void induced_lds_body(const InducedArgs &a, const unsigned *cin, unsi
         buf[m] = make_double2(p0, p1);
     }
     __syncthreads();
-    fft_lds<false>(buf, M, st, tw);
+    if constexpr (FL > 0)
+        wave_fft_batch<FL, false>(buf, 1, M, twl, (int)blockDim.x);
+    else
+        fft_lds<false>(buf, M, st, tw);
     // spectrum -> times impedance -> half-length inverse input, pairwise.
     // All pairs are read into registers before any is overwritten.
     constexpr int PMAX = 8;  // host guarantees M/2+1 <= PMAX * blockDim.x
@@ -416,7 +425,10 @@ __device__ void induced_lds_body(const InducedArgs &a, const unsigned *cin, unsi
         }
     }
     __syncthreads();
-    fft_lds<true>(buf, M, st, tw);
+    if constexpr (FL > 0)
+        wave_fft_batch<FL, true>(buf, 1, M, twl, (int)blockDim.x);
+    else
+        fft_lds<true>(buf, M, st, tw);
     // V = -(charge*e*ratio * irfft(...))[:n_slices]   (impedance.py:475-489)
     // Each thread rewrites only its own buf[m] as the two reals vl[2m], vl[2m+1].
     // U carries the 1/2 of the split, so 1/M here gives numpy's 1/n overall
@@ -443,6 +455,7 @@ __device__ void induced_lds_body(const InducedArgs &a, const unsigned *cin, unsi
     }
 }
 
+template <int FL>
 __global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
                                    const double2 *__restrict__ tw, const double2 *__restrict__ tw2) {
     const unsigned *cin = nullptr;
@@ -452,16 +465,17 @@ __global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
         cin = (t & 1) ? a.counts1 : a.counts0;
         cclr = (t & 1) ? a.counts0 : a.counts1;
     }
-    induced_lds_body(a, cin, cclr, M, st, tw, tw2);
+    induced_lds_body<FL>(a, cin, cclr, M, st, tw, tw2);
 }
 
 // One workgroup per bunch: bunch b's arguments are args[b]; its counts (one
 // copy) are read and cleared in place.
+template <int FL>
 __global__ void induced_lds_batch_kernel(const InducedArgs *__restrict__ args, int M, StageList st,
                                          const double2 *__restrict__ tw,
                                          const double2 *__restrict__ tw2) {
     const InducedArgs a = args[blockIdx.x];
-    induced_lds_body(a, a.counts0, a.counts0, M, st, tw, tw2);
+    induced_lds_body<FL>(a, a.counts0, a.counts0, M, st, tw, tw2);
 }
 
 // ---------------------------------------------------------------------------
@@ -923,6 +937,37 @@ int launch_irfft(const double2 *in, int64_t m_in, double *out, int64_t n_out, co
     return blmi_check_launch("irfft");
 }
 
+// The one-workgroup induced-voltage kernels for a plan's length: one
+// wavefront a transform (twiddles in LDS after the data, twice the LDS)
+// where M is a power of two in [16, 1024].
+static int wave_len(int M) { return M >= 16 && M <= 1024 && (M & (M - 1)) == 0 ? M : 0; }
+
+template <int FL>
+static void pick_lds_kernels(int M, decltype(&induced_lds_kernel<0>) &one,
+                             decltype(&induced_lds_batch_kernel<0>) &batch) {
+    if constexpr (FL <= 1024) {
+        if (M == FL) {
+            one = induced_lds_kernel<FL>;
+            batch = induced_lds_batch_kernel<FL>;
+            return;
+        }
+        pick_lds_kernels<FL * 2>(M, one, batch);
+    }
+}
+
+struct LdsKernels {
+    decltype(&induced_lds_kernel<0>) one = induced_lds_kernel<0>;
+    decltype(&induced_lds_batch_kernel<0>) batch = induced_lds_batch_kernel<0>;
+    size_t lds = 0;
+};
+
+static LdsKernels lds_kernels(int M) {
+    LdsKernels k;
+    if (wave_len(M)) pick_lds_kernels<16>(M, k.one, k.batch);
+    k.lds = sizeof(double2) * (size_t)M * (wave_len(M) ? 2 : 1);
+    return k;
+}
+
 int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream_t s,
                                const KernelEvents &ev) {
     if (p.M / 2 + 1 > 8 * p.threads || p.M > 16 * p.threads)
@@ -931,11 +976,10 @@ int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream
     if (a.n_slices > p.n_real || a.n_out < 0 || a.n_out > p.n_real)
         return blmi_set_error(BLMI_ERR_ARG, "n_slices %d / n_out %d exceed n_fft %d", a.n_slices,
                               a.n_out, p.n_real);
-    const size_t lds = sizeof(double2) * p.M;
-    int rc = set_lds_limit((const void *)induced_lds_kernel, lds);
+    const LdsKernels k = lds_kernels(p.M);
+    int rc = set_lds_limit((const void *)k.one, k.lds);
     if (rc) return rc;
-    launch(induced_lds_kernel, dim3(1), dim3(p.threads), lds, s, ev, a, p.M, stages_of(p), p.tw,
-           p.tw2);
+    launch(k.one, dim3(1), dim3(p.threads), k.lds, s, ev, a, p.M, stages_of(p), p.tw, p.tw2);
     return blmi_check_launch("induced_voltage");
 }
 
@@ -947,11 +991,11 @@ int launch_induced_lds_batch(const InducedArgs *d_args, int n_bunches, int n_sli
     if (n_slices > p.n_real)
         return blmi_set_error(BLMI_ERR_ARG, "n_slices %d exceeds n_fft %d", n_slices, p.n_real);
     if (n_bunches <= 0) return BLMI_OK;
-    const size_t lds = sizeof(double2) * p.M;
-    int rc = set_lds_limit((const void *)induced_lds_batch_kernel, lds);
+    const LdsKernels k = lds_kernels(p.M);
+    int rc = set_lds_limit((const void *)k.batch, k.lds);
     if (rc) return rc;
-    launch(induced_lds_batch_kernel, dim3(n_bunches), dim3(p.threads), lds, s, ev, d_args, p.M,
-           stages_of(p), p.tw, p.tw2);
+    launch(k.batch, dim3(n_bunches), dim3(p.threads), k.lds, s, ev, d_args, p.M, stages_of(p), p.tw,
+           p.tw2);
     return blmi_check_launch("induced_voltage_batch");
 }
 
