@@ -252,10 +252,11 @@ def cpu_baseline_adjoint(case, target_s: float) -> dict:
 # ---------------------------------------------------------------------------
 # FP64 operations of one plain c1 macroparticle-turn (1 RF system, simple
 # drift), contraction off: the sine's argument (2), Cephes reduction (1 + 6),
-# r^2 (1), the two degree-6 polynomials (20), their sine and cosine forms
-# (3 + 5), V sin and the sum (2), charge and acceleration (2), dE += (1),
+# r^2 (1), the octant's degree-6 polynomial (10; the kernel reads the
+# branch's coefficients from LDS instead of evaluating both), its final form
+# A + B P (5), V sin and the sum (2), charge and acceleration (2), dE += (1),
 # drift (2)
-C1_FP64_OPS = 45
+C1_FP64_OPS = 32
 FP64_PEAK_TFLOPS = 78.6  # published, an FMA counted as two operations
 
 

@@ -121,6 +121,56 @@ __device__ __forceinline__ double cephes_sin(double x) {
                             __double2loint(out));
 }
 
+// cephes_sin with one polynomial a lane, for kernels that keep the
+// coefficient table in LDS (kCephesTabDoubles doubles, 16-byte aligned:
+// kS0..kS5 of the sine branch, then kC0..kC5 of the cosine branch;
+// cephes_tab_fill writes it).  The lane's octant picks its branch's six
+// coefficients, evaluated in cephes_reduce's Horner order; the branch's last
+// steps, r + (r zz) P and (1 - zz/2) + (zz zz) P, become one A + B P with
+// A and B selected.  Every operation is one cephes_sin performs on the same
+// operands, so the result is bit-identical, and no lane evaluates the
+// polynomial its octant discards.
+constexpr int kCephesTabDoubles = 12;
+using LdsConstDoublePtr = __attribute__((address_space(3))) const double *;
+using LdsDoublePtr = __attribute__((address_space(3))) double *;
+
+__device__ __forceinline__ void cephes_tab_fill(LdsDoublePtr tab) {
+    const int i = (int)threadIdx.x;
+    if (i < kCephesTabDoubles) {
+        const double s[6] = {CephesD::kS0, CephesD::kS1, CephesD::kS2,
+                             CephesD::kS3, CephesD::kS4, CephesD::kS5};
+        const double c[6] = {CephesD::kC0, CephesD::kC1, CephesD::kC2,
+                             CephesD::kC3, CephesD::kC4, CephesD::kC5};
+        tab[i] = i < 6 ? s[i] : c[i - 6];
+    }
+}
+
+__device__ __forceinline__ double cephes_sin_tab(double x, LdsConstDoublePtr tab) {
+    const double ax = fabs(x);
+    int j = (int)(CephesD::kFourOverPi * ax);
+    j = (j + 1) & ~1;
+    const double y = (double)j;
+    const double r = ((ax - y * CephesD::kDP1) - y * CephesD::kDP2) - y * CephesD::kDP3;
+    const double zz = r * r;
+    const bool cosb = ((j - 2) & 2) == 0;
+    using D2 = double __attribute__((ext_vector_type(2)));
+    using LdsD2 = __attribute__((address_space(3))) const D2 *;
+    const LdsD2 c = reinterpret_cast<LdsD2>(tab + (cosb ? 6 : 0));
+    const D2 c01 = c[0], c23 = c[1], c45 = c[2];
+    double p = c01.x;
+    p = p * zz; p = p + c01.y;
+    p = p * zz; p = p + c23.x;
+    p = p * zz; p = p + c23.y;
+    p = p * zz; p = p + c45.x;
+    p = p * zz; p = p + c45.y;
+    const double A = cosb ? 1.0 - zz * 0.5 : r;
+    const double B = (cosb ? zz : r) * zz;
+    const double out = A + B * p;
+    const unsigned flip = ((unsigned)j << 29) ^ (x < 0.0 ? 0x80000000u : 0u);
+    return __hiloint2double(__double2hiint(out) ^ (int)(flip & 0x80000000u),
+                            __double2loint(out));
+}
+
 // Both functions from one reduction (the adjoint needs the cosine).
 __device__ __forceinline__ void cephes_sincos(double x, double &s, double &c) {
     double sp, cp;
@@ -195,6 +245,16 @@ template <> struct Trig<double> {
 };
 template <> struct Trig<float> {
     __device__ __forceinline__ static float sin(float x) { return cephes_sinf(x); }
+};
+// the sine with an LDS coefficient table (FP64; the FP32 sine ignores it)
+template <typename T> struct TrigTab;
+template <> struct TrigTab<double> {
+    __device__ __forceinline__ static double sin(double x, LdsConstDoublePtr tab) {
+        return cephes_sin_tab(x, tab);
+    }
+};
+template <> struct TrigTab<float> {
+    __device__ __forceinline__ static float sin(float x, LdsConstDoublePtr) { return cephes_sinf(x); }
 };
 
 // linear_interp_kick.cpp:33-48: the interpolation table of a voltage on

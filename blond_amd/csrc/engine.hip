@@ -204,6 +204,9 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
                                                     int64_t n, TurnArgs<T> a) {
     using V2 = typename Vec2<T>::type;
     extern __shared__ unsigned lds_hist[];
+    __shared__ __align__(16) double sin_tab[kCephesTabDoubles];
+    const LdsConstDoublePtr stab = (LdsConstDoublePtr)sin_tab;
+    cephes_tab_fill((LdsDoublePtr)sin_tab);
     int t = *a.base + a.off;
     if (t < 0) t = 0;
     if (t > a.last_turn) t = a.last_turn;  // host checks the range; never read past the tables
@@ -274,8 +277,8 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
             for (int k = threadIdx.x; k < a.n_tab; k += blockDim.x) ltab[k] = a.tab[k];
         if (tw_on)
             for (int k = threadIdx.x; k < a.tw_n; k += blockDim.x) ltab[k] = a.tab[a.tw_lo + k];
-        __syncthreads();
     }
+    __syncthreads();  // the LDS set-up, and the sine's coefficient table
     // the table window (lds_hist itself, never read, when there is none)
     const double2 *ltw = tw_on ? ltab : reinterpret_cast<const double2 *>(lds_hist);
     // linear_interp_kick.cpp:44-64 (acceleration kick 0): the table entry of
@@ -325,7 +328,7 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
     // keep the order e + interp + rf
     auto finish = [&](T &x, T &e, const Tab &g, auto nrf) -> int {
         constexpr int NRF = decltype(nrf)::value;
-        const T rf = rfp.template kick_n<NRF>(x, q_charge, q_acc);
+        const T rf = rfp.template kick_tab<NRF>(x, q_charge, q_acc, stab);
         if (INTERP) {
             // a select, not a branch: nothing waits on the gather before the sines
             const T ei = e + (x * (T)g.k.x + (T)g.k.y);
@@ -478,10 +481,12 @@ __global__ void __launch_bounds__(1024) turns_kernel(T *__restrict__ dt, T *__re
         publish_range(prev, a.profile, a.n_slices, a.groups, b0, b1);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) a.pending[tl & 1] = tl;
-    if (LDSH) {
+    __shared__ __align__(16) double sin_tab[kCephesTabDoubles];
+    const LdsConstDoublePtr stab = (LdsConstDoublePtr)sin_tab;
+    cephes_tab_fill((LdsDoublePtr)sin_tab);
+    if (LDSH)
         for (int b = threadIdx.x; b < a.n_slices; b += blockDim.x) lds_hist[b] = 0u;
-        __syncthreads();
-    }
+    __syncthreads();
     const T q_charge = a.charge, q_cl = a.cut_left, q_ibw = a.inv_bw, q_ns = (T)a.n_slices;
     auto count = [&](int bin) {
         if (LDSH)
@@ -541,7 +546,7 @@ __global__ void __launch_bounds__(1024) turns_kernel(T *__restrict__ dt, T *__re
                         T s = T(0);
 #pragma unroll
                         for (int r = 0; r < NRF; ++r)
-                            s = s + p.V[r] * Trig<T>::sin(p.w[r] * x[j] + p.phi[r]);
+                            s = s + p.V[r] * TrigTab<T>::sin(p.w[r] * x[j] + p.phi[r], stab);
                         e[j] = e[j] + (q_charge * s + p.acc);
                         x[j] = x[j] + e_drift(p.dc, e[j], SOLVER);
                     }
@@ -1116,8 +1121,9 @@ bool Engine::part_perm(int *shift, int *lshift, int *m1_lo) const {
     const int L = W / M2;
     if (L < 4 || (L & (L - 1)) || L > plan2.M1 || kColThreads % (plan2.cols * (L / 4)))
         return false;
-    // the padded LDS rows must still fit
-    if ((size_t)(W + W / M2) * sizeof(unsigned) > (size_t)device_max_lds()) return false;
+    // the padded LDS rows must still fit beside the kernel's static arrays
+    if ((size_t)(W + W / M2) * sizeof(unsigned) + kStaticLdsMargin > (size_t)device_max_lds())
+        return false;
     *shift = __builtin_ctz(M2);
     *lshift = __builtin_ctz(L);
     int lw = 0;

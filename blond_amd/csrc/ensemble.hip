@@ -61,6 +61,10 @@ __global__ void __launch_bounds__(1024) ensemble_turn_kernel(double *__restrict_
                                                              EnsTurnArgs a) {
     extern __shared__ unsigned lds_hist[];
     double2 *ltab = reinterpret_cast<double2 *>(lds_hist + ((a.n_slices + 3) & ~3));
+    __shared__ __align__(16) double sin_tab[kCephesTabDoubles];
+    const LdsConstDoublePtr stab = (LdsConstDoublePtr)sin_tab;
+    cephes_tab_fill((LdsDoublePtr)sin_tab);
+    __syncthreads();
     int t = *a.base + a.off;
     if (t < 0) t = 0;
     if (t > a.last_turn) t = a.last_turn;
@@ -101,7 +105,7 @@ __global__ void __launch_bounds__(1024) ensemble_turn_kernel(double *__restrict_
                     e = e + (x * tk.x + tk.y);
                 }
             }
-            e = e + rfp.kick(x, a.charge, acc);
+            e = e + rfp.template kick_tab<-1>(x, a.charge, acc, stab);
             x = x + e_drift(dc, e, SOLVER);
             if (HIST) {
                 const double f = floor((x - cl) * hbw);
