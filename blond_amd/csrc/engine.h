@@ -27,6 +27,8 @@ constexpr size_t kSinkBytes = 64 * 2 * 2 * sizeof(double);
 constexpr int kLdsTableBudget = 144 * 1024;
 // LDS kept free of dynamic use beside the particle kernel's static arrays
 constexpr size_t kStaticLdsMargin = 1024;
+// MI355X's last-level (Infinity) cache: particle arrays above it stream from HBM
+constexpr double kLastLevelCacheBytes = 256.0 * 1024 * 1024;
 // Bin-ordered working set (slice counts above kSortWindow): bins of a
 // workgroup's LDS window (u32 histogram + interpolation table) and the
 // coarse-digit width of the counting sort (2^shift bins).

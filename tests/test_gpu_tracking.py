@@ -113,8 +113,12 @@ def test_intensity_turns_fused_2e16_slices():
     _check_intensity(w, ref, 1e-9)
 
 
-def test_intensity_first_turn_exact_2e16_slices():
-    w = build_workload("c3", n_particles=300001, n_turns=4)
+@pytest.mark.parametrize("n_particles", [300001, 17_000_001])
+def test_intensity_first_turn_exact_2e16_slices(n_particles):
+    """The first fused c3 turn against the oracle's; at 17e6 macroparticles
+    (272 MB of coordinates, above the last-level cache) the particle kernel
+    reads and writes them with non-temporal loads and stores."""
+    w = build_workload("c3", n_particles=n_particles, n_turns=4)
     ref = orc.track(w.snapshot(), 1)
     w.fused(1)
     np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
