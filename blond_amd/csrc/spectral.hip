@@ -455,8 +455,13 @@ __device__ void induced_lds_body(const InducedArgs &a, const unsigned *cin, unsi
     }
 }
 
+// FL > 0 launches take 512 threads (lds_threads of M <= 1024): the bound
+// leaves their registers room (no scratch spills)
+constexpr int kLdsWaveThreads = 512;
+
 template <int FL>
-__global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
+__global__ void __launch_bounds__(FL > 0 ? kLdsWaveThreads : 1024)
+    induced_lds_kernel(InducedArgs a, int M, StageList st,
                                    const double2 *__restrict__ tw, const double2 *__restrict__ tw2) {
     const unsigned *cin = nullptr;
     unsigned *cclr = nullptr;
@@ -471,7 +476,8 @@ __global__ void induced_lds_kernel(InducedArgs a, int M, StageList st,
 // One workgroup per bunch: bunch b's arguments are args[b]; its counts (one
 // copy) are read and cleared in place.
 template <int FL>
-__global__ void induced_lds_batch_kernel(const InducedArgs *__restrict__ args, int M, StageList st,
+__global__ void __launch_bounds__(FL > 0 ? kLdsWaveThreads : 1024)
+    induced_lds_batch_kernel(const InducedArgs *__restrict__ args, int M, StageList st,
                                          const double2 *__restrict__ tw,
                                          const double2 *__restrict__ tw2) {
     const InducedArgs a = args[blockIdx.x];
@@ -961,10 +967,11 @@ struct LdsKernels {
     size_t lds = 0;
 };
 
-static LdsKernels lds_kernels(int M) {
+static LdsKernels lds_kernels(int M, int threads) {
     LdsKernels k;
-    if (wave_len(M)) pick_lds_kernels<16>(M, k.one, k.batch);
-    k.lds = sizeof(double2) * (size_t)M * (wave_len(M) ? 2 : 1);
+    const bool wave = wave_len(M) && threads == kLdsWaveThreads;
+    if (wave) pick_lds_kernels<16>(M, k.one, k.batch);
+    k.lds = sizeof(double2) * (size_t)M * (wave ? 2 : 1);
     return k;
 }
 
@@ -976,7 +983,7 @@ int launch_induced_voltage_lds(const InducedArgs &a, const FftPlan &p, hipStream
     if (a.n_slices > p.n_real || a.n_out < 0 || a.n_out > p.n_real)
         return blmi_set_error(BLMI_ERR_ARG, "n_slices %d / n_out %d exceed n_fft %d", a.n_slices,
                               a.n_out, p.n_real);
-    const LdsKernels k = lds_kernels(p.M);
+    const LdsKernels k = lds_kernels(p.M, p.threads);
     int rc = set_lds_limit((const void *)k.one, k.lds);
     if (rc) return rc;
     launch(k.one, dim3(1), dim3(p.threads), k.lds, s, ev, a, p.M, stages_of(p), p.tw, p.tw2);
@@ -991,7 +998,7 @@ int launch_induced_lds_batch(const InducedArgs *d_args, int n_bunches, int n_sli
     if (n_slices > p.n_real)
         return blmi_set_error(BLMI_ERR_ARG, "n_slices %d exceeds n_fft %d", n_slices, p.n_real);
     if (n_bunches <= 0) return BLMI_OK;
-    const LdsKernels k = lds_kernels(p.M);
+    const LdsKernels k = lds_kernels(p.M, p.threads);
     int rc = set_lds_limit((const void *)k.batch, k.lds);
     if (rc) return rc;
     launch(k.batch, dim3(n_bunches), dim3(p.threads), k.lds, s, ev, d_args, p.M, stages_of(p), p.tw,

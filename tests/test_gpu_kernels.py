@@ -221,12 +221,13 @@ def test_rfft_irfft_vs_numpy(n):
 
 @pytest.mark.parametrize("n_slices,n_fft", [(40, 64), (1000, 1024), (1500, 2048), (1000, 1080),
                                             (4096, 16384), (65536, 131072), (20000, 48000),
-                                            (1000, 262144)])
+                                            (1000, 262144), (100000, 2097152)])
 def test_induced_voltage_within_1e10_of_max(n_slices, n_fft):
     """One-workgroup LDS transform up to its limit (one wavefront a
     transform for half-lengths 32, 512 and 1024, workgroup-wide stages for
     540 and 8192), the four-step transform beyond it (131072 = the
-    2^16-slice workload, 48000 = 2^7 3 5^3, 262144 with a short profile)."""
+    2^16-slice workload, 256 x 256; 48000 = 2^7 3 5^3, workgroup-wide
+    stages; 262144 with a short profile, 512 x 256; 2^21, 1024 x 1024)."""
     dt, _ = bunch(400000)
     prof = np.zeros(n_slices)
     ORC.slice_beam(dt, prof, 0.0, 2.5e-9)
