@@ -279,6 +279,17 @@ __device__ __forceinline__ double2 interp_entry(double v0, double v1, double bcb
     return make_double2(k, (charge * v0 - bcb * k) + 0.0);
 }
 
+// A table entry as stored: a (slope, intercept) pair of doubles, or with f32
+// of floats, which hold the single build's FP32 values exactly and halve the
+// bytes the kick gathers (the table then holds float2 entries; its
+// (bin_centers[0], inv_bw) parameters stay doubles after 2 * n_slices doubles)
+__device__ __forceinline__ void store_interp_entry(double2 *tab, int i, double2 e, bool f32) {
+    if (f32)
+        reinterpret_cast<float2 *>(tab)[i] = make_float2((float)e.x, (float)e.y);
+    else
+        tab[i] = e;
+}
+
 // Agent-scope atomics (all XCDs of the device see one coherent value).
 __device__ __forceinline__ void atomic_add_agent(unsigned *p, unsigned v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

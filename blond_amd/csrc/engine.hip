@@ -199,6 +199,11 @@ __global__ void __launch_bounds__(512) partials_reduce_kernel(const unsigned *__
 template <typename V>
 using GlobalPtr = __attribute__((address_space(1))) V *;
 
+// an interpolation-table entry as the particle kernel reads it: (slope,
+// intercept) doubles, or floats for FP32 particles (store_interp_entry)
+template <typename T> struct TabEntry { using type = double2; };
+template <> struct TabEntry<float> { using type = float2; };
+
 // NT: the coordinates are read and written with non-temporal loads and
 // stores (for bunches larger than the last-level cache, where they stream
 // from HBM either way and, kept out of the caches' recently-used lines,
@@ -272,24 +277,26 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
     const int lwords = LDSH == 2 ? lds_word(hwords, a.part_shift) : hwords;
     // after them in LDS: INTERP == 2 the whole table; INTERP == 1 with u16
     // rows a window of it (entries tw_lo .. tw_lo + tw_n - 1)
-    double2 *ltab = reinterpret_cast<double2 *>(lds_hist + ((lwords + 3) & ~3));
+    using TabE = typename TabEntry<T>::type;
+    const TabE *gtab = reinterpret_cast<const TabE *>(a.tab);
+    TabE *ltab = reinterpret_cast<TabE *>(lds_hist + ((lwords + 3) & ~3));
     constexpr bool kTabWindow = INTERP == 1 && HIST && LDSH == 2;
     const bool tw_on = kTabWindow && a.tw_n > 0;
     if ((HIST && LDSH) || INTERP == 2) {
         for (int b = threadIdx.x; b < lwords; b += blockDim.x) lds_hist[b] = 0u;
         if (INTERP == 2)
-            for (int k = threadIdx.x; k < a.n_tab; k += blockDim.x) ltab[k] = a.tab[k];
+            for (int k = threadIdx.x; k < a.n_tab; k += blockDim.x) ltab[k] = gtab[k];
         if (tw_on)
-            for (int k = threadIdx.x; k < a.tw_n; k += blockDim.x) ltab[k] = a.tab[a.tw_lo + k];
+            for (int k = threadIdx.x; k < a.tw_n; k += blockDim.x) ltab[k] = gtab[a.tw_lo + k];
     }
     __syncthreads();  // the LDS set-up, and the sine's coefficient table
     // the table window (lds_hist itself, never read, when there is none)
-    const double2 *ltw = tw_on ? ltab : reinterpret_cast<const double2 *>(lds_hist);
+    const TabE *ltw = tw_on ? ltab : reinterpret_cast<const TabE *>(lds_hist);
     // linear_interp_kick.cpp:44-64 (acceleration kick 0): the table entry of
     // x, loaded unconditionally (bin 0 stands in outside the table; `in`
     // says whether the kick applies)
     struct Tab {
-        double2 k;
+        TabE k;
         bool in;
     };
     // the per-particle constants, moved into vector registers like the RF
@@ -307,7 +314,7 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
         if (HIST && LDSH == 2) asm volatile("" : "+v"(q_wlo), "+v"(q_wn), "+v"(q_cntp));
     };
     auto gather = [&](T x) -> Tab {
-        Tab g{make_double2(0.0, 0.0), false};
+        Tab g{TabE{}, false};
         if (INTERP) {
             const T f = floor((x - q_bc0) * q_tibw);
             g.in = f >= T(0) && f < q_ntab;
@@ -317,13 +324,15 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
                 // all ask L2 for entry 0, one request for all of them
                 const int kw = fi - q_twlo;
                 const bool intw = (unsigned)kw < (unsigned)q_twn;
-                using NV2 = double __attribute__((ext_vector_type(2)));
-                const NV2 kv = reinterpret_cast<GlobalPtr<const NV2>>(q_tabp)[intw ? 0 : fi];
-                const double2 kg = make_double2(kv.x, kv.y);
-                const double2 kl = ltw[intw ? kw : 0];
+                using NE = T __attribute__((ext_vector_type(2)));
+                const NE kv = reinterpret_cast<GlobalPtr<const NE>>(q_tabp)[intw ? 0 : fi];
+                TabE kg;
+                kg.x = kv.x;
+                kg.y = kv.y;
+                const TabE kl = ltw[intw ? kw : 0];
                 g.k = intw ? kl : kg;
             } else {
-                g.k = INTERP == 2 ? ltab[fi] : a.tab[fi];
+                g.k = INTERP == 2 ? ltab[fi] : gtab[fi];
             }
         }
         return g;
@@ -1408,7 +1417,8 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
     }
     int imode = interp ? 1 : 0;
     // stage the interpolation table in LDS when it fits beside a u32 histogram
-    const size_t tab_bytes = sizeof(double2) * (size_t)(e.n_slices > 1 ? e.n_slices - 1 : 0);
+    const size_t entry = sizeof(typename TabEntry<T>::type);
+    const size_t tab_bytes = entry * (size_t)(e.n_slices > 1 ? e.n_slices - 1 : 0);
     const size_t hist_bytes = ((lds + 15) / 16) * 16;
     if (interp && ldsh != 2 && hist_bytes + tab_bytes <= (size_t)kLdsTableBudget) {
         imode = 2;
@@ -1420,14 +1430,14 @@ static int launch_turn_t(const Engine &e, T *x, T *de, const T *rf, const T *acc
         const size_t base = (lds + 15) / 16 * 16;
         const size_t cap = (size_t)device_max_lds();
         const size_t avail = cap > base + kStaticLdsMargin ? cap - base - kStaticLdsMargin : 0;
-        int tw = (int)std::min<size_t>(avail / sizeof(double2), (size_t)a.n_tab) & ~63;
+        int tw = (int)std::min<size_t>(avail / entry, (size_t)a.n_tab) & ~63;
         if (tw >= 256) {
             const int c = a.win_bins ? a.win_lo + a.win_bins / 2 : a.n_tab / 2;
             int lo = c - tw / 2;
             lo = std::max(0, std::min(lo, a.n_tab - tw));
             a.tw_lo = lo;
             a.tw_n = tw;
-            lds = base + sizeof(double2) * (size_t)tw;
+            lds = base + entry * (size_t)tw;
         }
     }
     const bool publish = !e.intensity;

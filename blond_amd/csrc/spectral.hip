@@ -451,7 +451,9 @@ __device__ void induced_lds_body(const InducedArgs &a, const unsigned *cin, unsi
             a.tab_params[1] = inv_bw;
         }
         for (int i = threadIdx.x; i < ns - 1; i += blockDim.x)
-            a.tab[i] = interp_entry(vl[i], vl[i + 1], a.bc[i], a.charge, inv_bw, a.f32);
+            store_interp_entry(a.tab, i,
+                               interp_entry(vl[i], vl[i + 1], a.bc[i], a.charge, inv_bw, a.f32),
+                               a.f32);
     }
 }
 
@@ -724,8 +726,10 @@ __global__ void __launch_bounds__(TH) induced_direct_kernel(InducedArgs a) {
         if (threadIdx.x < C) {
             const int i = i0 + threadIdx.x;
             if (i < ns - 1)
-                a.tab[i] = interp_entry(vout[threadIdx.x], vout[threadIdx.x + 1], a.bc[i],
-                                        a.charge, inv_bw, a.f32);
+                store_interp_entry(a.tab, i,
+                                   interp_entry(vout[threadIdx.x], vout[threadIdx.x + 1], a.bc[i],
+                                                a.charge, inv_bw, a.f32),
+                                   a.f32);
         }
     }
 }
@@ -1336,14 +1340,16 @@ __global__ void __launch_bounds__(kColThreads) fft2_c_kernel(InducedArgs a, int 
         if (2 * m + 1 < a.n_out) a.volt[2 * m + 1] = v1;
         if (a.tab && 2 * m < ns - 1) {
             const double b0 = one ? bc_pre[0] : a.bc[2 * m];
-            a.tab[2 * m] = interp_entry(v0, v1, b0, a.charge, inv_bw, a.f32);
+            store_interp_entry(a.tab, 2 * m, interp_entry(v0, v1, b0, a.charge, inv_bw, a.f32),
+                               a.f32);
             if (2 * m + 1 < ns - 1) {
                 // V[2m+2]: next column, same row (column 0, next row after the last)
                 const double2 un = c + 1 < cols ? buf[(c + 1) * ld + m1]
                                    : buf[cols * ld + (last ? m1 + 1 : m1)];
                 const double v2 = -(a.scale * (un.x * fct));
                 const double b1 = one ? bc_pre[1] : a.bc[2 * m + 1];
-                a.tab[2 * m + 1] = interp_entry(v1, v2, b1, a.charge, inv_bw, a.f32);
+                store_interp_entry(a.tab, 2 * m + 1,
+                                   interp_entry(v1, v2, b1, a.charge, inv_bw, a.f32), a.f32);
             }
         }
     }
