@@ -329,7 +329,8 @@ def test_interpolation_intensity_turns(fused, n_slices, n_fft):
 
 
 @needs_ref32
-@pytest.mark.parametrize("name,n_slices,n_fft", [("c2", 4096, 16384), ("c3", 65536, 131072)])
+@pytest.mark.parametrize("name,n_slices,n_fft", [("c2", 4096, 16384), ("c2", 16384, 32768),
+                                                 ("c3", 65536, 131072)])
 def test_intensity_turn_f32_bit_exact_given_the_voltage(name, n_slices, n_fft):
     """FP32 mode, one intensity turn.  The reference's single build holds
     the induced voltage as a float array and forms the interpolation table
@@ -339,7 +340,9 @@ def test_intensity_turn_f32_bit_exact_given_the_voltage(name, n_slices, n_fft):
     (float) voltage gives the engine's coordinates bit for bit, and the
     profile is the single build's histogram.  (The voltage itself differs:
     numpy transforms the float profile in single precision, the engine in
-    double.)"""
+    double.)  The float table sits in LDS at 4096 slices, is gathered from
+    global memory at 16384 (u32 histogram, table too large for LDS) and
+    partly from an LDS window at 65536."""
     w = build_workload(name, n_particles=100001, n_turns=4, n_slices=n_slices, n_fft=n_fft)
     _to_f32(w)
     snap = w.snapshot()
