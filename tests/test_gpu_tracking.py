@@ -94,8 +94,12 @@ def _check_intensity(w, ref, rtol):
 
 @pytest.mark.parametrize("n_slices,n_fft,method", [(256, 1024, None), (4096, 16384, None),
                                                    (4096, 16384, "fft"),
-                                                   (4096, 16384, "fourstep")])
+                                                   (4096, 16384, "fourstep"),
+                                                   (256, 1024, "fourstep")])
 def test_intensity_turns_fused(n_slices, n_fft, method):
+    """Fused intensity turns against the oracle's; the four-step transform
+    at n_fft 16384 (64 x 128, 4 columns a workgroup) and 1024 (32 x 16, 8
+    columns a workgroup: pass C's 9 transforms on 8 wavefronts)."""
     w = build_workload("c2", n_particles=40001, n_turns=30, n_slices=n_slices, n_fft=n_fft)
     ref = orc.track(w.snapshot(), 20)
     ft = w.fused(20, method=method)
