@@ -134,15 +134,28 @@ constexpr int kCephesTabDoubles = 12;
 using LdsConstDoublePtr = __attribute__((address_space(3))) const double *;
 using LdsDoublePtr = __attribute__((address_space(3))) double *;
 
-__device__ __forceinline__ void cephes_tab_fill(LdsDoublePtr tab) {
-    const int i = (int)threadIdx.x;
-    if (i < kCephesTabDoubles) {
-        const double s[6] = {CephesD::kS0, CephesD::kS1, CephesD::kS2,
-                             CephesD::kS3, CephesD::kS4, CephesD::kS5};
-        const double c[6] = {CephesD::kC0, CephesD::kC1, CephesD::kC2,
-                             CephesD::kC3, CephesD::kC4, CephesD::kC5};
-        tab[i] = i < 6 ? s[i] : c[i - 6];
+__device__ __forceinline__ double cephes_coef(int i) {
+    // a select chain, not an indexed local array (which would live in
+    // scratch memory)
+    switch (i) {
+        case 0: return CephesD::kS0;
+        case 1: return CephesD::kS1;
+        case 2: return CephesD::kS2;
+        case 3: return CephesD::kS3;
+        case 4: return CephesD::kS4;
+        case 5: return CephesD::kS5;
+        case 6: return CephesD::kC0;
+        case 7: return CephesD::kC1;
+        case 8: return CephesD::kC2;
+        case 9: return CephesD::kC3;
+        case 10: return CephesD::kC4;
+        default: return CephesD::kC5;
     }
+}
+
+__device__ __forceinline__ void cephes_tab_fill(LdsDoublePtr tab) {
+    for (int i = (int)threadIdx.x; i < kCephesTabDoubles; i += (int)blockDim.x)
+        tab[i] = cephes_coef(i);
 }
 
 __device__ __forceinline__ double cephes_sin_tab(double x, LdsConstDoublePtr tab) {
