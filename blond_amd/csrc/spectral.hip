@@ -1169,10 +1169,12 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
     // with one output element a thread (the usual split), its twiddle is
     // loaded before the transforms, so that load's latency overlaps them
     const bool one = cols * M1 <= kColThreads;
-    double2 tw_out = make_double2(0.0, 0.0);
+    double tw_x = 0.0, tw_y = 0.0;  // two doubles: a double2 here went to scratch
     if (one && (int)threadIdx.x < cols * M1) {
         const int c = threadIdx.x % cols, k1 = threadIdx.x / cols;
-        tw_out = twA[(size_t)(c0 + c) * M1 + k1];
+        const double2 t = twA[(size_t)(c0 + c) * M1 + k1];
+        tw_x = t.x;
+        tw_y = t.y;
     }
     __syncthreads();
     if constexpr (FL > 0)
@@ -1182,7 +1184,7 @@ __global__ void __launch_bounds__(kColThreads) fft2_a_kernel(InducedArgs a, int 
     for (int j = threadIdx.x; j < cols * M1; j += kColThreads) {
         const int c = j % cols, k1 = j / cols;
         const int m2 = c0 + c;
-        const double2 w = one ? tw_out : twA[(size_t)m2 * M1 + k1];
+        const double2 w = one ? make_double2(tw_x, tw_y) : twA[(size_t)m2 * M1 + k1];
         T[(size_t)k1 * M2 + m2] = cmul(buf[c * ld + k1], w);
     }
 }
