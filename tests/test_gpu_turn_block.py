@@ -90,6 +90,18 @@ def test_turn_block_matches_the_oracle_on_a_ramp():
     np.testing.assert_array_equal(host(wa.profile.n_macroparticles), ref["profile"])
 
 
+@pytest.mark.parametrize("solver", ["simple", "exact"])
+def test_turn_block_three_rf_matches_the_oracle(solver):
+    """Three RF systems take the run-time-count sine loops of both kernels
+    (the multi-turn one's and the one-turn kernel's table sine): the
+    oracle's coordinates and profile after 113 turns."""
+    wa, snap, done = _compare_splits(lambda: _ring_case(3, solver))
+    ref = orc.track(snap, done, ORC)
+    np.testing.assert_array_equal(host(wa.beam.dt), ref["dt"])
+    np.testing.assert_array_equal(host(wa.beam.dE), ref["dE"])
+    np.testing.assert_array_equal(host(wa.profile.n_macroparticles), ref["profile"])
+
+
 def test_turn_block_c1_odd_count_matches_the_oracle():
     """c1 (100 slices) at an odd particle count, 113 turns in the splits."""
     wa, snap, done = _compare_splits(lambda: build_workload("c1", n_particles=100_001, n_turns=200))
