@@ -85,3 +85,23 @@ def test_plain_ensemble_tiny_bunches_bit_exact():
         np.testing.assert_array_equal(host(ens.trackers[k].beam.dE), ref["dE"])
         np.testing.assert_array_equal(host(ens.profiles[k].n_macroparticles), ref["profile"])
     et.close()
+
+
+def test_intensity_ensemble_tiny_bunches_first_turn():
+    """The same tiny bunches with their induced voltages (one workgroup a
+    bunch, one wavefront a 512-point transform): each bunch's first turn
+    against the oracle's (exact profile, induced voltage within 1e-10 of its
+    maximum, dE within 1e-12 of its maximum)."""
+    ens = build_ensemble(n_bunches=12, n_turns=20, n_slices=100, n_min=1, n_max=70, seed=5)
+    snaps = [ens.bunch(k).snapshot() for k in range(12)]
+    et = ens.tracker()
+    et.track(1)
+    for k in range(12):
+        ref = orc.track(snaps[k], 1)
+        np.testing.assert_array_equal(host(ens.profiles[k].n_macroparticles), ref["profile"])
+        vr = ref["induced_voltage"]
+        v = host(ens.tivs[k].induced_voltage)
+        assert np.max(np.abs(v - vr)) <= 1e-10 * max(np.abs(vr).max(), 1e-300)
+        sc = max(np.abs(ref["dE"]).max(), 1.0)
+        np.testing.assert_allclose(host(ens.trackers[k].beam.dE), ref["dE"], rtol=0, atol=1e-12 * sc)
+    et.close()
