@@ -6,7 +6,10 @@
 // fits the 160 KiB of a CDNA4 workgroup).  Each Stockham stage reads all of
 // its butterfly inputs into registers, synchronises, and writes the outputs
 // back to the same LDS buffer, so one buffer suffices.  Twiddles come from an
-// exact table computed on the host in extended precision.
+// exact table computed on the host in extended precision.  Power-of-two
+// lengths up to 1024 (the four-step passes' columns and rows, the batched
+// bunches' transforms) run as one wavefront a transform instead: radix-4
+// stages with compile-time lengths and no workgroup barrier (wave_fft).
 #include <math.h>
 
 #include <algorithm>
