@@ -246,10 +246,11 @@ __device__ __forceinline__ float cephes_sinf(float x) {
                       1.6666654611E-1f) * z * r) + r;
     const float c = ((2.443315711809948E-005f * z - 1.388731625493765E-003f) * z +
                      4.166664568298827E-002f) * z * z - 0.5f * z + 1.0f;
-    float out = (((j - 2) & 2) == 0) ? c : s;
-    if (j & 4) out = -out;
-    if (x < 0.0f) out = -out;
-    return out;
+    const float out = (((j - 2) & 2) == 0) ? c : s;
+    // the two sign changes (octant, then x < 0) as one: they cancel when
+    // both apply
+    const bool neg = ((j & 4) != 0) != (x < 0.0f);
+    return neg ? -out : out;
 }
 
 template <typename T> struct Trig;

@@ -366,10 +366,16 @@ __global__ void __launch_bounds__(1024) turn_kernel(T *__restrict__ dt, T *__res
     auto count = [&](int bin) {
         if (!HIST) return;
         if (LDSH == 2 && a.win_bins) {
-            // windowed rows: bins outside the window with device atomics
+            // windowed rows: bins outside the window with device atomics.
+            // No count (-1) is never inside: with 0 <= win_lo < n_slices,
+            // -1 - win_lo read unsigned is at least 2^32 - n_slices, above
+            // any window's width
             const int wb = bin - q_wlo;
-            const bool in = bin >= 0 && (unsigned)wb < (unsigned)q_wn;
-            lds_count<true>(lds_hist, in ? wb : -1, a.max_peel, a.part_shift);
+            const bool in = (unsigned)wb < (unsigned)q_wn;
+            if (a.max_peel > 0)
+                lds_count<true>(lds_hist, in ? wb : -1, a.max_peel, a.part_shift);
+            else if (in)
+                lds_add1<true>(lds_hist, wb, a.part_shift);
             if (bin >= 0 && !in)
                 __hip_atomic_fetch_add(&reinterpret_cast<GlobalPtr<unsigned>>(q_cntp)[bin], 1u,
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

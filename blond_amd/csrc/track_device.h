@@ -146,9 +146,24 @@ __device__ __forceinline__ T e_drift(const DriftCoeffs<T> &c, T e, int solver) {
 // odd stride, on distinct banks); pad_shift = 31 is the plain layout.
 __host__ __device__ __forceinline__ int lds_word(int w, int pad_shift) { return w + (w >> pad_shift); }
 
+// One count of bin (>= 0) into the LDS histogram, no merging.
+template <bool PACK16>
+__device__ __forceinline__ void lds_add1(unsigned *__restrict__ h, int bin, int pad_shift) {
+    if (PACK16)
+        atomicAdd(&h[lds_word(bin >> 1, pad_shift)], 1u << (16 * (bin & 1)));
+    else
+        atomicAdd(&h[bin], 1u);
+}
+
 template <bool PACK16>
 __device__ __forceinline__ void lds_count(unsigned *__restrict__ h, int bin, int max_peel,
                                           int pad_shift = 31) {
+    if (max_peel <= 0) {
+        // no merging (a uniform branch): each counted lane adds its one,
+        // without the ballot and the pending-mask test
+        if (bin >= 0) lds_add1<PACK16>(h, bin, pad_shift);
+        return;
+    }
     unsigned long long pending = __ballot(bin >= 0);
     const int lane = (int)__lane_id();
     for (int it = 0; it < max_peel && pending != 0ull; ++it) {
@@ -164,12 +179,7 @@ __device__ __forceinline__ void lds_count(unsigned *__restrict__ h, int bin, int
         }
         pending &= ~same;
     }
-    if ((pending >> lane) & 1ull) {
-        if (PACK16)
-            atomicAdd(&h[lds_word(bin >> 1, pad_shift)], 1u << (16 * (bin & 1)));
-        else
-            atomicAdd(&h[bin], 1u);
-    }
+    if ((pending >> lane) & 1ull) lds_add1<PACK16>(h, bin, pad_shift);
 }
 
 // A workgroup's u16-pair row stored transposed for the first pass of the
