@@ -82,6 +82,13 @@ struct CephesD {
     static constexpr double kC5 = 4.16666666666665929218E-2;
 };
 
+// The reference's branch test ((j - 2) & 2) == 0 (sincos.h: the cosine
+// polynomial serves the octants j = 2, 6 mod 8).  j is even there (rounded
+// up to even), and for even j, modulo 2^32 too, bit 1 of j - 2 is bit 1 of
+// j inverted, so the test is (j & 2) != 0: the same branch, one subtraction
+// fewer.
+__device__ __forceinline__ bool cephes_cos_branch(int j) { return (j & 2) != 0; }
+
 // Reduced-argument polynomials: returns sin-branch and cos-branch values of r
 // and the octant index of |x|.
 __device__ __forceinline__ void cephes_reduce(double x, double &sp, double &cp,
@@ -113,7 +120,7 @@ __device__ __forceinline__ double cephes_sin(double x) {
     double sp, cp;
     int j;
     cephes_reduce(x, sp, cp, j);
-    const double out = (((j - 2) & 2) == 0) ? cp : sp;
+    const double out = cephes_cos_branch(j) ? cp : sp;
     // the reference's two negations (octant j & 4, then x < 0) as one flip
     // of the sign bit; x < 0 (not the sign bit of x) keeps sin(-0.0) = +0.0
     const unsigned flip = ((unsigned)j << 29) ^ (x < 0.0 ? 0x80000000u : 0u);
@@ -165,7 +172,7 @@ __device__ __forceinline__ double cephes_sin_tab(double x, LdsConstDoublePtr tab
     const double y = (double)j;
     const double r = ((ax - y * CephesD::kDP1) - y * CephesD::kDP2) - y * CephesD::kDP3;
     const double zz = r * r;
-    const bool cosb = ((j - 2) & 2) == 0;
+    const bool cosb = cephes_cos_branch(j);
     using D2 = double __attribute__((ext_vector_type(2)));
     using LdsD2 = __attribute__((address_space(3))) const D2 *;
     const LdsD2 c = reinterpret_cast<LdsD2>(tab + (cosb ? 6 : 0));
@@ -189,7 +196,7 @@ __device__ __forceinline__ void cephes_sincos(double x, double &s, double &c) {
     double sp, cp;
     int j;
     cephes_reduce(x, sp, cp, j);
-    const bool swap = ((j - 2) & 2) == 0;
+    const bool swap = cephes_cos_branch(j);
     s = swap ? cp : sp;
     c = swap ? sp : cp;
     if (j & 4) s = -s;
@@ -246,7 +253,7 @@ __device__ __forceinline__ float cephes_sinf(float x) {
                       1.6666654611E-1f) * z * r) + r;
     const float c = ((2.443315711809948E-005f * z - 1.388731625493765E-003f) * z +
                      4.166664568298827E-002f) * z * z - 0.5f * z + 1.0f;
-    const float out = (((j - 2) & 2) == 0) ? c : s;
+    const float out = cephes_cos_branch(j) ? c : s;
     // the two sign changes (octant, then x < 0) as one: they cancel when
     // both apply
     const bool neg = ((j & 4) != 0) != (x < 0.0f);
