@@ -209,6 +209,12 @@ class FusedTracker:
         if self._stats:
             self._setup_statistics()
         _lib.call("blmi_engine_set_turn", self._h, int(rp.counter[0]))
+        self._engine_turn = int(rp.counter[0])
+        # state that lives on the device between calls and that turns taken
+        # outside this tracker would not update: the phase loop's, the
+        # multi-turn wake memory's and the bin-ordered particle copies
+        self._device_state = (self.fb is not None or bool(sort_every)
+                              or getattr(self, "_mtw", None) is not None)
         self._primed = False
 
     @property
@@ -228,6 +234,7 @@ class FusedTracker:
         and advances the host state like ``track``."""
         a, b = ct.c_float(), ct.c_float()
         stream = ct.c_void_p(self._torch.cuda.current_stream().cuda_stream)
+        self._sync_turn()
         if self.tiv is not None and not self._primed:
             _lib.call("blmi_engine_prime", self._h, stream)
             self._primed = True
@@ -276,6 +283,7 @@ class FusedTracker:
                 and iv.dtype == dtype and iv.is_contiguous()):
             iv = torch.zeros(p.n_slices, dtype=dtype, device="cuda")
             self.tiv.induced_voltage = iv
+        self._iv = iv
         bc = p.bin_centers
         if not (hasattr(bc, "is_cuda") and bc.dtype == dtype and bc.is_contiguous()):
             bc = torch.as_tensor(np.asarray(_host_array(bc)), device="cuda").to(dtype).contiguous()
@@ -391,12 +399,31 @@ class FusedTracker:
         histogram of the current turn is rebuilt on the next track()."""
         self._primed = False
 
+    def _sync_turn(self):
+        """Turns tracked outside this tracker (the objects' own track()
+        between two calls) moved the RF station's counter: continue from
+        that turn, with the staged histogram rebuilt from the coordinates as
+        they are.  Refused where the engine keeps state of its own that
+        those turns did not update."""
+        t = int(self.rf.counter[0])
+        if t == self._engine_turn:
+            return
+        if self._device_state:
+            raise RuntimeError(
+                f"the RF counter moved from turn {self._engine_turn} to {t} outside the "
+                "fused tracker, whose phase loop, multi-turn wake or particle order lives "
+                "on the device: track every turn with one of them")
+        _lib.call("blmi_engine_set_turn", self._h, t)
+        self._engine_turn = t
+        self._primed = False
+
     def track(self, n_turns: int = 1):
         """Advance ``n_turns`` turns (k passes of the map)."""
         n_turns = int(n_turns)
         if n_turns <= 0:
             return
         stream = ct.c_void_p(self._torch.cuda.current_stream().cuda_stream)
+        self._sync_turn()
         if self.tiv is not None and not self._primed:
             _lib.call("blmi_engine_prime", self._h, stream)
             self._primed = True
@@ -421,6 +448,12 @@ class FusedTracker:
                 rp.phi_rf[:, start + 1:end + 1] = torch.from_numpy(
                     np.ascontiguousarray(self._phi_prog[:, start + 1:end + 1])).to(rp.phi_rf.device)
         rp.counter[0] = end
+        self._engine_turn = end
+        if self.tiv is not None and self.tiv.induced_voltage is not self._iv:
+            # the objects' own induced-voltage step (turns tracked between
+            # two calls) installs a new array; the engine writes the one it
+            # was given
+            self.tiv.induced_voltage = self._iv
         self.beam.beta = rp.beta[end]
         self.beam.gamma = rp.gamma[end]
         self.beam.energy = rp.energy[end]

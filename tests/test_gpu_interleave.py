@@ -1,0 +1,68 @@
+"""Fused turns interleaved with the objects' own turns (the map a BLonD
+main script runs, element by element) on one workload: the fused tracker
+continues from the RF station's counter as the objects left it, with the
+histogram of the current turn rebuilt from the coordinates as they are.
+Plain turns are bit-identical to the oracle's uninterrupted run; intensity
+turns held as the other multi-turn intensity tests hold them.  Where the
+engine keeps state of its own on the device (here the bin-ordered particle
+copies), a turn taken outside it is refused with an error instead."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("no GPU", allow_module_level=True)
+
+from blond_amd.workloads import build_workload  # noqa: E402
+
+from tests.test_gpu_tracking import _check_intensity, host  # noqa: E402
+
+ORC = orc.oracle_lib()
+
+
+def _objects(w, turns):
+    for _ in range(turns):
+        for m in w.map():
+            m.track()
+
+
+@pytest.mark.parametrize("turn_block", [25, 1])
+def test_plain_fused_and_object_turns_bit_exact(turn_block):
+    """c1: 5 fused, 3 by the objects, 4 fused, 1 by the objects, 2 fused."""
+    w = build_workload("c1", n_particles=40_001, n_turns=40)
+    ref = orc.track(w.snapshot(), 15, ORC)
+    w.fused(5, turn_block=turn_block)
+    _objects(w, 3)
+    w.fused(4)
+    _objects(w, 1)
+    w.fused(2)
+    assert w.rf.counter[0] == 15
+    np.testing.assert_array_equal(host(w.beam.dt), ref["dt"])
+    np.testing.assert_array_equal(host(w.beam.dE), ref["dE"])
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
+
+
+@pytest.mark.parametrize("name", ["c2", "c3"])
+def test_intensity_fused_and_object_turns(name):
+    """2 fused, 1 by the objects, 3 fused, 2 by the objects, 2 fused."""
+    w = build_workload(name, n_particles=60_001, n_turns=20)
+    ref = orc.track(w.snapshot(), 10, ORC)
+    w.fused(2)
+    _objects(w, 1)
+    w.fused(3)
+    _objects(w, 2)
+    w.fused(2)
+    assert w.rf.counter[0] == 10
+    _check_intensity(w, ref, 1e-9)
+
+
+def test_turns_outside_an_ordered_engine_refused():
+    w = build_workload("c1", n_particles=20_001, n_turns=20, n_slices=16384)
+    w.fused(2, sort_every=2)
+    _objects(w, 1)
+    with pytest.raises(RuntimeError, match="outside the fused tracker"):
+        w.fused(1)
