@@ -180,6 +180,7 @@ class FusedTracker:
         self._h = ct.c_void_p(h)
         _lib.call("blmi_engine_set_particles_f32" if self.f32 else "blmi_engine_set_particles",
                   self._h, _ptr(self.beam.dt), _ptr(self.beam.dE), n)
+        self._dt, self._dE = self.beam.dt, self.beam.dE
         self._upload_programs()
         if self.profile is not None:
             p = self.profile
@@ -234,6 +235,7 @@ class FusedTracker:
         and advances the host state like ``track``."""
         a, b = ct.c_float(), ct.c_float()
         stream = ct.c_void_p(self._torch.cuda.current_stream().cuda_stream)
+        self._sync_particles()
         self._sync_turn()
         if self.tiv is not None and not self._primed:
             _lib.call("blmi_engine_prime", self._h, stream)
@@ -399,6 +401,28 @@ class FusedTracker:
         histogram of the current turn is rebuilt on the next track()."""
         self._primed = False
 
+    def _sync_particles(self):
+        """beam.dt or beam.dE replaced by other arrays since the last call
+        (an assignment, a copy): track those, with the staged histogram
+        rebuilt.  Their precision is the tracker's, fixed at construction."""
+        dt, dE = self.beam.dt, self.beam.dE
+        if dt is self._dt and dE is self._dE:
+            return
+        if self._device_state:
+            raise RuntimeError("beam.dt or beam.dE was replaced, but the fused tracker's phase "
+                               "loop, multi-turn wake or particle order lives on the device")
+        if not all(hasattr(a, "is_cuda") and a.is_cuda and a.is_contiguous() for a in (dt, dE)):
+            raise TypeError("beam coordinates must be contiguous device tensors")
+        if dt.numel() != self.beam.n_macroparticles or dE.numel() != self.beam.n_macroparticles:
+            raise ValueError("the number of macroparticles is fixed at construction")
+        want = self._torch.float32 if self.f32 else self._torch.float64
+        if dt.dtype != want or dE.dtype != want:
+            raise TypeError(f"beam coordinates must stay {want} for this tracker")
+        _lib.call("blmi_engine_set_particles_f32" if self.f32 else "blmi_engine_set_particles",
+                  self._h, _ptr(dt), _ptr(dE), self.beam.n_macroparticles)
+        self._dt, self._dE = dt, dE
+        self._primed = False
+
     def _sync_turn(self):
         """Turns tracked outside this tracker (the objects' own track()
         between two calls) moved the RF station's counter: continue from
@@ -423,6 +447,7 @@ class FusedTracker:
         if n_turns <= 0:
             return
         stream = ct.c_void_p(self._torch.cuda.current_stream().cuda_stream)
+        self._sync_particles()
         self._sync_turn()
         if self.tiv is not None and not self._primed:
             _lib.call("blmi_engine_prime", self._h, stream)

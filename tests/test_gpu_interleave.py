@@ -5,7 +5,9 @@ histogram of the current turn rebuilt from the coordinates as they are.
 Plain turns are bit-identical to the oracle's uninterrupted run; intensity
 turns held as the other multi-turn intensity tests hold them.  Where the
 engine keeps state of its own on the device (here the bin-ordered particle
-copies), a turn taken outside it is refused with an error instead."""
+copies), a turn taken outside it is refused with an error instead.  New
+arrays assigned to beam.dt and beam.dE between two calls are followed (in
+the tracker's precision; another is refused)."""
 import numpy as np
 import pytest
 
@@ -65,4 +67,34 @@ def test_turns_outside_an_ordered_engine_refused():
     w.fused(2, sort_every=2)
     _objects(w, 1)
     with pytest.raises(RuntimeError, match="outside the fused tracker"):
+        w.fused(1)
+
+
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_replaced_coordinate_arrays_followed(name):
+    """beam.dt and beam.dE assigned new arrays (copies) between two calls:
+    the tracker tracks those, the old ones are left alone."""
+    w = build_workload(name, n_particles=40_001, n_turns=20)
+    ref = orc.track(w.snapshot(), 9, ORC)
+    w.fused(4)
+    old_dt = w.beam.dt
+    kept = host(old_dt).copy()
+    w.beam.dt = w.beam.dt.clone()
+    w.beam.dE = w.beam.dE.clone()
+    w.fused(5)
+    np.testing.assert_array_equal(host(old_dt), kept)
+    if name == "c1":
+        np.testing.assert_array_equal(host(w.beam.dt), ref["dt"])
+        np.testing.assert_array_equal(host(w.beam.dE), ref["dE"])
+        np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
+    else:
+        _check_intensity(w, ref, 1e-9)
+
+
+def test_replaced_coordinates_of_another_precision_refused():
+    w = build_workload("c1", n_particles=20_001, n_turns=20)
+    w.fused(2)
+    w.beam.dt = w.beam.dt.to(torch.float32).contiguous()
+    w.beam.dE = w.beam.dE.to(torch.float32).contiguous()
+    with pytest.raises(TypeError, match="must stay"):
         w.fused(1)
