@@ -128,16 +128,18 @@ class EnsembleTracker:
             Z = np.ascontiguousarray(np.stack(zs))
             scale = np.array([tr.beam.particle.charge * e * tr.beam.ratio for tr in trackers])
             self.volt = torch.zeros((B, ns), dtype=torch.float64, device="cuda")
+            self._volt_rows = [self.volt[b] for b in range(B)]
             for b, t in enumerate(tivs):
                 if t.profile is not profiles[b]:
                     raise ValueError("each total induced voltage must use its bunch's profile")
                 t.to_gpu(recursive=False)
-                t.induced_voltage = self.volt[b]
+                t.induced_voltage = self._volt_rows[b]
             self._keep += [Z, scale]
             _lib.call("blmi_ensemble_set_induced", self._h, int(nffts[0]), _dptr(Z), _dptr(scale),
                       _ptr(self.volt), _ptr(self.bc))
         self.set_launch(block, wg_per_cu, graph_chunk, max_peel)
         _lib.call("blmi_ensemble_set_turn", self._h, int(rp0.counter[0]))
+        self._engine_turn = int(rp0.counter[0])
         self._primed = False
 
     @property
@@ -156,7 +158,28 @@ class EnsembleTracker:
     def _stream(self):
         return ct.c_void_p(self._torch.cuda.current_stream().cuda_stream)
 
+    def _sync(self):
+        """Turns tracked by the bunches' own objects between two calls
+        (their coordinates are views of the ensemble's arrays, so those
+        turns moved them in place): continue from the bunches' common turn
+        with the staged histograms rebuilt."""
+        turns = {int(tr.rf_params.counter[0]) for tr in self.trackers}
+        if len(turns) != 1:
+            raise RuntimeError(f"the bunches' RF counters differ ({sorted(turns)}): the "
+                               "ensemble tracks all of them through the same turns")
+        for tr, o in zip(self.trackers, self.offsets):
+            if (tr.beam.dt.data_ptr() != self.dt[int(o):].data_ptr()
+                    or tr.beam.dE.data_ptr() != self.dE[int(o):].data_ptr()):
+                raise RuntimeError("a bunch's beam.dt or beam.dE is no longer a view of the "
+                                   "ensemble's arrays")
+        t = turns.pop()
+        if t != self._engine_turn:
+            _lib.call("blmi_ensemble_set_turn", self._h, t)
+            self._engine_turn = t
+            self._primed = False
+
     def _prime(self, stream):
+        self._sync()
         if self.tivs is not None and not self._primed:
             _lib.call("blmi_ensemble_prime", self._h, stream)
             self._primed = True
@@ -193,6 +216,13 @@ class EnsembleTracker:
         return a.value, b.value
 
     def _advance(self, start: int, end: int):
+        self._engine_turn = end
+        if self.tivs is not None:
+            # the objects' own induced-voltage step (turns tracked between
+            # two calls) installs new arrays; the engine writes these rows
+            for b, t in enumerate(self.tivs):
+                if t.induced_voltage is not self._volt_rows[b]:
+                    t.induced_voltage = self._volt_rows[b]
         for tr, (phi, dphi, moves) in zip(self.trackers, self._phase):
             rp, beam = tr.rf_params, tr.beam
             if end > start and moves:

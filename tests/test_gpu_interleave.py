@@ -7,7 +7,9 @@ turns held as the other multi-turn intensity tests hold them.  Where the
 engine keeps state of its own on the device (here the bin-ordered particle
 copies), a turn taken outside it is refused with an error instead.  New
 arrays assigned to beam.dt and beam.dE between two calls are followed (in
-the tracker's precision; another is refused)."""
+the tracker's precision; another is refused).  The batched ensemble the
+same: its bunches' own turns in between are picked up, bunches whose
+counters diverged are refused."""
 import numpy as np
 import pytest
 
@@ -98,3 +100,61 @@ def test_replaced_coordinates_of_another_precision_refused():
     w.beam.dE = w.beam.dE.to(torch.float32).contiguous()
     with pytest.raises(TypeError, match="must stay"):
         w.fused(1)
+
+
+def test_ensemble_and_object_turns_plain_bit_exact():
+    """Batched turns, then each bunch's own tracker and profile for 2
+    turns (their coordinates are views of the ensemble's arrays), then
+    batched turns again: every bunch equals the oracle's 9 turns."""
+    from blond_amd.trackers.ensemble import EnsembleTracker
+    from blond_amd.workloads import build_ensemble
+
+    ens = build_ensemble(n_bunches=6, n_turns=20, n_slices=100, n_min=50, n_max=3000, seed=3)
+    snaps = [ens.bunch(k).snapshot() for k in range(6)]
+    for s in snaps:
+        s["intensity"] = False
+    et = EnsembleTracker(ens.trackers, ens.profiles, None, graph_chunk=5)
+    et.track(3)
+    for k in range(6):
+        for _ in range(2):
+            ens.trackers[k].track()
+            ens.profiles[k].track()
+    et.track(4)
+    for k in range(6):
+        ref = orc.track(snaps[k], 9)
+        np.testing.assert_array_equal(host(ens.trackers[k].beam.dt), ref["dt"])
+        np.testing.assert_array_equal(host(ens.trackers[k].beam.dE), ref["dE"])
+        np.testing.assert_array_equal(host(ens.profiles[k].n_macroparticles), ref["profile"])
+    et.close()
+
+
+def test_ensemble_and_object_turns_intensity():
+    """The same with every bunch's induced voltage: 2 batched, 1 turn of
+    each bunch's map [profile, induced voltage, tracker], 2 batched."""
+    from blond_amd.workloads import build_ensemble
+
+    ens = build_ensemble(n_bunches=4, n_turns=20, n_slices=256, n_min=20_000, n_max=40_000,
+                         seed=4)
+    snaps = [ens.bunch(k).snapshot() for k in range(4)]
+    et = ens.tracker()
+    et.track(2)
+    for k in range(4):
+        for m in ens.bunch(k).map():
+            m.track()
+    et.track(2)
+    for k in range(4):
+        _check_intensity(ens.bunch(k), orc.track(snaps[k], 5), 1e-9)
+    et.close()
+
+
+def test_ensemble_with_diverged_counters_refused():
+    from blond_amd.trackers.ensemble import EnsembleTracker
+    from blond_amd.workloads import build_ensemble
+
+    ens = build_ensemble(n_bunches=3, n_turns=20, n_slices=100, n_min=50, n_max=500, seed=2)
+    et = EnsembleTracker(ens.trackers, ens.profiles, None, graph_chunk=5)
+    et.track(1)
+    ens.trackers[0].track()
+    with pytest.raises(RuntimeError, match="counters differ"):
+        et.track(1)
+    et.close()
