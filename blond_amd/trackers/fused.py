@@ -189,6 +189,7 @@ class FusedTracker:
             _lib.call("blmi_engine_set_slices_f32" if self.f32 else "blmi_engine_set_slices",
                       self._h, p.n_slices, float(p.cut_left), float(p.cut_right),
                       _ptr(p.n_macroparticles))
+        self._slicing = self._profile_slicing()
         if self.tiv is not None:
             self._setup_induced()
         if self.fb is not None:
@@ -235,6 +236,7 @@ class FusedTracker:
         and advances the host state like ``track``."""
         a, b = ct.c_float(), ct.c_float()
         stream = ct.c_void_p(self._torch.cuda.current_stream().cuda_stream)
+        self._check_slicing()
         self._sync_particles()
         self._sync_turn()
         if self.tiv is not None and not self._primed:
@@ -401,6 +403,18 @@ class FusedTracker:
         histogram of the current turn is rebuilt on the next track()."""
         self._primed = False
 
+    def _profile_slicing(self):
+        p = self.profile
+        return None if p is None else (int(p.n_slices), float(p.cut_left), float(p.cut_right))
+
+    def _check_slicing(self):
+        """The cuts and slice count are the engine's from construction (the
+        impedance on the bins and the interpolation table depend on them)."""
+        if self._profile_slicing() != self._slicing:
+            raise RuntimeError(f"the profile's slicing changed from {self._slicing} to "
+                               f"{self._profile_slicing()} since the fused tracker was built: "
+                               "build a new FusedTracker for the new slicing")
+
     def _sync_particles(self):
         """beam.dt or beam.dE replaced by other arrays since the last call
         (an assignment, a copy): track those, with the staged histogram
@@ -447,6 +461,7 @@ class FusedTracker:
         if n_turns <= 0:
             return
         stream = ct.c_void_p(self._torch.cuda.current_stream().cuda_stream)
+        self._check_slicing()
         self._sync_particles()
         self._sync_turn()
         if self.tiv is not None and not self._primed:

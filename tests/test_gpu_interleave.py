@@ -158,3 +158,13 @@ def test_ensemble_with_diverged_counters_refused():
     with pytest.raises(RuntimeError, match="counters differ"):
         et.track(1)
     et.close()
+
+
+def test_resliced_profile_refused():
+    """Slicing is fixed at construction (the impedance on the bins and the
+    interpolation table depend on it): new cuts are refused, not ignored."""
+    w = build_workload("c1", n_particles=20_001, n_turns=20)
+    w.fused(1)
+    w.profile.cut_left = w.profile.cut_left - 1e-10
+    with pytest.raises(RuntimeError, match="slicing changed"):
+        w.fused(1)
