@@ -403,6 +403,22 @@ class FusedTracker:
         histogram of the current turn is rebuilt on the next track()."""
         self._primed = False
 
+    def update_programs(self):
+        """Re-read the RF station's per-turn programs (voltage, omega_rf,
+        phi_rf; on the GPU the device arrays, which the objects' own turns
+        read too) and the ring's, after the caller changed them; the next
+        track() uses them from the current turn on.  With a phase loop the
+        device loop owns the phase program: refused."""
+        if self.fb is not None:
+            raise RuntimeError("with a beam phase loop the phase program lives on the device")
+        rp = self.rf
+        # nothing still in flight may read the tables being replaced
+        self._torch.cuda.synchronize()
+        if rp._device == "GPU":
+            for name in ("voltage", "omega_rf", "phi_rf"):
+                rp._host[name][...] = getattr(rp, name).cpu().numpy()
+        self._upload_programs()
+
     def _profile_slicing(self):
         p = self.profile
         return None if p is None else (int(p.n_slices), float(p.cut_left), float(p.cut_right))

@@ -168,3 +168,21 @@ def test_resliced_profile_refused():
     w.profile.cut_left = w.profile.cut_left - 1e-10
     with pytest.raises(RuntimeError, match="slicing changed"):
         w.fused(1)
+
+
+def test_updated_rf_programs_followed():
+    """The RF voltage of every turn from the current one on raised by 10 %
+    (on the device arrays, which the objects' own turns read) between two
+    calls: after update_programs() the fused turns equal the oracle's run
+    of the new program."""
+    w = build_workload("c1", n_particles=30_001, n_turns=30)
+    ft = w.fused(3)
+    w.rf.voltage[:, 3:] *= 1.1
+    ft.update_programs()
+    snap = w.snapshot()
+    assert snap["voltage"][0, 3] == pytest.approx(1.1 * snap["voltage"][0, 2])
+    ref = orc.track(snap, 6, ORC)
+    w.fused(6)
+    np.testing.assert_array_equal(host(w.beam.dt), ref["dt"])
+    np.testing.assert_array_equal(host(w.beam.dE), ref["dE"])
+    np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
