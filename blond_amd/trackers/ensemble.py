@@ -107,16 +107,8 @@ class EnsembleTracker:
         self._h = ct.c_void_p(h)
         _lib.call("blmi_ensemble_set_particles", self._h, _ptr(self.dt), _ptr(self.dE),
                   _dptr(offsets), _dptr(counts))
-        # programs
-        self._phase = [phase_program(tr.rf_params) for tr in trackers]
-        rf = np.ascontiguousarray(np.stack([rf_program_array(tr.rf_params, ph[0])
-                                            for tr, ph in zip(trackers, self._phase)]))
-        acc = np.zeros(n_turns + 1, dtype=np.float64)
-        acc[: len(trackers[0].acceleration_kick)] = trackers[0].acceleration_kick
-        self._keep = [rf, acc, ring0]
-        _lib.call("blmi_ensemble_set_programs", self._h, _dptr(rf), _dptr(acc), _dptr(ring0[0]),
-                  float(rp0.length_ratio), float(rp0.alpha_order), *[_dptr(a) for a in ring0[1:]],
-                  float(rp0.particle.charge))
+        self._keep = []
+        self._upload_programs()
         cl = np.array([float(p.cut_left) for p in profiles])
         cr = np.array([float(p.cut_right) for p in profiles])
         self._keep += [cl, cr]
@@ -157,6 +149,34 @@ class EnsembleTracker:
 
     def _stream(self):
         return ct.c_void_p(self._torch.cuda.current_stream().cuda_stream)
+
+    def _upload_programs(self):
+        """Every bunch's RF programs (its phase offsets accumulated from the
+        current turn on) and the shared ring programs to the engine."""
+        trackers = self.trackers
+        rp0 = trackers[0].rf_params
+        ring0 = ring_program_arrays(rp0)
+        self._phase = [phase_program(tr.rf_params) for tr in trackers]
+        rf = np.ascontiguousarray(np.stack([rf_program_array(tr.rf_params, ph[0])
+                                            for tr, ph in zip(trackers, self._phase)]))
+        acc = np.zeros(rp0.n_turns + 1, dtype=np.float64)
+        acc[: len(trackers[0].acceleration_kick)] = trackers[0].acceleration_kick
+        self._programs = [rf, acc, ring0]
+        _lib.call("blmi_ensemble_set_programs", self._h, _dptr(rf), _dptr(acc), _dptr(ring0[0]),
+                  float(rp0.length_ratio), float(rp0.alpha_order), *[_dptr(a) for a in ring0[1:]],
+                  float(rp0.particle.charge))
+
+    def update_programs(self):
+        """Re-read the bunches' per-turn RF programs (on the GPU their
+        device arrays, which the objects' own turns read) after the caller
+        changed them; the next track() uses them from the current turn on."""
+        self._torch.cuda.synchronize()  # nothing in flight reads the old tables
+        for tr in self.trackers:
+            rp = tr.rf_params
+            if rp._device == "GPU":
+                for name in ("voltage", "omega_rf", "phi_rf"):
+                    rp._host[name][...] = getattr(rp, name).cpu().numpy()
+        self._upload_programs()
 
     def _sync(self):
         """Turns tracked by the bunches' own objects between two calls

@@ -186,3 +186,24 @@ def test_updated_rf_programs_followed():
     np.testing.assert_array_equal(host(w.beam.dt), ref["dt"])
     np.testing.assert_array_equal(host(w.beam.dE), ref["dE"])
     np.testing.assert_array_equal(host(w.profile.n_macroparticles), ref["profile"])
+
+
+def test_ensemble_updated_rf_programs_followed():
+    from blond_amd.trackers.ensemble import EnsembleTracker
+    from blond_amd.workloads import build_ensemble
+
+    ens = build_ensemble(n_bunches=5, n_turns=20, n_slices=100, n_min=50, n_max=2000, seed=6)
+    et = EnsembleTracker(ens.trackers, ens.profiles, None, graph_chunk=5)
+    et.track(3)
+    for tr in ens.trackers:
+        tr.rf_params.voltage[:, 3:] *= 0.9
+    et.update_programs()
+    snaps = [ens.bunch(k).snapshot() for k in range(5)]
+    for s in snaps:
+        s["intensity"] = False
+    et.track(4)
+    for k in range(5):
+        ref = orc.track(snaps[k], 4)
+        np.testing.assert_array_equal(host(ens.trackers[k].beam.dt), ref["dt"])
+        np.testing.assert_array_equal(host(ens.trackers[k].beam.dE), ref["dE"])
+    et.close()
